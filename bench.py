@@ -1,0 +1,295 @@
+"""Benchmark of the OpenStaPLE fermion hot path on one MI355X.
+
+Workload (32^4, the lattice the headline metric is quoted on): random SU(3)
+links with eps = 0.5 (generate_Conf_cold rule, seed 1), staggered phases with
+antiperiodic time, Gaussian source.  A timed step is one application of the
+FP64 staggered Dslash pair (D_oe then D_eo, the two hops of M = m^2 - D_eo D_oe)
+on HBM-resident fields; `value` is the Dslash bandwidth in GB/s under the
+928 B / output half-site algorithmic model (8 links x 2 stored rows x 3 complex
+x 16 B + 8 phases x 8 B + 48 B in + 48 B out).
+
+After the timed steps the same fields are solved to ||r||/||b|| < 1e-10 (FP64
+true residual) with (a) the rational multi-shift CG-M over the 19 shifts of the
+order-19 x^(-1/4) approximation rescaled to [m^2, 64], m = 0.01, in FP64, and
+(b) the FP32-inner variant (FP32 CG-M + per-shift FP32 CG with FP64 reliable
+updates), plus (c) the plain FP64 CG at m = 0.01; their wall times are reported
+in "solves".  A bounded sample of the same Dslash is timed on the host with the
+CPU oracle (cpu_baseline).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solves] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Dslash GB/s; s per 32^4 solve to 1e-10 rel. residual, FP64 vs mixed/learned"
+BYTES_PER_SITE_F64 = 928
+BYTES_PER_SITE_F32 = 464
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--lattice", type=str, default="32,32,32,32")
+    ap.add_argument("--no-solves", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
+                                   "(t-slab halo exchange, see DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import openstaple_amd as ost
+    from openstaple_amd import lib
+
+    shape = tuple(int(x) for x in args.lattice.split(","))
+    L = ost.Lattice(*shape)
+    L.init()
+    sizeh = L.sizeh
+    vb, gb, pb = L.vec3_bytes, 8 * L.su3_bytes, 8 * 8 * sizeh
+
+    def dmalloc(n):
+        p = lib.ostp_malloc(n)
+        if not p:
+            raise MemoryError(lib.ostp_last_error())
+        lib.ostp_memset(p, 0, n)
+        return p
+
+    u = dmalloc(gb)
+    ph = dmalloc(pb)
+    b = dmalloc(vb)
+    h = dmalloc(vb)
+    out = dmalloc(vb)
+    lib.ostp_generate_Conf_cold(u, 0.5, 1)
+    lib.calc_u1_phases(ph, ost.BfParam(0, 0, 0, 0, 0, 0), 0.0, 0.0)
+    lib.ostp_generate_vec3_soa_gauss(b, 1, 0)
+    ost.check(0, "setup")
+
+    # ---------------- timed region: K Dslash pairs (Doe then Deo)
+    for _ in range(args.warmup):
+        lib.acc_Doe(u, h, b, ph)
+        lib.acc_Deo(u, out, h, ph)
+    lib.ostp_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lib.acc_Doe(u, h, b, ph)
+        lib.acc_Deo(u, out, h, ph)
+    lib.ostp_synchronize()
+    t1 = time.perf_counter()
+    ost.check(0, "dslash loop")
+    ms_per_step = 1000.0 * (t1 - t0) / args.steps
+    alg_bytes_step = 2 * BYTES_PER_SITE_F64 * sizeh
+    value = alg_bytes_step / (ms_per_step * 1e-3) / 1e9
+
+    # per-launch device time of the dominant kernel, HIP events on the library stream
+    reps = max(50, args.steps)
+    us_deo = lib.ostp_time_dslash(u, out, h, ph, 0, 0, reps)
+    us_doe = lib.ostp_time_dslash(u, h, b, ph, 1, 0, reps)
+    us_launch = 0.5 * (us_deo + us_doe)
+    achieved = BYTES_PER_SITE_F64 * sizeh / (us_launch * 1e-6) / 1e9
+    # plain copy of the gauge field's size: what the box's HBM delivers
+    cbytes = 512 * 1024 * 1024
+    ca, cb = dmalloc(cbytes), dmalloc(cbytes)
+    us_copy = lib.ostp_time_copy(cb, ca, cbytes, 20)
+    copy_gbs = 2 * cbytes / (us_copy * 1e-6) / 1e9
+    lib.ostp_free(ca)
+    lib.ostp_free(cb)
+
+    # FP32 Dslash
+    uf = dmalloc(8 * L.su3f_bytes)
+    phf = dmalloc(4 * 8 * sizeh)
+    bf = dmalloc(vb // 2)
+    hf = dmalloc(vb // 2)
+    lib.ostp_convert_double_to_float_gauge(u, uf)
+    lib.ostp_phases_to_float(ph, phf)
+    lib.convert_double_to_float_vec3_soa(b, bf)
+    us_f32 = 0.5 * (lib.ostp_time_dslash(uf, hf, bf, phf, 1, 1, reps) + lib.ostp_time_dslash(uf, bf, hf, phf, 0, 1, reps))
+    f32_gbs = BYTES_PER_SITE_F32 * sizeh / (us_f32 * 1e-6) / 1e9
+    ost.check(0, "timing")
+
+    traffic = None
+    prof = ROOT / "profiles" / f"dslash_fp64_{'x'.join(map(str, shape))}_traffic.json"
+    if prof.exists():
+        try:
+            traffic = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{'x'.join(map(str, shape))} staggered Dslash pair (Doe+Deo) FP64 on HBM-resident "
+                        "fields; random SU(3) eps=0.5 seed 1; then 19-shift rational CG-M and plain CG to 1e-10",
+            "lattice": list(shape),
+            "sizeh": sizeh,
+            "bytes_per_half_site": BYTES_PER_SITE_F64,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "k_dslash<double> (Deo/Doe mean)",
+            "us_per_launch": round(us_launch, 3),
+            "copy_kernel_GBs": round(copy_gbs, 1),
+            "frac_of_copy": round(achieved / copy_gbs, 4),
+            "fp32_dslash_GBs": round(f32_gbs, 1),
+            "fp32_us_per_launch": round(us_f32, 3),
+        },
+    }
+
+    if not args.no_solves:
+        result["solves"] = run_solves(ost, lib, L, u, ph, b, uf, phf)
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(shape, args.cpu_seconds)
+
+    print(json.dumps(result))
+
+
+def run_solves(ost, lib, L, u, ph, b, uf, phf):
+    """Wall time of each solve to ||b - M x|| / ||b|| < 1e-10 (FP64 true residual)."""
+    sizeh = L.sizeh
+    vb = L.vec3_bytes
+    mass = 0.01
+    ra = ost.read_remez(str(ROOT / "tests" / "golden" / "approx_-1_over_4_order_19_mloglm_6.4.REMEZ"))
+    lmax = 64.0
+    eps = lmax ** (ra.exponent_num / ra.exponent_den)
+    ra.RA_a0 *= eps
+    for k in range(ra.approx_order):
+        ra.RA_a[k] *= lmax * eps
+        ra.RA_b[k] *= lmax
+    order = ra.approx_order
+    fp = ost.ferm_param(mass, ph, phf)
+
+    def dm(n):
+        p = lib.ostp_malloc(n)
+        lib.ostp_memset(p, 0, n)
+        return p
+
+    d64 = [dm(vb) for _ in range(4)]
+    d32 = [dm(vb // 2) for _ in range(5)]
+    out = dm(order * vb)
+    sf = dm(order * vb)
+    sf32 = dm(order * vb // 2)
+    x = dm(vb)
+    res = {}
+
+    # (a) FP64 CG-M
+    cg = C.c_int(0)
+    sres = (C.c_double * 25)()
+    lib.ostp_synchronize()
+    t0 = time.perf_counter()
+    st = lib.ostp_multishift_solve(u, C.byref(fp), C.byref(ra), out, b, 1e-10, *d64, sf, 20000, 0, C.byref(cg), sres)
+    lib.ostp_synchronize()
+    t1 = time.perf_counter()
+    res["multishift_fp64"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "status": st,
+                              "max_shift_true_residual": max(sres[k] for k in range(order)), "shifts": order}
+
+    # (b) FP32-inner: FP32 CG-M then per-shift FP32 CG with FP64 reliable updates
+    ip = ost.InverterPackage()
+    ip.u, ip.u_f = u, uf
+    ip.ferm_shift_temp, ip.ferm_shift_temp_f, ip.nshifts = sf, sf32, order
+    ip.loc_r, ip.loc_h, ip.loc_s, ip.loc_p = d64
+    ip.loc_r_f, ip.loc_h_f, ip.loc_s_f, ip.loc_p_f, ip.out_f = d32
+    lib.ostp_set_inverter_tricks(1, 1, 0.1, 10000)
+    lib.ostp_synchronize()
+    t0 = time.perf_counter()
+    it = lib.inverter_multishift_wrapper(ip, C.byref(fp), C.byref(ra), out, b, 1e-10, 20000, 0)
+    lib.ostp_synchronize()
+    t1 = time.perf_counter()
+    lib.ostp_set_inverter_tricks(0, 0, 0.1, 10000)
+    # true residual of every shift, FP64
+    worst = 0.0
+    s_tmp, h_tmp = d64[2], d64[1]
+    bn = lib.l2norm2_global(b)
+    for k in range(order):
+        xk = out + k * vb
+        lib.fermion_matrix_multiplication_shifted(u, s_tmp, xk, h_tmp, C.byref(fp), ra.RA_b[k])
+        lib.combine_in1_minus_in2(b, s_tmp, h_tmp)
+        worst = max(worst, (lib.l2norm2_global(h_tmp) / bn) ** 0.5)
+    res["multishift_mixed"] = {"seconds": round(t1 - t0, 4), "wrapper_iterations": it,
+                               "max_shift_true_residual": worst}
+    res["multishift_speedup_mixed_vs_fp64"] = round(res["multishift_fp64"]["seconds"] / (t1 - t0), 3)
+
+    # (c) plain FP64 CG and mixed CG at the same mass
+    tr = C.c_double(0)
+    lib.ostp_memset(x, 0, vb)
+    lib.ostp_synchronize()
+    t0 = time.perf_counter()
+    st = lib.ostp_cg_solve(u, C.byref(fp), x, b, 1e-10, *d64, 20000, 0.0, 10000, C.byref(cg), C.byref(tr))
+    lib.ostp_synchronize()
+    t1 = time.perf_counter()
+    res["cg_fp64"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "true_residual": tr.value,
+                      "status": st, "mass": mass}
+    ru = C.c_int(0)
+    lib.ostp_memset(x, 0, vb)
+    lib.ostp_synchronize()
+    t0 = time.perf_counter()
+    st = lib.ostp_mixed_solve(ip, C.byref(fp), x, b, 1e-10, 20000, 0.0, 0.1, C.byref(cg), C.byref(ru), C.byref(tr))
+    lib.ostp_synchronize()
+    t1 = time.perf_counter()
+    res["cg_mixed"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "reliable_updates": ru.value,
+                       "true_residual": tr.value, "status": st}
+    res["cg_speedup_mixed_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
+    ost.check(0, "solves")
+    for p in [*d64, *d32, out, sf, sf32, x]:
+        lib.ostp_free(p)
+    return res
+
+
+def cpu_baseline(shape, seconds):
+    """The CPU oracle's Dslash (a scalar restatement of the reference loop, one
+    core) on the same lattice: time Doe+Deo pairs for about `seconds`."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle import Oracle
+
+    o = Oracle(*shape)
+    u = o.gen_cold(0.5, 1)
+    ph = o.phases()
+    psi = o.gen_gauss(2)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        h = o.doe(u, psi, ph)
+        o.deo(u, h, ph)
+        n += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    gbs = 2 * n * BYTES_PER_SITE_F64 * o.sizeh / dt / 1e9
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"{n} Doe+Deo pairs at {'x'.join(map(str, shape))} in {dt:.1f} s (gcc -O2 serial oracle)"}
+
+
+if __name__ == "__main__":
+    main()

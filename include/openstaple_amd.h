@@ -1,0 +1,210 @@
+/*
+ * openstaple_amd.h -- C ABI of the MI355X-native OpenStaPLE fermion hot path.
+ *
+ * Drop-in boundary: every function below that carries a reference name keeps
+ * the reference's signature (with `__restrict` dropped) and semantics, and
+ * cites the declaration it replaces.  Field arguments are pointers to the
+ * reference's structure-of-arrays types (OpenAcc/struct_c_def.h:16-58) laid out
+ * exactly as the reference lays them out, with the lattice extents supplied at
+ * run time through ostp_init() instead of compile-time macros.
+ *
+ * Pointer arguments may be either
+ *   - host addresses registered with ostp_acc_create()/ostp_acc_copyin()
+ *     (the replacement of `#pragma acc enter data create/copyin`); the library
+ *     works on the device mirror, like the reference's `present(...)` kernels,
+ *     and host copies change only through ostp_acc_update_self(); or
+ *   - device addresses (ostp_malloc(), hipMalloc(), a torch tensor's data_ptr).
+ * Any other address is an error (the reference's OpenACC runtime aborts on a
+ * missing `present` datum; see ostp_set_abort_on_error()).
+ *
+ * All entry points are synchronous at return, like the reference's
+ * non-async `acc kernels` regions.  Not re-entrant (the reference is not
+ * either: it uses globals).
+ */
+#ifndef OPENSTAPLE_AMD_H
+#define OPENSTAPLE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference types (complete definitions: openstaple_amd_types.h) ----- */
+typedef struct vec3_soa_t vec3_soa;         /* struct_c_def.h:16-20 */
+typedef struct su3_soa_t su3_soa;           /* struct_c_def.h:35-42 */
+typedef struct double_soa_t double_soa;     /* struct_c_def.h:25-27 */
+typedef struct tamat_soa_t tamat_soa;       /* struct_c_def.h:45-51 */
+typedef struct thmat_soa_t thmat_soa;       /* struct_c_def.h:52-58 */
+typedef struct vec3_soa_f_t vec3_soa_f;     /* generated sp_struct_c_def.h twin of vec3_soa */
+typedef struct su3_soa_f_t su3_soa_f;       /* generated twin of su3_soa */
+typedef struct float_soa_t float_soa;       /* generated twin of double_soa */
+typedef struct ferm_param_t ferm_param;     /* Include/fermion_parameters.h:9-41 */
+typedef struct RationalApprox_t RationalApprox; /* RationalApprox/rationalapprox.h:15-26 */
+typedef struct inverter_package_t inverter_package; /* OpenAcc/inverter_package.h:12-29 */
+typedef struct bf_param_t bf_param;         /* OpenAcc/backfield_parameters.h:5-9 */
+typedef struct inv_tricks_t inv_tricks;     /* Include/inverter_tricks.h:4-11 */
+
+/* Reference globals the replaced code reads or writes.  The library defines
+ * them with the reference defaults; a host program that defines them itself
+ * (the reference's setting_file_parser.c / inverter_wrappers.c) takes
+ * precedence by ordinary symbol interposition. */
+extern int verbosity_lv;                    /* Include/setting_file_parser.c */
+extern int multishift_invert_iterations;    /* inverter_wrappers.h:14 */
+
+#define INVERTER_SUCCESS 1                  /* inverter_full.h:12-13 */
+#define INVERTER_FAILURE 0
+#define CONVERGENCE_CRITICAL 1              /* inverter_wrappers.h:10-11 */
+#define CONVERGENCE_NONCRITICAL 0
+
+/* ======================================================== runtime (new) === */
+/* Lattice extents nd0..nd3 (d0 fastest, nd0 even) and whether su3_soa holds
+ * the PAR_TEMP K plane.  Replaces the compile-time LOC_N0..3/NREPLICAS
+ * (Mpi/geometry_multidev.h:17-27).  Returns 0 on success. */
+int  ostp_init(int nd0, int nd1, int nd2, int nd3, int par_temp);
+void ostp_finalize(void);
+/* physical->logical direction map (geom_parameters xmap..tmap, geometry.h:33-37) */
+int  ostp_set_direction_map(int xmap, int ymap, int zmap, int tmap);
+/* sizeof checks for a host compiled against the reference headers */
+int  ostp_check_sizes(size_t sizeof_vec3_soa, size_t sizeof_su3_soa, size_t sizeof_double_soa);
+long ostp_sizeh(void);
+
+/* OpenACC data-directive replacements (acc_create/acc_copyin/acc_update_*) */
+int   ostp_acc_create(void *host, size_t bytes);
+int   ostp_acc_copyin(void *host, size_t bytes);
+int   ostp_acc_update_device(const void *host, size_t bytes);
+int   ostp_acc_update_self(void *host, size_t bytes);
+int   ostp_acc_delete(void *host);
+void *ostp_acc_deviceptr(const void *host);
+int   ostp_acc_is_present(const void *host, size_t bytes);
+
+/* plain device memory */
+void *ostp_malloc(size_t bytes);
+void  ostp_free(void *dev);
+int   ostp_memcpy_htod(void *dev, const void *host, size_t bytes);
+int   ostp_memcpy_dtoh(void *host, const void *dev, size_t bytes);
+int   ostp_memset(void *dev, int value, size_t bytes);
+int   ostp_synchronize(void);
+
+/* errors: by default a bad pointer or a HIP failure prints a message and aborts
+ * (as the OpenACC runtime does for a missing `present` datum); with 0 the call
+ * returns (INVERTER_FAILURE for solvers) and the message is kept here. */
+void        ostp_set_abort_on_error(int abort_on_error);
+const char *ostp_last_error(void);
+void        ostp_clear_error(void);
+
+/* inverter_tricks (Include/inverter_tricks.h:4-11) as seen by the solvers */
+void ostp_set_inverter_tricks(int singlePInvAccelMultiInv, int useMixedPrecision,
+                              double mixedPrecisionDelta, int restartingEvery);
+
+/* ======================================================== geometry ======= */
+/* compute_nnp_and_nnm_openacc (OpenAcc/geometry.c:14-100): fills host arrays
+ * laid out [sizeh][4][2] like nnp_openacc/nnm_openacc (geometry.h:59-60). */
+int ostp_compute_nnp_and_nnm(int *nnp, int *nnm);
+
+/* calc_u1_phases (OpenAcc/backfield.c:180-187): staggered phases, e.m. field,
+ * imaginary chemical potential and antiperiodic time, rebounded, times 2 pi.
+ * Computed on the device, bit-identical to the reference's double arithmetic. */
+void calc_u1_phases(double_soa *phases, bf_param bf_pars, double im_chem_pot, double ferm_charge);
+/* convert_double_to_float_real_soa on all 8 phase arrays (float_double_conv.h:31-34) */
+void ostp_phases_to_float(const double_soa *phases, float_soa *phases_f);
+
+/* ======================================================== Dslash ========= */
+/* fermion_matrix.h:20-28 */
+void acc_Deo(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void acc_Doe(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+/* fermion_matrix.h:95-106 */
+void fermion_matrix_multiplication(const su3_soa *u, vec3_soa *out, const vec3_soa *in,
+                                   vec3_soa *temp1, ferm_param *pars);
+void fermion_matrix_multiplication_shifted(const su3_soa *u, vec3_soa *out, const vec3_soa *in,
+                                           vec3_soa *temp1, ferm_param *pars, double shift);
+/* generated FP32 twins (sp_fermion_matrix.h: names + "_f", float data) */
+void acc_Deo_f(const su3_soa_f *u, vec3_soa_f *out, const vec3_soa_f *in, const float_soa *backfield);
+void acc_Doe_f(const su3_soa_f *u, vec3_soa_f *out, const vec3_soa_f *in, const float_soa *backfield);
+void fermion_matrix_multiplication_shifted_f(const su3_soa_f *u, vec3_soa_f *out, const vec3_soa_f *in,
+                                             vec3_soa_f *temp1, ferm_param *pars, double shift);
+
+/* ======================================================== BLAS =========== */
+/* fermionic_utilities.h:45-118 (single rank) */
+double l2norm2_global(const vec3_soa *in_vect1);
+double real_scal_prod_global(const vec3_soa *in_vect1, const vec3_soa *in_vect2);
+/* scal_prod_global returns d_complex; split into re/im for a plain C ABI */
+void   ostp_scal_prod_global(const vec3_soa *in_vect1, const vec3_soa *in_vect2, double *re, double *im);
+void combine_in1xfactor_plus_in2(const vec3_soa *in_vect1, const double factor,
+                                 const vec3_soa *in_vect2, vec3_soa *out);
+void combine_add_factor_x_in2_to_in1(vec3_soa *in1, const vec3_soa *in2, double factor);
+void combine_in1xferm_mass_minus_in2(const vec3_soa *in_vect1, double ferm_mass2, vec3_soa *in_vect2);
+void combine_in1_minus_in2(const vec3_soa *in_vect1, const vec3_soa *in_vect2, vec3_soa *out);
+void assign_in_to_out(const vec3_soa *in_vect1, vec3_soa *out);
+void set_vec3_soa_to_zero(vec3_soa *fermion);
+/* float_double_conv.h:14-45 */
+void convert_double_to_float_vec3_soa(const vec3_soa *d_var, vec3_soa_f *f_var);
+void convert_float_to_double_vec3_soa(const vec3_soa_f *f_var, vec3_soa *d_var);
+/* converts all 8 su3_soa of a gauge field (the reference call sites pass conf[0..7]) */
+void ostp_convert_double_to_float_gauge(const su3_soa *d_var, su3_soa_f *f_var);
+
+/* ======================================================== solvers ======== */
+/* inverter_full.h:20-31 */
+int ker_invert_openacc(const su3_soa *u, ferm_param *pars, vec3_soa *solution, const vec3_soa *in,
+                       double res, vec3_soa *loc_r, vec3_soa *loc_h, vec3_soa *loc_s, vec3_soa *loc_p,
+                       const int max_cg, double shift, int *cg_return);
+/* inverter_mixedp.h:15-22 */
+int inverter_mixed_precision(inverter_package ip, ferm_param *pars, vec3_soa *solution,
+                             const vec3_soa *in, double res, const int max_cg, double shift,
+                             int *cg_return);
+/* inverter_multishift_full.h:15-27 */
+int multishift_invert(const su3_soa *u, ferm_param *pars, RationalApprox *approx, vec3_soa *out,
+                      const vec3_soa *in, double residuo, vec3_soa *loc_r, vec3_soa *loc_h,
+                      vec3_soa *loc_s, vec3_soa *loc_p, vec3_soa *shiftferm, const int max_cg,
+                      int *cg_return);
+/* inverter_multishift_full.h:30-33 */
+void recombine_shifted_vec3_to_vec3(const vec3_soa *in_shifted, const vec3_soa *in, vec3_soa *out,
+                                    const RationalApprox *approx);
+/* inverter_wrappers.h:16-34 */
+void convergence_messages(int conv_importance, int inverter_status);
+int inverter_wrapper(inverter_package ip, ferm_param *pars, vec3_soa *out, const vec3_soa *in,
+                     double res, int max_cg, double shift, int convergence_importance);
+int inverter_multishift_wrapper(inverter_package ip, ferm_param *pars, RationalApprox *approx,
+                                vec3_soa *out, const vec3_soa *in, double res, int max_cg,
+                                int convergence_importance);
+
+/* Same solvers with the diagnostics the reference only prints: iteration
+ * count, FP64 true residual ||b - M x|| / ||b||, reliable-update count, per-shift
+ * residuals; explicit restart / reliable-update parameters instead of the
+ * inverter_tricks global; `single` runs the CG-M in FP32 (multishift_invert_f). */
+int ostp_cg_solve(const su3_soa *u, ferm_param *pars, vec3_soa *solution, const vec3_soa *in, double res,
+                  vec3_soa *loc_r, vec3_soa *loc_h, vec3_soa *loc_s, vec3_soa *loc_p, int max_cg,
+                  double shift, int restarting_every, int *cg_return, double *true_res);
+int ostp_mixed_solve(inverter_package ip, ferm_param *pars, vec3_soa *solution, const vec3_soa *in,
+                     double res, int max_cg, double shift, double mixed_delta, int *cg_return,
+                     int *reliable_updates, double *true_res);
+int ostp_multishift_solve(const su3_soa *u, ferm_param *pars, RationalApprox *approx, vec3_soa *out,
+                          const vec3_soa *in, double residuo, vec3_soa *loc_r, vec3_soa *loc_h,
+                          vec3_soa *loc_s, vec3_soa *loc_p, vec3_soa *shiftferm, int max_cg, int single,
+                          int *cg_return, double *shift_res);
+
+/* ======================================================== generators ===== */
+/* Reference field rules (OpenAcc/random_assignement.c:67-92,106-142,188-250)
+ * drawn from a counter-based Philox4x32-10 stream keyed by `seed` instead of
+ * the sequential dSFMT state, so every site is generated independently on the
+ * device.  `stream` separates independent sources under one seed. */
+void ostp_generate_vec3_soa_gauss(vec3_soa *vect, uint64_t seed, uint32_t stream);
+void ostp_generate_vec3_soa_z2noise(vec3_soa *vect, uint64_t seed, uint32_t stream);
+void ostp_generate_Conf_cold(su3_soa *conf, double factor, uint64_t seed);
+/* U = exp(i*beta*H), H traceless Hermitian with standard-normal entries */
+void ostp_generate_Conf_expH(su3_soa *conf, double beta, uint64_t seed);
+
+/* ======================================================== timing (new) === */
+/* Mean device time in microseconds of one Deo (eo=0) or Doe (eo=1) launch
+ * (FP64 when single==0, FP32 otherwise), over `reps` back-to-back launches,
+ * timed with HIP events on the library's stream. */
+double ostp_time_dslash(const void *u, void *out, const void *in, const void *phases,
+                        int eo, int single, int reps);
+/* Mean device time in microseconds of one plain copy of `bytes` bytes. */
+double ostp_time_copy(void *dst, const void *src, size_t bytes, int reps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPENSTAPLE_AMD_H */

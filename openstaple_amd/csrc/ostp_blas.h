@@ -1,0 +1,36 @@
+// Internal launch interface of the BLAS, Dslash and solver kernels.
+#pragma once
+
+#include "ostp_reduce.h"
+
+namespace ostp {
+
+template <typename C> __host__ __device__ __forceinline__ C mkc(double a, double b);
+template <> __host__ __device__ __forceinline__ double2 mkc<double2>(double a, double b) { return make_double2(a, b); }
+template <> __host__ __device__ __forceinline__ float2 mkc<float2>(double a, double b) {
+    return make_float2((float)a, (float)b);
+}
+
+// Dslash (dslash.hip).  mode: 0 plain, 1 out = m2*x - D in, 2 = 1 + fused
+// Re<x,out> reduction handed to finalizer `fin`.
+template <typename R>
+bool launch_dslash(int eo, int mode, int fin, const void* u, const void* ph, const void* in, void* out,
+                   const void* x, double m2, const SolverCtl& ctl);
+template <typename R>
+bool launch_mdagm(const void* u, const void* ph, const void* in, void* temp, void* out, double m2, int mode,
+                  int fin, const SolverCtl& ctl);
+
+// BLAS (blas.hip)
+template <typename C>
+bool launch_axpy(const void* in1, double factor, const void* in2, void* out, const SolverCtl& ctl);
+bool launch_sub(const void* in1, const void* in2, void* out);
+bool launch_mass_minus(const void* in1, double m2, void* in2);
+bool launch_d2f(const void* d, void* f, long n);
+bool launch_f2d(const void* f, void* d, long n);
+template <typename C, int F>
+bool launch_norm2(const void* a, const SolverCtl& ctl);
+bool launch_dot(const void* a, const void* b, int complex_dot, const SolverCtl& ctl);
+bool reduce_to_host(int kind, const void* a, const void* b, int single, double* re, double* im);
+SolverCtl plain_ctl();
+
+}  // namespace ostp

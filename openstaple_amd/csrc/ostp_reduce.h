@@ -1,0 +1,175 @@
+// Deterministic grid reductions with an on-device scalar epilogue.
+//
+// Every block reduces its partial with a fixed shuffle/LDS tree and publishes
+// it at agent scope; the last block to take a ticket sums the partials in
+// block-index order (so the result does not depend on scheduling) and runs
+// the solver's scalar update ("finalizer") on the device.  The solver
+// recurrences therefore never wait for a host round trip.
+#pragma once
+
+#include "ostp_runtime.h"
+
+#include <math.h>
+
+namespace ostp {
+
+enum Fin : int {
+    kFinStore = 0,      // st->sum = total (stand-alone reduction)
+    kFinCgAlpha,        // alpha, omega = delta/alpha          (inverter_full.c:76-80)
+    kFinCgLambda,       // lambda, gammag, delta, counters, exit test (inverter_full.c:86-105)
+    kFinCgRestart,      // delta = |r|^2 at a (re)start          (inverter_full.c:60-66)
+    kFinMsAlpha,        // CG-M alpha + shift recurrences     (inverter_multishift_full.c:116-136)
+    kFinMsLambda,       // CG-M lambda, gammas, flags          (inverter_multishift_full.c:141-171)
+    kFinMxAlpha,        // mixed: omega + reliable-update test (inverter_mixedp.c:103-114)
+    kFinMxLambda,       // mixed: lambda, gammag, exit test     (inverter_mixedp.c:137-155)
+    kFinStore2,         // st->sum, st->sum_im (complex dot)
+};
+
+// Predicates evaluated by every block at kernel entry (uniform).
+enum Pred : int {
+    kAlways = 0,
+    kIfRunning = 1,      // status == kRunning
+    kIfIter = 2,         // cg == iteration number this launch belongs to
+    kIfRunningRU = 3,    // running and reliable update scheduled
+};
+
+struct SolverCtl {
+    SolverState* st = nullptr;
+    double* partials = nullptr;
+    unsigned int* ticket = nullptr;
+    int pred = kAlways;
+    int iter = 0;
+};
+
+__device__ __forceinline__ bool pred_pass(const SolverCtl& c) {
+    if (c.pred == kAlways) return true;
+    const SolverState* st = c.st;
+    if (c.pred == kIfRunning) return st->status == kRunning;
+    if (c.pred == kIfIter) return st->cg == c.iter;
+    if (c.pred == kIfRunningRU) return st->status == kRunning && st->ru != 0;
+    return false;
+}
+
+template <int F>
+__device__ __forceinline__ void finalize(SolverState* st, double tot, double tot_im) {
+    if (F == kFinStore) {
+        st->sum = tot;
+    } else if (F == kFinStore2) {
+        st->sum = tot;
+        st->sum_im = tot_im;
+    } else if (F == kFinCgAlpha) {
+        st->alpha = tot;
+        st->omega = st->delta / tot;
+    } else if (F == kFinCgRestart) {
+        st->delta = tot;
+        st->cg_restarted = 0;
+        st->status = kRunning;
+    } else if (F == kFinCgLambda) {
+        const double lambda = tot;
+        st->lambda = lambda;
+        st->gammag = lambda / st->delta;
+        st->delta = lambda;
+        st->cg += 1;
+        st->cg_restarted += 1;
+        const bool cont = (sqrt(lambda / st->source_norm) > st->res * st->safety) &&
+                          st->cg_restarted < st->restart_every;
+        if (!cont) st->status = kInnerExit;
+    } else if (F == kFinMsAlpha) {
+        // flags/maxiter computed at the end of the previous iteration become current
+        st->flag_cur ^= 1;
+        const int cur = st->flag_cur;
+        st->maxiter = st->maxiter_next;
+        st->alpha = tot;
+        st->omega_save = st->omega;
+        st->omega = -st->delta / tot;
+        const double omega = st->omega, omega_save = st->omega_save, gammag = st->gammag;
+        for (int k = 0; k < st->maxiter; k++) {
+            if (st->flags[cur][k] == 1) {
+                const double zi = st->zeta_i[k], zii = st->zeta_ii[k];
+                const double ziii =
+                    (zi * zii * omega_save) / (omega * gammag * (zi - zii) + zi * omega_save * (1.0 - (st->shifts[k]) * omega));
+                st->zeta_iii[k] = ziii;
+                st->omegas[k] = omega * ziii / zii;
+            }
+        }
+    } else if (F == kFinMsLambda) {
+        const double lambda = tot;
+        const int cur = st->flag_cur, nxt = cur ^ 1;
+        st->lambda = lambda;
+        const double gammag = lambda / st->delta;
+        st->gammag = gammag;
+        for (int k = 0; k < st->order; k++)
+            if (st->flags[cur][k] == 1)
+                st->gammas[k] = gammag * st->zeta_iii[k] * st->omegas[k] / (st->zeta_ii[k] * st->omega);
+        int maxiter = 0;
+        const double target = st->res * st->safety;
+        for (int k = 0; k < st->order; k++) {
+            int f = st->flags[cur][k];
+            if (f == 1) {
+                const double fact = sqrt(st->delta * st->zeta_ii[k] * st->zeta_ii[k] / st->source_norm);
+                if (fact < target) f = 0;
+                else maxiter = k + 1;
+                st->zeta_i[k] = st->zeta_ii[k];
+                st->zeta_ii[k] = st->zeta_iii[k];
+            }
+            st->flags[nxt][k] = f;
+        }
+        st->maxiter_next = maxiter;
+        st->delta = lambda;
+        st->cg += 1;
+        if (!(maxiter > 0 && st->cg < st->max_cg)) st->status = kDone;
+    } else if (F == kFinMxAlpha) {
+        st->alpha = tot;
+        st->omega = st->delta / tot;
+        if (st->last_max_res < st->delta) st->last_max_res = st->delta;
+        st->ru = (st->delta < st->mixed_delta * st->last_max_res) ? 1 : 0;
+    } else if (F == kFinMxLambda) {
+        const double lambda = tot;
+        st->lambda = lambda;
+        st->gammag = lambda / st->delta;
+        st->delta = lambda;
+        st->cg += 1;
+        if (st->ru) {
+            st->last_max_res = 0;
+            st->ru_count += 1;
+        }
+        const bool cont = (sqrt(lambda / st->source_norm) > st->res * st->safety) && st->cg < st->max_cg;
+        if (!cont) st->status = kDone;
+    }
+}
+
+// All threads of the block must call this (with 0 contributions from idle
+// threads).  NV = 1 or 2 values per thread.
+template <int F, int NV = 1>
+__device__ __forceinline__ void grid_reduce_finalize(double v0, double v1, const SolverCtl& c) {
+    __shared__ double lds[2][kBlock / 64];
+    __shared__ int am_last;
+    double b0 = block_sum<kBlock>(v0, lds[0]);
+    double b1 = 0.0;
+    if (NV == 2) b1 = block_sum<kBlock>(v1, lds[1]);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&c.partials[2 * blockIdx.x], b0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (NV == 2)
+            __hip_atomic_store(&c.partials[2 * blockIdx.x + 1], b1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned int t = __hip_atomic_fetch_add(c.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        am_last = (t == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) {
+        a0 += __hip_atomic_load(&c.partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (NV == 2) a1 += __hip_atomic_load(&c.partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    double t0 = block_sum<kBlock>(a0, lds[0]);
+    double t1 = 0.0;
+    if (NV == 2) t1 = block_sum<kBlock>(a1, lds[1]);
+    if (threadIdx.x == 0) {
+        finalize<F>(c.st, t0, t1);
+        __hip_atomic_store(c.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+}  // namespace ostp

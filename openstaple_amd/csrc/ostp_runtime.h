@@ -1,0 +1,81 @@
+// Internal runtime interface shared by the library's translation units.
+#pragma once
+
+#include "../../include/openstaple_amd_types.h"
+#include "ostp_common.h"
+
+namespace ostp {
+
+constexpr int kMaxShifts = MAX_APPROX_ORDER;
+constexpr int kBlock = 256;
+
+// Scalar state of a solver, resident in device memory so that iterations can be
+// queued without a host round trip per reduction.  Written only by the last
+// block of a reducing kernel (thread 0).
+enum SolverStatus : int { kRunning = 0, kInnerExit = 1, kDone = 2 };
+
+struct SolverState {
+    // CG scalars (inverter_full.c:28, inverter_mixedp.c:54-56)
+    double delta, alpha, lambda, omega, omega_save, gammag, source_norm;
+    double res, safety, last_max_res, mixed_delta, shift_m2;
+    double sum;               // result slot of stand-alone reductions
+    double sum_im;
+    int cg, cg_restarted, restart_every, max_cg;
+    int status;               // SolverStatus
+    int ru;                   // mixed precision: reliable update in this iteration
+    int ru_count;
+    // multi-shift CG-M (inverter_multishift_full.c:48-53)
+    int order, maxiter, maxiter_next, flag_cur;
+    int flags[2][kMaxShifts];
+    double zeta_i[kMaxShifts], zeta_ii[kMaxShifts], zeta_iii[kMaxShifts];
+    double omegas[kMaxShifts], gammas[kMaxShifts], shifts[kMaxShifts];
+};
+
+struct Context {
+    bool initialized = false;
+    Lattice lat{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int abort_on_error = 1;
+    // device scratch: reduction partials + tickets + solver state
+    double* partials = nullptr;     // 2 * max_blocks
+    unsigned int* tickets = nullptr;  // 8 counters, zero between launches
+    SolverState* state = nullptr;   // device
+    SolverState* state_host = nullptr;  // pinned mirror
+    int max_blocks = 0;
+};
+
+Context& ctx();
+
+// Error policy: record `msg`; abort the process if configured (default).
+// Returns false so callers can `return fail(...)`.
+bool fail(const char* fmt, ...);
+bool hip_ok(hipError_t e, const char* what);
+bool require_init(const char* what);
+
+// Translate a host address registered with ostp_acc_* (or a device address)
+// into the device address of `bytes` bytes; nullptr plus recorded error if the
+// address is unknown or the range does not fit its allocation.
+void* resolve(const void* p, size_t bytes, const char* what);
+template <typename T>
+inline T* resolve_as(const void* p, size_t bytes, const char* what) {
+    return reinterpret_cast<T*>(resolve(p, bytes, what));
+}
+
+// Launch geometry over sizeh half-sites.
+inline int blocks_for(long n, int nt = kBlock) { return (int)((n + nt - 1) / nt); }
+
+// Read the device solver state into the pinned mirror (synchronises the stream).
+bool pull_state();
+bool push_state();
+
+}  // namespace ostp
+
+// Map the hardware workgroup id to a logical tile so that each of the 8 XCDs
+// (which receive workgroups round-robin) works on a contiguous range of the
+// lattice, keeping neighbour re-reads inside one XCD's L2.
+__device__ __forceinline__ int xcd_block_id() {
+    const int G = gridDim.x, i = blockIdx.x;
+    const int q = G >> 3, r = G & 7, x = i & 7, k = i >> 3;
+    return x * q + (x < r ? x : r) + k;
+}

@@ -1,0 +1,795 @@
+// Krylov solvers of the fermion hot path with device-resident scalars.
+//
+//  ker_invert_openacc        OpenAcc/inverter_full.c:19-132      (CG, restarts, SAFETY 0.95)
+//  inverter_mixed_precision  OpenAcc/inverter_mixedp.c:41-181    (FP32 CG + FP64 reliable update, SAFETY 0.9)
+//  multishift_invert         OpenAcc/inverter_multishift_full.c:23-252 (CG-M, per-shift flags, SAFETY 0.95)
+//  recombine_shifted_vec3_to_vec3                                :254-282
+//  inverter_wrapper / inverter_multishift_wrapper  OpenAcc/inverter_wrappers.c:45-159
+//
+// The reference returns every dot product to the host (two device->host syncs
+// per iteration).  Here each reduction ends in an on-device finalizer that
+// applies the same scalar recurrence, and every iteration's kernels carry a
+// predicate (solver still running / this iteration started), so the host
+// queues iterations in batches and synchronises only between batches.  The
+// sequence of floating-point scalar operations is the reference's; vector
+// updates are fused (x and r together; the multishift x_k and p_k together
+// with p) without changing what each element computes.
+#include "ostp_blas.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <type_traits>
+
+namespace ostp {
+
+// ------------------------------------------------------------------ kernels
+// x += omega p ; r -= omega s ; lambda = |r|^2   (inverter_full.c:82-86)
+template <typename C>
+__global__ __launch_bounds__(kBlock) void k_cg_xr(C* __restrict__ x, C* __restrict__ r, const C* __restrict__ p,
+                                                  const C* __restrict__ s, int sizeh, SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const double om = ctl.st->omega;
+    double v = 0.0;
+    if (i < sizeh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = (long)c * sizeh + i;
+            const C pp = p[k], ss = s[k], xx = x[k], rr = r[k];
+            x[k] = mkc<C>((double)pp.x * om + (double)xx.x, (double)pp.y * om + (double)xx.y);
+            const C rn = mkc<C>((double)ss.x * (-om) + (double)rr.x, (double)ss.y * (-om) + (double)rr.y);
+            r[k] = rn;
+            v += (double)rn.x * rn.x + (double)rn.y * rn.y;
+        }
+    }
+    grid_reduce_finalize<kFinCgLambda, 1>(v, 0.0, ctl);
+}
+
+// p = p*gammag + r   (inverter_full.c:96)
+template <typename C>
+__global__ __launch_bounds__(kBlock) void k_cg_p(C* __restrict__ p, const C* __restrict__ r, long n3, SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n3) return;
+    const double g = ctl.st->gammag;
+    const C pp = p[i], rr = r[i];
+    p[i] = mkc<C>((double)pp.x * g + (double)rr.x, (double)pp.y * g + (double)rr.y);
+}
+
+// r = b - s ; p = r ; |r|^2 -> finalizer F   (inverter_full.c:56-63)
+template <int F>
+__global__ __launch_bounds__(kBlock) void k_cg_restart(const double2* __restrict__ b, const double2* __restrict__ s,
+                                                       double2* __restrict__ r, double2* __restrict__ p, int sizeh,
+                                                       SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double v = 0.0;
+    if (i < sizeh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = (long)c * sizeh + i;
+            const double2 bb = b[k], ss = s[k];
+            const double2 rr = make_double2(bb.x - ss.x, bb.y - ss.y);
+            r[k] = rr;
+            if (p) p[k] = rr;
+            v += rr.x * rr.x + rr.y * rr.y;
+        }
+    }
+    grid_reduce_finalize<F, 1>(v, 0.0, ctl);
+}
+
+// r += omega s ; lambda = |r|^2   (inverter_multishift_full.c:141-142)
+template <typename C>
+__global__ __launch_bounds__(kBlock) void k_ms_r(C* __restrict__ r, const C* __restrict__ s, int sizeh, SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const double om = ctl.st->omega;
+    double v = 0.0;
+    if (i < sizeh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = (long)c * sizeh + i;
+            const C rr = r[k], ss = s[k];
+            const C rn = mkc<C>((double)rr.x + om * (double)ss.x, (double)rr.y + om * (double)ss.y);
+            r[k] = rn;
+            v += (double)rn.x * rn.x + (double)rn.y * rn.y;
+        }
+    }
+    grid_reduce_finalize<kFinMsLambda, 1>(v, 0.0, ctl);
+}
+
+// p = p*gammag + r ; for active shifts k: x_k -= omega_k p_k ; p_k = gamma_k p_k + zeta_k r
+// (inverter_multishift_full.c:138,145,158 = fermionic_utilities.c:315-378)
+template <typename C>
+__global__ __launch_bounds__(kBlock) void k_ms_update(const C* __restrict__ r, C* __restrict__ p, C* __restrict__ xs,
+                                                      C* __restrict__ ps, long vstride, long n3, SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n3) return;
+    const SolverState* st = ctl.st;
+    const double g = st->gammag;
+    const C rr = r[i], pp = p[i];
+    p[i] = mkc<C>((double)pp.x * g + (double)rr.x, (double)pp.y * g + (double)rr.y);
+    const int cur = st->flag_cur, maxiter = st->maxiter;
+    for (int k = 0; k < maxiter; k++) {
+        if (st->flags[cur][k] != 1) continue;
+        const double om = st->omegas[k], gk = st->gammas[k], zk = st->zeta_iii[k];
+        C* xk = xs + k * vstride;
+        C* pk = ps + k * vstride;
+        const C pv = pk[i], xv = xk[i];
+        xk[i] = mkc<C>((double)xv.x - om * (double)pv.x, (double)xv.y - om * (double)pv.y);
+        pk[i] = mkc<C>(gk * (double)pv.x + zk * (double)rr.x, gk * (double)pv.y + zk * (double)rr.y);
+    }
+}
+
+// out = a0*in + sum_k a_k x_k   (inverter_multishift_full.c:254-282)
+struct RecombineCoeffs {
+    double a0;
+    double a[kMaxShifts];
+    int order;
+};
+__global__ __launch_bounds__(kBlock) void k_recombine(const double2* __restrict__ xs, const double2* __restrict__ in,
+                                                      double2* __restrict__ out, long vstride, long n3,
+                                                      RecombineCoeffs c) {
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n3) return;
+    const double2 v = in[i];
+    double ox = v.x * c.a0, oy = v.y * c.a0;
+    for (int k = 0; k < c.order; k++) {
+        const double2 x = xs[k * vstride + i];
+        ox += c.a[k] * x.x;
+        oy += c.a[k] * x.y;
+    }
+    out[i] = make_double2(ox, oy);
+}
+
+// ---- mixed precision ----
+// r_f = (float)(b - s) ; p_f = r_f ; out_f = 0 ; delta = |r_f|^2   (inverter_mixedp.c:74-88)
+__global__ __launch_bounds__(kBlock) void k_mx_init(const double2* __restrict__ b, const double2* __restrict__ s,
+                                                    double2* __restrict__ r64, float2* __restrict__ rf,
+                                                    float2* __restrict__ pf, float2* __restrict__ outf, int sizeh,
+                                                    SolverCtl ctl) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double v = 0.0;
+    if (i < sizeh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = (long)c * sizeh + i;
+            const double2 bb = b[k], ss = s[k];
+            const double2 rr = make_double2(bb.x - ss.x, bb.y - ss.y);
+            r64[k] = rr;
+            const float2 f = make_float2((float)rr.x, (float)rr.y);
+            rf[k] = f;
+            pf[k] = f;
+            outf[k] = make_float2(0.f, 0.f);
+            v += (double)f.x * f.x + (double)f.y * f.y;
+        }
+    }
+    grid_reduce_finalize<kFinCgRestart, 1>(v, 0.0, ctl);
+}
+
+// out_f += omega p_f; then either r_f -= omega s_f with lambda = |r_f|^2, or
+// (reliable update) x += out_f, out_f = 0   (inverter_mixedp.c:107-134)
+__global__ __launch_bounds__(kBlock) void k_mx_xr(float2* __restrict__ outf, const float2* __restrict__ pf,
+                                                  float2* __restrict__ rf, const float2* __restrict__ sf,
+                                                  double2* __restrict__ x, int sizeh, SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const double om = ctl.st->omega;
+    const int ru = ctl.st->ru;
+    double v = 0.0;
+    if (i < sizeh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = (long)c * sizeh + i;
+            const float2 pp = pf[k], oo = outf[k];
+            const float2 on = make_float2((float)((double)pp.x * om + (double)oo.x), (float)((double)pp.y * om + (double)oo.y));
+            if (!ru) {
+                outf[k] = on;
+                const float2 ss = sf[k], rr = rf[k];
+                const float2 rn = make_float2((float)((double)ss.x * (-om) + (double)rr.x),
+                                              (float)((double)ss.y * (-om) + (double)rr.y));
+                rf[k] = rn;
+                v += (double)rn.x * rn.x + (double)rn.y * rn.y;
+            } else {
+                const double2 xx = x[k];
+                x[k] = make_double2(xx.x + (double)on.x, xx.y + (double)on.y);
+                outf[k] = make_float2(0.f, 0.f);
+            }
+        }
+    }
+    if (!ru) grid_reduce_finalize<kFinMxLambda, 1>(v, 0.0, ctl);
+}
+
+// reliable update tail: r = b - s (FP64), r_f = (float) r, lambda = |r_f|^2  (inverter_mixedp.c:121-124,137)
+__global__ __launch_bounds__(kBlock) void k_mx_resid(const double2* __restrict__ b, const double2* __restrict__ s,
+                                                     double2* __restrict__ r64, float2* __restrict__ rf, int sizeh,
+                                                     SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double v = 0.0;
+    if (i < sizeh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = (long)c * sizeh + i;
+            const double2 bb = b[k], ss = s[k];
+            const double2 rr = make_double2(bb.x - ss.x, bb.y - ss.y);
+            r64[k] = rr;
+            const float2 f = make_float2((float)rr.x, (float)rr.y);
+            rf[k] = f;
+            v += (double)f.x * f.x + (double)f.y * f.y;
+        }
+    }
+    grid_reduce_finalize<kFinMxLambda, 1>(v, 0.0, ctl);
+}
+
+// x += (double) out_f   (combine_add_in2_into_in1_mixed_precision, inverter_mixedp.c:24-34)
+__global__ __launch_bounds__(kBlock) void k_add_f_to_d(double2* __restrict__ x, const float2* __restrict__ f, long n3) {
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n3) return;
+    const double2 xx = x[i];
+    const float2 ff = f[i];
+    x[i] = make_double2(xx.x + (double)ff.x, xx.y + (double)ff.y);
+}
+
+// -------------------------------------------------------------- host driver
+static inline hipStream_t S() { return ctx().stream; }
+static inline SolverState& H() { return *ctx().state_host; }
+
+static SolverCtl ctl_of(int pred, int iter = 0) {
+    SolverCtl c = plain_ctl();
+    c.pred = pred;
+    c.iter = iter;
+    return c;
+}
+
+static bool check_launch(const char* what) { return hip_ok(hipGetLastError(), what); }
+
+static int batch_next(int b) { return std::min(2 * b, 64); }
+
+// Queue iterations until the device state leaves kRunning.
+template <typename Enq>
+static bool run_batches(Enq enqueue, int first_batch = 4) {
+    int batch = first_batch;
+    int last_cg = -1;
+    for (;;) {
+        if (!pull_state()) return false;
+        if (H().status != kRunning) return true;
+        const int cg0 = H().cg;
+        // every queued iteration of a running solver advances the counter on
+        // the device; a stalled counter means the scalar epilogue did not run
+        if (cg0 == last_cg) return fail("solver state did not advance (iteration %d)", cg0);
+        last_cg = cg0;
+        for (int j = 1; j <= batch; j++)
+            if (!enqueue(cg0 + j)) return false;
+        batch = batch_next(batch);
+    }
+}
+
+static void init_state(double res, double safety, int max_cg, int restart_every) {
+    SolverState& s = H();
+    std::memset(&s, 0, sizeof s);
+    s.res = res;
+    s.safety = safety;
+    s.max_cg = max_cg;
+    s.restart_every = restart_every;
+    s.status = kRunning;
+}
+
+// CG of ker_invert_openacc on device pointers.  Returns status, *cg_out.
+struct CgBufs {
+    const void *u, *ph, *b;
+    void *x, *r, *h, *s, *p;
+};
+
+static bool source_norm(const void* b, double* sn) { return reduce_to_host(0, b, nullptr, 0, sn, nullptr); }
+
+int cg_solve(const CgBufs& B, double mass, double shift, double res, int max_cg, int restart_every, int* cg_out,
+             double* true_res) {
+    const long sizeh = ctx().lat.sizeh;
+    const long n3 = 3 * sizeh;
+    const int nbh = blocks_for(sizeh), nb3 = blocks_for(n3);
+    const double m2 = mass * mass + shift;
+    double sn = 0.0;
+    if (!source_norm(B.b, &sn)) return INVERTER_FAILURE;
+    init_state(res, 0.95, max_cg, restart_every);
+    H().source_norm = sn;
+    if (!push_state()) return INVERTER_FAILURE;
+
+    auto restart = [&]() -> bool {
+        // s = M x ; r = b - s ; p = r ; delta = |r|^2 ; cg_restarted = 0
+        if (!launch_mdagm<double>(B.u, B.ph, B.x, B.h, B.s, m2, 1, kFinStore, ctl_of(kAlways))) return false;
+        hipLaunchKernelGGL((k_cg_restart<kFinCgRestart>), dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b,
+                           (const double2*)B.s, (double2*)B.r, (double2*)B.p, (int)sizeh, ctl_of(kAlways));
+        return check_launch("k_cg_restart");
+    };
+    auto iteration = [&](int j) -> bool {
+        // s = M p ; alpha = (p,s) ; omega = delta/alpha
+        if (!launch_mdagm<double>(B.u, B.ph, B.p, B.h, B.s, m2, 2, kFinCgAlpha, ctl_of(kIfRunning))) return false;
+        hipLaunchKernelGGL((k_cg_xr<double2>), dim3(nbh), dim3(kBlock), 0, S(), (double2*)B.x, (double2*)B.r,
+                           (const double2*)B.p, (const double2*)B.s, (int)sizeh, ctl_of(kIfRunning));
+        hipLaunchKernelGGL((k_cg_p<double2>), dim3(nb3), dim3(kBlock), 0, S(), (double2*)B.p, (const double2*)B.r, n3,
+                           ctl_of(kIfIter, j));
+        return check_launch("cg iteration");
+    };
+
+    if (!restart()) return INVERTER_FAILURE;
+    for (;;) {
+        if (!run_batches(iteration)) return INVERTER_FAILURE;
+        const SolverState& s = H();
+        if (s.status == kInnerExit && std::sqrt(s.lambda / s.source_norm) > res && s.cg < max_cg) {
+            if (!restart()) return INVERTER_FAILURE;
+            continue;
+        }
+        break;
+    }
+    const int cg = H().cg;
+    // final true residual (inverter_full.c:114-130)
+    if (!launch_mdagm<double>(B.u, B.ph, B.x, B.h, B.s, m2, 1, kFinStore, ctl_of(kAlways))) return INVERTER_FAILURE;
+    hipLaunchKernelGGL((k_cg_restart<kFinStore>), dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b,
+                       (const double2*)B.s, (double2*)B.h, (double2*)nullptr, (int)sizeh, ctl_of(kAlways));
+    if (!check_launch("final residual") || !pull_state()) return INVERTER_FAILURE;
+    const double current_res = H().sum / sn;
+    if (verbosity_lv > 1)
+        printf("Terminated invert after   %d    iterations[res/stop_res=  %e , stop_res=%e ]\n", cg,
+               std::sqrt(current_res) / res, res);
+    if (cg == max_cg) printf("WARNING: maximum number of iterations reached in invert\n");
+    *cg_out = cg;
+    if (true_res) *true_res = std::sqrt(current_res);
+    return (std::sqrt(current_res) <= res) ? INVERTER_SUCCESS : INVERTER_FAILURE;
+}
+
+// Mixed precision CG (inverter_mixed_precision) on device pointers.
+struct MxBufs {
+    const void *u, *u_f, *ph, *ph_f, *b;
+    void *x, *r, *h, *s;              // FP64 scratch (ip.loc_r/h/s)
+    void *rf, *hf, *sf, *pf, *outf;   // FP32 scratch (ip.loc_*_f, ip.out_f)
+};
+
+int mixed_solve(const MxBufs& B, double mass, double shift, double res, int max_cg, double mixed_delta,
+                int* cg_out, int* ru_count, double* true_res) {
+    const long sizeh = ctx().lat.sizeh;
+    const long n3 = 3 * sizeh;
+    const int nbh = blocks_for(sizeh), nb3 = blocks_for(n3);
+    const double m2 = mass * mass + shift;
+    const double m2f = mass * mass + (double)(float)shift;  // sp_ code passes `float shift`
+    double sn = 0.0;
+    if (!source_norm(B.b, &sn)) return INVERTER_FAILURE;
+    init_state(res, 0.9, max_cg, 0);
+    H().source_norm = sn;
+    H().mixed_delta = mixed_delta;
+    if (!push_state()) return INVERTER_FAILURE;
+
+    // r = b - M x (FP64) ; r_f = (float) r ; p_f = r_f ; out_f = 0 ; delta = |r_f|^2
+    if (!launch_mdagm<double>(B.u, B.ph, B.x, B.h, B.s, m2, 1, kFinStore, ctl_of(kAlways))) return INVERTER_FAILURE;
+    hipLaunchKernelGGL(k_mx_init, dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b, (const double2*)B.s,
+                       (double2*)B.r, (float2*)B.rf, (float2*)B.pf, (float2*)B.outf, (int)sizeh, ctl_of(kAlways));
+    if (!check_launch("k_mx_init")) return INVERTER_FAILURE;
+
+    auto iteration = [&](int j) -> bool {
+        if (!launch_mdagm<float>(B.u_f, B.ph_f, B.pf, B.hf, B.sf, m2f, 2, kFinMxAlpha, ctl_of(kIfRunning)))
+            return false;
+        hipLaunchKernelGGL(k_mx_xr, dim3(nbh), dim3(kBlock), 0, S(), (float2*)B.outf, (const float2*)B.pf,
+                           (float2*)B.rf, (const float2*)B.sf, (double2*)B.x, (int)sizeh, ctl_of(kIfRunning));
+        // reliable update: s = M x (FP64), r = b - s, r_f = (float) r
+        if (!launch_mdagm<double>(B.u, B.ph, B.x, B.h, B.s, m2, 1, kFinStore, ctl_of(kIfRunningRU))) return false;
+        hipLaunchKernelGGL(k_mx_resid, dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b, (const double2*)B.s,
+                           (double2*)B.r, (float2*)B.rf, (int)sizeh, ctl_of(kIfRunningRU));
+        hipLaunchKernelGGL((k_cg_p<float2>), dim3(nb3), dim3(kBlock), 0, S(), (float2*)B.pf, (const float2*)B.rf, n3,
+                           ctl_of(kIfIter, j));
+        return check_launch("mixed iteration");
+    };
+    if (!run_batches(iteration)) return INVERTER_FAILURE;
+    const int cg = H().cg;
+    if (ru_count) *ru_count = H().ru_count;
+    hipLaunchKernelGGL(k_add_f_to_d, dim3(nb3), dim3(kBlock), 0, S(), (double2*)B.x, (const float2*)B.outf, n3);
+    if (!launch_mdagm<double>(B.u, B.ph, B.x, B.h, B.s, m2, 1, kFinStore, ctl_of(kAlways))) return INVERTER_FAILURE;
+    hipLaunchKernelGGL((k_cg_restart<kFinStore>), dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b,
+                       (const double2*)B.s, (double2*)B.h, (double2*)nullptr, (int)sizeh, ctl_of(kAlways));
+    if (!check_launch("final residual") || !pull_state()) return INVERTER_FAILURE;
+    const double giustoono = H().sum / sn;
+    if (verbosity_lv > 1)
+        printf("Terminated invert after   %d    iterations[res/stop_res=  %e , stop_res=%e ]\n", cg,
+               std::sqrt(giustoono) / res, res);
+    if (cg == max_cg) printf("WARNING: maximum number of iterations reached in invert\n");
+    *cg_out = cg;
+    if (true_res) *true_res = std::sqrt(giustoono);
+    return (std::sqrt(giustoono) <= res) ? INVERTER_SUCCESS : INVERTER_FAILURE;
+}
+
+// CG-M (multishift_invert) on device pointers; R = double or float vectors
+// (the float variant is the sp_ multishift_invert_f: float data, double scalars).
+struct MsBufs {
+    const void *u, *ph, *b;
+    void *out, *r, *h, *s, *p, *shiftferm;
+};
+
+template <typename R>
+int multishift_solve(const MsBufs& B, double mass, int order, const double* shifts, double residuo, int max_cg,
+                     int* cg_out, double* shift_res) {
+    using C = typename std::conditional<sizeof(R) == 8, double2, float2>::type;
+    const Lattice& L = ctx().lat;
+    const long sizeh = L.sizeh;
+    const long n3 = 3 * sizeh;
+    const long vstride = n3;  // in complex elements
+    const size_t vb = sizeof(R) == 8 ? L.vec3_bytes() : L.vec3f_bytes();
+    const int nbh = blocks_for(sizeh), nb3 = blocks_for(n3);
+    const double m2 = mass * mass;
+    if (order < 0 || order > kMaxShifts) return fail("multishift: order %d out of range", order) ? 0 : INVERTER_FAILURE;
+
+    // x_k = 0 ; r = p = b ; p_k = b
+    for (int k = 0; k < order; k++) {
+        if (!hip_ok(hipMemsetAsync((char*)B.out + k * vb, 0, vb, S()), "zero x_k")) return INVERTER_FAILURE;
+        if (!hip_ok(hipMemcpyAsync((char*)B.shiftferm + k * vb, B.b, vb, hipMemcpyDeviceToDevice, S()), "p_k = b"))
+            return INVERTER_FAILURE;
+    }
+    if (!hip_ok(hipMemcpyAsync(B.r, B.b, vb, hipMemcpyDeviceToDevice, S()), "r = b") ||
+        !hip_ok(hipMemcpyAsync(B.p, B.b, vb, hipMemcpyDeviceToDevice, S()), "p = b"))
+        return INVERTER_FAILURE;
+    double sn = 0.0;
+    bool ok = sizeof(R) == 8 ? reduce_to_host(0, B.b, nullptr, 0, &sn, nullptr)
+                             : reduce_to_host(0, B.b, nullptr, 1, &sn, nullptr);
+    if (!ok) return INVERTER_FAILURE;
+    init_state(residuo, 0.95, max_cg, 0);
+    SolverState& s = H();
+    s.delta = sn;
+    s.source_norm = sn;
+    s.omega = 1.0;
+    s.gammag = 0.0;
+    s.order = order;
+    s.flag_cur = 0;
+    for (int k = 0; k < order; k++) {
+        s.flags[1][k] = 1;  // becomes current at the first iteration
+        s.zeta_i[k] = 1.0;
+        s.zeta_ii[k] = 1.0;
+        s.gammas[k] = 0.0;
+        s.shifts[k] = shifts[k];
+    }
+    s.maxiter_next = order;
+    if (order == 0) s.status = kDone;  // the reference's loop runs once; nothing to update without shifts
+    if (!push_state()) return INVERTER_FAILURE;
+
+    auto iteration = [&](int j) -> bool {
+        if (!launch_mdagm<R>(B.u, B.ph, B.p, B.h, B.s, m2, 2, kFinMsAlpha, ctl_of(kIfRunning))) return false;
+        hipLaunchKernelGGL((k_ms_r<C>), dim3(nbh), dim3(kBlock), 0, S(), (C*)B.r, (const C*)B.s, (int)sizeh,
+                           ctl_of(kIfRunning));
+        hipLaunchKernelGGL((k_ms_update<C>), dim3(nb3), dim3(kBlock), 0, S(), (const C*)B.r, (C*)B.p, (C*)B.out,
+                           (C*)B.shiftferm, vstride, n3, ctl_of(kIfIter, j));
+        return check_launch("multishift iteration");
+    };
+    if (!run_batches(iteration)) return INVERTER_FAILURE;
+    const int cg = H().cg;
+    multishift_invert_iterations += cg;
+    if (cg == max_cg) printf("WARNING: maximum number of iterations reached in invert\n");
+
+    // per-shift verification with (M + sigma_k) (inverter_multishift_full.c:212-229)
+    int check = 1;
+    for (int k = 0; k < order; k++) {
+        const double m2k = sizeof(R) == 8 ? m2 + shifts[k] : m2 + (double)(float)shifts[k];
+        const void* xk = (const char*)B.out + k * vb;
+        if (!launch_mdagm<R>(B.u, B.ph, xk, B.h, B.s, m2k, 1, kFinStore, ctl_of(kAlways))) return INVERTER_FAILURE;
+        double g = 0.0;
+        if (sizeof(R) == 8) {
+            hipLaunchKernelGGL((k_cg_restart<kFinStore>), dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b,
+                               (const double2*)B.s, (double2*)B.h, (double2*)nullptr, (int)sizeh, ctl_of(kAlways));
+            if (!check_launch("verify") || !pull_state()) return INVERTER_FAILURE;
+            g = H().sum / sn;
+        } else {
+            if (!launch_axpy<float2>(B.s, -1.0, B.b, B.h, ctl_of(kAlways))) return INVERTER_FAILURE;
+            double nn = 0.0;
+            if (!reduce_to_host(0, B.h, nullptr, 1, &nn, nullptr)) return INVERTER_FAILURE;
+            g = nn / sn;
+        }
+        if (shift_res) shift_res[k] = std::sqrt(g);
+        check *= (g <= 1) ? 1 : 0;
+    }
+    if (!hip_ok(hipMemcpyAsync(B.p, (const char*)B.out + (order > 0 ? (order - 1) : 0) * vb, vb,
+                               hipMemcpyDeviceToDevice, S()),
+                "loc_p"))
+        return INVERTER_FAILURE;
+    if (verbosity_lv > 0)
+        printf("Terminated multishift_invert ( target res = %1.1e,source_norm = %1.1e )\tCG count %d\n", residuo, sn,
+               cg);
+    *cg_out = cg;
+    if (!hip_ok(hipStreamSynchronize(S()), "synchronize")) return INVERTER_FAILURE;
+    return check == 1 ? INVERTER_SUCCESS : INVERTER_FAILURE;
+}
+
+template int multishift_solve<double>(const MsBufs&, double, int, const double*, double, int, int*, double*);
+template int multishift_solve<float>(const MsBufs&, double, int, const double*, double, int, int*, double*);
+
+bool recombine(const void* xs, const void* in, void* out, const RationalApprox* ra) {
+    RecombineCoeffs c;
+    c.a0 = ra->RA_a0;
+    c.order = ra->approx_order;
+    if (c.order < 0 || c.order > kMaxShifts) return fail("recombine: order %d out of range", c.order);
+    for (int k = 0; k < kMaxShifts; k++) c.a[k] = k < c.order ? ra->RA_a[k] : 0.0;
+    const long n3 = 3 * ctx().lat.sizeh;
+    hipLaunchKernelGGL(k_recombine, dim3(blocks_for(n3)), dim3(kBlock), 0, S(), (const double2*)xs,
+                       (const double2*)in, (double2*)out, n3, n3, c);
+    return check_launch("k_recombine") && hip_ok(hipStreamSynchronize(S()), "synchronize");
+}
+
+// library-owned FP32 scratch for the single-precision-accelerated multishift
+// (the reference's globals aux1_f and ferm_shiftmulti_acc_f, alloc_vars.c)
+static void* g_aux1_f = nullptr;
+static void* g_shiftmulti_f = nullptr;
+static size_t g_aux1_bytes = 0, g_shiftmulti_bytes = 0;
+
+static void* scratch(void*& p, size_t& have, size_t need) {
+    if (have >= need) return p;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    have = 0;
+    if (!hip_ok(hipMalloc(&p, need), "scratch allocation")) return nullptr;
+    have = need;
+    return p;
+}
+
+}  // namespace ostp
+
+using namespace ostp;
+
+// ------------------------------------------------------------ C entry points
+extern "C" {
+
+int ker_invert_openacc(const su3_soa* u, ferm_param* pars, vec3_soa* solution, const vec3_soa* in, double res,
+                       vec3_soa* loc_r, vec3_soa* loc_h, vec3_soa* loc_s, vec3_soa* loc_p, const int max_cg,
+                       double shift, int* cg_return) {
+    const char* nm = "ker_invert_openacc";
+    if (!require_init(nm)) return INVERTER_FAILURE;
+    if (!pars || !cg_return) return fail("%s: null argument", nm) ? 0 : INVERTER_FAILURE;
+    const Lattice& L = ctx().lat;
+    const size_t vb = L.vec3_bytes();
+    CgBufs B;
+    B.u = resolve(u, 8 * L.su3_bytes(), nm);
+    B.ph = resolve(pars->phases, 8 * L.dsoa_bytes(), nm);
+    B.b = resolve(in, vb, nm);
+    B.x = resolve(solution, vb, nm);
+    B.r = resolve(loc_r, vb, nm);
+    B.h = resolve(loc_h, vb, nm);
+    B.s = resolve(loc_s, vb, nm);
+    B.p = resolve(loc_p, vb, nm);
+    if (!B.u || !B.ph || !B.b || !B.x || !B.r || !B.h || !B.s || !B.p) return INVERTER_FAILURE;
+    int cg = 0;
+    int st = cg_solve(B, pars->ferm_mass, shift, res, max_cg, inverter_tricks.restartingEvery, &cg, nullptr);
+    *cg_return = cg;
+    return st;
+}
+
+int inverter_mixed_precision(inverter_package ip, ferm_param* pars, vec3_soa* solution, const vec3_soa* in,
+                             double res, const int max_cg, double shift, int* cg_return) {
+    const char* nm = "inverter_mixed_precision";
+    if (!require_init(nm)) return INVERTER_FAILURE;
+    if (!pars || !cg_return) return fail("%s: null argument", nm) ? 0 : INVERTER_FAILURE;
+    const Lattice& L = ctx().lat;
+    const size_t vb = L.vec3_bytes(), vbf = L.vec3f_bytes();
+    MxBufs B;
+    B.u = resolve(ip.u, 8 * L.su3_bytes(), nm);
+    B.u_f = resolve(ip.u_f, 8 * L.su3f_bytes(), nm);
+    B.ph = resolve(pars->phases, 8 * L.dsoa_bytes(), nm);
+    B.ph_f = resolve(pars->phases_f, 8 * L.fsoa_bytes(), nm);
+    B.b = resolve(in, vb, nm);
+    B.x = resolve(solution, vb, nm);
+    B.r = resolve(ip.loc_r, vb, nm);
+    B.h = resolve(ip.loc_h, vb, nm);
+    B.s = resolve(ip.loc_s, vb, nm);
+    B.rf = resolve(ip.loc_r_f, vbf, nm);
+    B.hf = resolve(ip.loc_h_f, vbf, nm);
+    B.sf = resolve(ip.loc_s_f, vbf, nm);
+    B.pf = resolve(ip.loc_p_f, vbf, nm);
+    B.outf = resolve(ip.out_f, vbf, nm);
+    if (!B.u || !B.u_f || !B.ph || !B.ph_f || !B.b || !B.x || !B.r || !B.h || !B.s || !B.rf || !B.hf || !B.sf ||
+        !B.pf || !B.outf)
+        return INVERTER_FAILURE;
+    int cg = 0;
+    int st = mixed_solve(B, pars->ferm_mass, shift, res, max_cg, inverter_tricks.mixedPrecisionDelta, &cg, nullptr,
+                         nullptr);
+    *cg_return = cg;
+    return st;
+}
+
+int multishift_invert(const su3_soa* u, ferm_param* pars, RationalApprox* approx, vec3_soa* out, const vec3_soa* in,
+                      double residuo, vec3_soa* loc_r, vec3_soa* loc_h, vec3_soa* loc_s, vec3_soa* loc_p,
+                      vec3_soa* shiftferm, const int max_cg, int* cg_return) {
+    const char* nm = "multishift_invert";
+    if (!require_init(nm)) return INVERTER_FAILURE;
+    if (!pars || !approx || !cg_return) return fail("%s: null argument", nm) ? 0 : INVERTER_FAILURE;
+    const int order = approx->approx_order;
+    if (order < 0 || order > kMaxShifts) return fail("%s: approx_order %d", nm, order) ? 0 : INVERTER_FAILURE;
+    const Lattice& L = ctx().lat;
+    const size_t vb = L.vec3_bytes();
+    const size_t nv = order > 0 ? (size_t)order : 1;
+    MsBufs B;
+    B.u = resolve(u, 8 * L.su3_bytes(), nm);
+    B.ph = resolve(pars->phases, 8 * L.dsoa_bytes(), nm);
+    B.b = resolve(in, vb, nm);
+    B.out = resolve(out, nv * vb, nm);
+    B.shiftferm = resolve(shiftferm, nv * vb, nm);
+    B.r = resolve(loc_r, vb, nm);
+    B.h = resolve(loc_h, vb, nm);
+    B.s = resolve(loc_s, vb, nm);
+    B.p = resolve(loc_p, vb, nm);
+    if (!B.u || !B.ph || !B.b || !B.out || !B.shiftferm || !B.r || !B.h || !B.s || !B.p) return INVERTER_FAILURE;
+    int cg = 0;
+    int st = multishift_solve<double>(B, pars->ferm_mass, order, approx->RA_b, residuo, max_cg, &cg, nullptr);
+    *cg_return = cg;
+    return st;
+}
+
+void recombine_shifted_vec3_to_vec3(const vec3_soa* in_shifted, const vec3_soa* in, vec3_soa* out,
+                                    const RationalApprox* approx) {
+    const char* nm = "recombine_shifted_vec3_to_vec3";
+    if (!require_init(nm)) return;
+    if (!approx) { fail("%s: null approx", nm); return; }
+    const Lattice& L = ctx().lat;
+    const size_t vb = L.vec3_bytes();
+    const size_t nv = approx->approx_order > 0 ? (size_t)approx->approx_order : 1;
+    const void* xs = resolve(in_shifted, nv * vb, nm);
+    const void* di = resolve(in, vb, nm);
+    void* dout = resolve(out, vb, nm);
+    if (xs && di && dout) recombine(xs, di, dout, approx);
+}
+
+void convergence_messages(int conv_importance, int inverter_status) {
+    if (INVERTER_FAILURE == inverter_status) {
+        if (CONVERGENCE_CRITICAL == conv_importance) {
+            printf("\n\n\t\tERROR : inverter failed to converge in a critical region of the code. Exiting now!!\n\n");
+            exit(1);
+        }
+        printf("\n\t\tWARNING : inverter failed to converge.\n");
+    }
+}
+
+int inverter_wrapper(inverter_package ip, ferm_param* pars, vec3_soa* out, const vec3_soa* in, double res,
+                     int max_cg, double shift, int convergence_importance) {
+    int cg_return = 0;
+    int st;
+    if (inverter_tricks.useMixedPrecision)
+        st = inverter_mixed_precision(ip, pars, out, in, res, max_cg, shift, &cg_return);
+    else
+        st = ker_invert_openacc(ip.u, pars, out, in, res, ip.loc_r, ip.loc_h, ip.loc_s, ip.loc_p, max_cg, shift,
+                                &cg_return);
+    convergence_messages(convergence_importance, st);
+    return cg_return;
+}
+
+int inverter_multishift_wrapper(inverter_package ip, ferm_param* pars, RationalApprox* approx, vec3_soa* out,
+                                const vec3_soa* in, double res, int max_cg, int convergence_importance) {
+    const char* nm = "inverter_multishift_wrapper";
+    if (!require_init(nm)) return 0;
+    int total_iterations = 0;
+    int cg_return = 0;
+    int st;
+    if (inverter_tricks.singlePInvAccelMultiInv) {
+        // SP multishift to a loose target, then per-shift DP (or mixed) CG refinement
+        // (inverter_wrappers.c:58-98)
+        const Lattice& L = ctx().lat;
+        const int order = approx->approx_order;
+        const size_t nv = order > 0 ? (size_t)order : 1;
+        void* aux1 = scratch(g_aux1_f, g_aux1_bytes, L.vec3f_bytes());
+        void* xsf = scratch(g_shiftmulti_f, g_shiftmulti_bytes, nv * L.vec3f_bytes());
+        const void* db = resolve(in, L.vec3_bytes(), nm);
+        if (!aux1 || !xsf || !db) return 0;
+        if (!launch_d2f(db, aux1, 3 * L.sizeh)) return 0;
+        float target = 8e-7f * sqrtf((float)L.sizeh);
+        if (target < res) target = (float)res;
+        MsBufs B;
+        B.u = resolve(ip.u_f, 8 * L.su3f_bytes(), nm);
+        B.ph = resolve(pars->phases_f, 8 * L.fsoa_bytes(), nm);
+        B.b = aux1;
+        B.out = xsf;
+        B.shiftferm = resolve(ip.ferm_shift_temp_f, nv * L.vec3f_bytes(), nm);
+        B.r = resolve(ip.loc_r_f, L.vec3f_bytes(), nm);
+        B.h = resolve(ip.loc_h_f, L.vec3f_bytes(), nm);
+        B.s = resolve(ip.loc_s_f, L.vec3f_bytes(), nm);
+        B.p = resolve(ip.loc_p_f, L.vec3f_bytes(), nm);
+        void* dout = resolve(out, nv * L.vec3_bytes(), nm);
+        if (!B.u || !B.ph || !B.shiftferm || !B.r || !B.h || !B.s || !B.p || !dout) return 0;
+        st = multishift_solve<float>(B, pars->ferm_mass, order, approx->RA_b, (double)target, max_cg, &cg_return,
+                                     nullptr);
+        convergence_messages(convergence_importance, st);
+        total_iterations += cg_return;
+        for (int k = 0; k < order; k++) {
+            double bshift = approx->RA_b[k];
+            if (verbosity_lv > 3) printf("Shift %d, %f\n", k, bshift);
+            if (!launch_f2d((const char*)xsf + k * L.vec3f_bytes(), (char*)dout + k * L.vec3_bytes(), 3 * L.sizeh))
+                return 0;
+            if (!hip_ok(hipStreamSynchronize(ctx().stream), nm)) return 0;
+            st = inverter_wrapper(ip, pars, (vec3_soa*)((char*)out + k * L.vec3_bytes()), in, res, max_cg, bshift,
+                                  convergence_importance);
+            convergence_messages(convergence_importance, st);
+            // the reference adds the multishift's cg_return here (inverter_wrappers.c:96)
+            total_iterations += cg_return;
+        }
+    } else {
+        st = multishift_invert(ip.u, pars, approx, out, in, res, ip.loc_r, ip.loc_h, ip.loc_s, ip.loc_p,
+                               ip.ferm_shift_temp, max_cg, &cg_return);
+        convergence_messages(convergence_importance, st);
+        total_iterations += cg_return;
+    }
+    return total_iterations;
+}
+
+// Extended entry points returning the diagnostics the reference only prints.
+int ostp_cg_solve(const su3_soa* u, ferm_param* pars, vec3_soa* solution, const vec3_soa* in, double res,
+                  vec3_soa* loc_r, vec3_soa* loc_h, vec3_soa* loc_s, vec3_soa* loc_p, int max_cg, double shift,
+                  int restarting_every, int* cg_return, double* true_res) {
+    const char* nm = "ostp_cg_solve";
+    if (!require_init(nm)) return INVERTER_FAILURE;
+    const Lattice& L = ctx().lat;
+    const size_t vb = L.vec3_bytes();
+    CgBufs B;
+    B.u = resolve(u, 8 * L.su3_bytes(), nm);
+    B.ph = resolve(pars->phases, 8 * L.dsoa_bytes(), nm);
+    B.b = resolve(in, vb, nm);
+    B.x = resolve(solution, vb, nm);
+    B.r = resolve(loc_r, vb, nm);
+    B.h = resolve(loc_h, vb, nm);
+    B.s = resolve(loc_s, vb, nm);
+    B.p = resolve(loc_p, vb, nm);
+    if (!B.u || !B.ph || !B.b || !B.x || !B.r || !B.h || !B.s || !B.p) return INVERTER_FAILURE;
+    return cg_solve(B, pars->ferm_mass, shift, res, max_cg, restarting_every, cg_return, true_res);
+}
+
+int ostp_mixed_solve(inverter_package ip, ferm_param* pars, vec3_soa* solution, const vec3_soa* in, double res,
+                     int max_cg, double shift, double mixed_delta, int* cg_return, int* reliable_updates,
+                     double* true_res) {
+    const char* nm = "ostp_mixed_solve";
+    if (!require_init(nm)) return INVERTER_FAILURE;
+    const Lattice& L = ctx().lat;
+    const size_t vb = L.vec3_bytes(), vbf = L.vec3f_bytes();
+    MxBufs B;
+    B.u = resolve(ip.u, 8 * L.su3_bytes(), nm);
+    B.u_f = resolve(ip.u_f, 8 * L.su3f_bytes(), nm);
+    B.ph = resolve(pars->phases, 8 * L.dsoa_bytes(), nm);
+    B.ph_f = resolve(pars->phases_f, 8 * L.fsoa_bytes(), nm);
+    B.b = resolve(in, vb, nm);
+    B.x = resolve(solution, vb, nm);
+    B.r = resolve(ip.loc_r, vb, nm);
+    B.h = resolve(ip.loc_h, vb, nm);
+    B.s = resolve(ip.loc_s, vb, nm);
+    B.rf = resolve(ip.loc_r_f, vbf, nm);
+    B.hf = resolve(ip.loc_h_f, vbf, nm);
+    B.sf = resolve(ip.loc_s_f, vbf, nm);
+    B.pf = resolve(ip.loc_p_f, vbf, nm);
+    B.outf = resolve(ip.out_f, vbf, nm);
+    if (!B.u || !B.u_f || !B.ph || !B.ph_f || !B.b || !B.x || !B.r || !B.h || !B.s || !B.rf || !B.hf || !B.sf ||
+        !B.pf || !B.outf)
+        return INVERTER_FAILURE;
+    return mixed_solve(B, pars->ferm_mass, shift, res, max_cg, mixed_delta, cg_return, reliable_updates, true_res);
+}
+
+int ostp_multishift_solve(const su3_soa* u, ferm_param* pars, RationalApprox* approx, vec3_soa* out,
+                          const vec3_soa* in, double residuo, vec3_soa* loc_r, vec3_soa* loc_h, vec3_soa* loc_s,
+                          vec3_soa* loc_p, vec3_soa* shiftferm, int max_cg, int single, int* cg_return,
+                          double* shift_res) {
+    const char* nm = "ostp_multishift_solve";
+    if (!require_init(nm)) return INVERTER_FAILURE;
+    const int order = approx->approx_order;
+    if (order < 0 || order > kMaxShifts) return fail("%s: approx_order %d", nm, order) ? 0 : INVERTER_FAILURE;
+    const Lattice& L = ctx().lat;
+    const size_t vb = single ? L.vec3f_bytes() : L.vec3_bytes();
+    const size_t nv = order > 0 ? (size_t)order : 1;
+    MsBufs B;
+    B.u = resolve(u, 8 * (single ? L.su3f_bytes() : L.su3_bytes()), nm);
+    B.ph = resolve(single ? (const void*)pars->phases_f : (const void*)pars->phases,
+                   8 * (single ? L.fsoa_bytes() : L.dsoa_bytes()), nm);
+    B.b = resolve(in, vb, nm);
+    B.out = resolve(out, nv * vb, nm);
+    B.shiftferm = resolve(shiftferm, nv * vb, nm);
+    B.r = resolve(loc_r, vb, nm);
+    B.h = resolve(loc_h, vb, nm);
+    B.s = resolve(loc_s, vb, nm);
+    B.p = resolve(loc_p, vb, nm);
+    if (!B.u || !B.ph || !B.b || !B.out || !B.shiftferm || !B.r || !B.h || !B.s || !B.p) return INVERTER_FAILURE;
+    return single ? multishift_solve<float>(B, pars->ferm_mass, order, approx->RA_b, residuo, max_cg, cg_return,
+                                            shift_res)
+                  : multishift_solve<double>(B, pars->ferm_mass, order, approx->RA_b, residuo, max_cg, cg_return,
+                                             shift_res);
+}
+
+}  // extern "C"

@@ -1,0 +1,60 @@
+/*
+ * Test driver around the reference's own FP64 single-rank sources -- TEST
+ * INFRASTRUCTURE ONLY.  Built by oracle/ref.mk from the files where they lie
+ * under the reference tree (nothing is copied); output goes to oracle/_ref/.
+ *
+ * The reference's host program normally owns the globals below (its input
+ * parser and allocator, which need MPI / generated FP32 headers and so do not
+ * build here); this driver plays that role with plain values.  It exposes
+ * small C functions that tests/ call through ctypes to cross-check the oracle.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "OpenAcc/geometry.h"
+#include "OpenAcc/struct_c_def.h"
+#include "OpenAcc/backfield.h"
+#include "OpenAcc/action.h"
+#include "Include/debug.h"
+#include "OpenAcc/alloc_vars.h"
+#include "OpenAcc/fermionic_utilities.h"
+#include "RationalApprox/rationalapprox.h"
+
+int verbosity_lv = 0;
+action_param act_params;
+debug_settings_t debug_settings;
+su3_soa *auxbis_conf_acc;
+su3_soa *glocal_staples;
+tamat_soa *gipdot;
+
+long ref_sizeh(void) { return sizeh; }
+int ref_nd(int i) { int nd[4] = {nd0, nd1, nd2, nd3}; return nd[i]; }
+
+/* compute_nnp_and_nnm_openacc (OpenAcc/geometry.c:14-100) */
+void ref_nn(int *nnp, int *nnm)
+{
+    compute_nnp_and_nnm_openacc();
+    memcpy(nnp, nnp_openacc, sizeof(nnp_openacc));
+    memcpy(nnm, nnm_openacc, sizeof(nnm_openacc));
+}
+
+int ref_snum(int d0, int d1, int d2, int d3) { return snum_acc(d0, d1, d2, d3); }
+
+/* calc_u1_phases (OpenAcc/backfield.c:180-187) */
+void ref_phases(double *out, const int *map, const double *bf, double im_chem_pot, double charge)
+{
+    int nd[4] = {nd0, nd1, nd2, nd3};
+    geom_par.xmap = map[0]; geom_par.ymap = map[1]; geom_par.zmap = map[2]; geom_par.tmap = map[3];
+    geom_par.gnx = nd[map[0]]; geom_par.gny = nd[map[1]]; geom_par.gnz = nd[map[2]]; geom_par.gnt = nd[map[3]];
+    bf_param b;
+    b.ex = bf[0]; b.ey = bf[1]; b.ez = bf[2]; b.bx = bf[3]; b.by = bf[4]; b.bz = bf[5];
+    calc_u1_phases((double_soa *)out, b, im_chem_pot, charge);
+}
+
+/* reductions (OpenAcc/fermionic_utilities.c:126-175) */
+double ref_l2norm2(const void *v) { return l2norm2_global((const vec3_soa *)v); }
+double ref_real_scal_prod(const void *a, const void *b) { return real_scal_prod_global((const vec3_soa *)a, (const vec3_soa *)b); }
+
+/* rational approximations (RationalApprox/rationalapprox.c:83-236) */
+int ref_ratapprox_read(RationalApprox *ra, char *name) { return rationalapprox_read_custom_nomefile(ra, name); }
+double ref_ratapprox_evaluate(RationalApprox *ra, double x) { return rational_approx_evaluate(ra, x); }

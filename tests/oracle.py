@@ -1,0 +1,227 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) -- test infrastructure.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
+Buffers are numpy arrays in the reference layout (see openstaple_amd.Lattice).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+LIB_PATH = ORACLE_DIR / "liboracle.so"
+
+
+def build_oracle() -> pathlib.Path:
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR), "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+if not LIB_PATH.exists():
+    build_oracle()
+
+lib = C.CDLL(str(LIB_PATH))
+
+
+class Geom(C.Structure):
+    _fields_ = [("nd", C.c_int * 4), ("sizeh", C.c_long), ("par_temp", C.c_int)]
+
+
+class Tricks(C.Structure):
+    _fields_ = [("restarting_every", C.c_int), ("mixed_precision_delta", C.c_double)]
+
+
+class RatApprox(C.Structure):
+    _fields_ = [
+        ("exponent_num", C.c_int), ("exponent_den", C.c_int), ("approx_order", C.c_int),
+        ("lambda_min", C.c_double), ("lambda_max", C.c_double), ("gmp_remez_precision", C.c_int),
+        ("error", C.c_double), ("RA_a0", C.c_double),
+        ("RA_a", C.c_double * 25), ("RA_b", C.c_double * 25),
+    ]
+
+
+_P, _I, _D = C.c_void_p, C.c_int, C.c_double
+_G = C.POINTER(Geom)
+for name, res, args in [
+    ("or_geom_init", None, [_G, _I, _I, _I, _I, _I]),
+    ("or_snum", _I, [_G, _I, _I, _I, _I]),
+    ("or_nn_tables", None, [_G, _P, _P]),
+    ("or_calc_u1_phases", None, [_G, _P, _P, _D, _D, _P]),
+    ("or_phases_to_float", None, [_G, _P, _P]),
+    ("or_acc_Deo", None, [_G, _P, _P, _P, _P]),
+    ("or_acc_Doe", None, [_G, _P, _P, _P, _P]),
+    ("or_acc_Deo_f", None, [_G, _P, _P, _P, _P]),
+    ("or_acc_Doe_f", None, [_G, _P, _P, _P, _P]),
+    ("or_fermion_matrix_multiplication_shifted", None, [_G, _P, _P, _P, _P, _P, _D, _D]),
+    ("or_l2norm2", _D, [_G, _P]),
+    ("or_real_scal_prod", _D, [_G, _P, _P]),
+    ("or_ker_invert", _I, [_G, _P, _P, _D, _P, _P, _D, _P, _P, _P, _P, _I, _D, C.POINTER(Tricks), C.POINTER(_I)]),
+    ("or_inverter_mixed_precision", _I,
+     [_G, _P, _P, _P, _P, _D, _P, _P, _D, _I, _D, C.POINTER(Tricks), C.POINTER(_I)]),
+    ("or_multishift_invert", _I, [_G, _P, _P, _D, _I, _P, _P, _P, _D, _P, _P, _P, _P, _P, _I, C.POINTER(_I)]),
+    ("or_recombine", None, [_G, _P, _P, _P, _I, _D, _P]),
+    ("or_vec3_d2f", None, [_G, _P, _P]),
+    ("or_su3_d2f", None, [_G, _P, _P, _I]),
+    ("or_philox4x32_10", None, [_P, _P, _P]),
+    ("or_gen_vec3_gauss", None, [_G, _P, C.c_ulonglong, C.c_uint]),
+    ("or_gen_vec3_z2", None, [_G, _P, C.c_ulonglong, C.c_uint]),
+    ("or_gen_conf_cold", None, [_G, _P, _D, C.c_ulonglong]),
+    ("or_gen_conf_expH", None, [_G, _P, _D, C.c_ulonglong]),
+    ("or_ratapprox_read", _I, [C.POINTER(RatApprox), C.c_char_p]),
+    ("or_ratapprox_evaluate", _D, [C.POINTER(RatApprox), _D]),
+    ("or_ratapprox_rescale", None, [C.POINTER(RatApprox), C.POINTER(RatApprox), _P]),
+]:
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def p(a: np.ndarray) -> int:
+    assert a.flags.c_contiguous
+    return a.ctypes.data
+
+
+class Oracle:
+    """Oracle bound to one lattice shape."""
+
+    def __init__(self, nd0, nd1, nd2, nd3, par_temp=False):
+        self.g = Geom()
+        lib.or_geom_init(C.byref(self.g), nd0, nd1, nd2, nd3, int(par_temp))
+        self.sizeh = self.g.sizeh
+        self.nd = (nd0, nd1, nd2, nd3)
+        self.par_temp = par_temp
+
+    @property
+    def gp(self):
+        return C.byref(self.g)
+
+    # containers in reference layout
+    def vec3(self, single=False):
+        return np.zeros((3, self.sizeh), dtype=np.complex64 if single else np.complex128)
+
+    def gauge(self, single=False):
+        per = (72 + (4 if self.par_temp else 0)) if single else (144 + (8 if self.par_temp else 0))
+        return np.zeros(8 * per * self.sizeh, dtype=np.uint8)
+
+    def phases(self, map4=(0, 1, 2, 3), bf=(0, 0, 0, 0, 0, 0), mu=0.0, charge=0.0):
+        ph = np.zeros((8, self.sizeh))
+        m = (C.c_int * 4)(*map4)
+        b = (C.c_double * 6)(*bf)
+        lib.or_calc_u1_phases(self.gp, m, b, mu, charge, p(ph))
+        return ph
+
+    def nn_tables(self):
+        nnp = np.zeros((self.sizeh, 4, 2), dtype=np.int32)
+        nnm = np.zeros((self.sizeh, 4, 2), dtype=np.int32)
+        lib.or_nn_tables(self.gp, p(nnp), p(nnm))
+        return nnp, nnm
+
+    def deo(self, u, inp, ph):
+        out = self.vec3()
+        lib.or_acc_Deo(self.gp, p(u), p(out), p(inp), p(ph))
+        return out
+
+    def doe(self, u, inp, ph):
+        out = self.vec3()
+        lib.or_acc_Doe(self.gp, p(u), p(out), p(inp), p(ph))
+        return out
+
+    def deo_f(self, u_f, inp, ph_f):
+        out = self.vec3(single=True)
+        lib.or_acc_Deo_f(self.gp, p(u_f), p(out), p(inp), p(ph_f))
+        return out
+
+    def doe_f(self, u_f, inp, ph_f):
+        out = self.vec3(single=True)
+        lib.or_acc_Doe_f(self.gp, p(u_f), p(out), p(inp), p(ph_f))
+        return out
+
+    def mdagm(self, u, inp, ph, mass, shift=0.0):
+        out, tmp = self.vec3(), self.vec3()
+        lib.or_fermion_matrix_multiplication_shifted(self.gp, p(u), p(out), p(inp), p(tmp), p(ph), mass, shift)
+        return out
+
+    def cg(self, u, ph, mass, b, res, max_cg=10000, shift=0.0, restart=10000, x0=None):
+        x = self.vec3() if x0 is None else x0.copy()
+        r, h, s, pp = (self.vec3() for _ in range(4))
+        tr = Tricks(restart, 0.1)
+        cg = C.c_int(0)
+        st = lib.or_ker_invert(self.gp, p(u), p(ph), mass, p(x), p(b), res, p(r), p(h), p(s), p(pp), max_cg, shift,
+                               C.byref(tr), C.byref(cg))
+        return st, cg.value, x
+
+    def mixed(self, u, u_f, ph, ph_f, mass, b, res, max_cg=10000, shift=0.0, delta=0.1, x0=None):
+        x = self.vec3() if x0 is None else x0.copy()
+        tr = Tricks(10000, delta)
+        cg = C.c_int(0)
+        st = lib.or_inverter_mixed_precision(self.gp, p(u), p(u_f), p(ph), p(ph_f), mass, p(x), p(b), res, max_cg,
+                                             shift, C.byref(tr), C.byref(cg))
+        return st, cg.value, x
+
+    def multishift(self, u, ph, mass, shifts, b, res, max_cg=10000):
+        order = len(shifts)
+        out = np.zeros((max(order, 1), 3, self.sizeh), dtype=np.complex128)
+        sf = np.zeros_like(out)
+        r, h, s, pp = (self.vec3() for _ in range(4))
+        sh = np.ascontiguousarray(shifts, dtype=np.float64)
+        cg = C.c_int(0)
+        st = lib.or_multishift_invert(self.gp, p(u), p(ph), mass, order, p(sh), p(out), p(b), res, p(r), p(h),
+                                      p(s), p(pp), p(sf), max_cg, C.byref(cg))
+        return st, cg.value, out[:order]
+
+    def recombine(self, xs, b, a0, a):
+        out = self.vec3()
+        aa = np.ascontiguousarray(a, dtype=np.float64)
+        xs = np.ascontiguousarray(xs)
+        lib.or_recombine(self.gp, p(xs), p(b), p(out), len(a), a0, p(aa))
+        return out
+
+    def to_float_gauge(self, u):
+        uf = self.gauge(single=True)
+        lib.or_su3_d2f(self.gp, p(u), p(uf), 8)
+        return uf
+
+    def gen_gauss(self, seed, stream=0):
+        v = self.vec3()
+        lib.or_gen_vec3_gauss(self.gp, p(v), seed, stream)
+        return v
+
+    def gen_z2(self, seed, stream=0):
+        v = self.vec3()
+        lib.or_gen_vec3_z2(self.gp, p(v), seed, stream)
+        return v
+
+    def gen_cold(self, eps, seed):
+        u = self.gauge()
+        lib.or_gen_conf_cold(self.gp, p(u), eps, seed)
+        return u
+
+    def gen_expH(self, beta, seed):
+        u = self.gauge()
+        lib.or_gen_conf_expH(self.gp, p(u), beta, seed)
+        return u
+
+    def links(self, u):
+        """(8,3,3,sizeh) complex view (par_temp False only)."""
+        assert not self.par_temp
+        return u.view(np.complex128).reshape(8, 3, 3, self.sizeh)
+
+
+def philox(ctr, key):
+    c = (C.c_uint * 4)(*ctr)
+    k = (C.c_uint * 2)(*key)
+    o = (C.c_uint * 4)()
+    lib.or_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def read_remez(path) -> RatApprox:
+    ra = RatApprox()
+    rc = lib.or_ratapprox_read(C.byref(ra), str(path).encode())
+    if rc != 0:
+        raise ValueError(f"or_ratapprox_read({path}) -> {rc}")
+    return ra

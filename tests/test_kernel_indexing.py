@@ -1,0 +1,64 @@
+"""CPU checks of the index arithmetic the HIP kernels use, before any launch:
+the Dslash neighbour formula (k_dslash in openstaple_amd/csrc/dslash.hip) must
+reproduce the oracle's neighbour tables and stay in [0, sizeh), and the
+XCD-aware workgroup remap must be a bijection of [0, G)."""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import Oracle
+
+
+def kernel_neighbours(nd, eo):
+    """Vectorised restatement of the k_dslash index block for all threads."""
+    nd0, nd1, nd2, nd3 = nd
+    nd0h = nd0 // 2
+    sizeh = nd0 * nd1 * nd2 * nd3 // 2
+    idx = np.arange(sizeh, dtype=np.int64)
+    t = idx // nd0h
+    hd0 = idx - t * nd0h
+    t2 = t // nd1
+    d1 = t - t2 * nd1
+    d3 = t2 // nd2
+    d2 = t2 - d3 * nd2
+    d0 = 2 * hd0 + ((d1 + d2 + d3 + eo) & 1)
+    row = idx - hd0
+    s1, s2 = nd0h, nd0h * nd1
+    s3 = s2 * nd2
+    ip0 = row + np.where(d0 + 1 == nd0, 0, (d0 + 1) >> 1)
+    im0 = row + np.where(d0 == 0, (nd0 - 1) >> 1, (d0 - 1) >> 1)
+    ip1 = idx + np.where(d1 == nd1 - 1, -(nd1 - 1) * s1, s1)
+    im1 = idx + np.where(d1 == 0, (nd1 - 1) * s1, -s1)
+    ip2 = idx + np.where(d2 == nd2 - 1, -(nd2 - 1) * s2, s2)
+    im2 = idx + np.where(d2 == 0, (nd2 - 1) * s2, -s2)
+    ip3 = idx + np.where(d3 == nd3 - 1, -(nd3 - 1) * s3, s3)
+    im3 = idx + np.where(d3 == 0, (nd3 - 1) * s3, -s3)
+    return np.stack([ip0, ip1, ip2, ip3], 1), np.stack([im0, im1, im2, im3], 1), sizeh
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4, 4), (4, 6, 8, 10), (16, 4, 6, 2), (2, 2, 2, 2), (32, 32, 32, 32),
+                                   (24, 24, 24, 8), (48, 48, 48, 12)])
+@pytest.mark.parametrize("eo", [0, 1])
+def test_dslash_neighbours_match_tables(shape, eo):
+    ip, im, sizeh = kernel_neighbours(shape, eo)
+    assert ip.min() >= 0 and ip.max() < sizeh and im.min() >= 0 and im.max() < sizeh
+    if np.prod(shape) > 300000:
+        return  # bounds only; the table comparison runs on the smaller shapes
+    o = Oracle(*shape)
+    nnp, nnm = o.nn_tables()
+    par = 0 if eo == 0 else 1  # Deo writes even sites, Doe odd sites
+    np.testing.assert_array_equal(ip, nnp[:, :, par])
+    np.testing.assert_array_equal(im, nnm[:, :, par])
+
+
+def xcd_block_id(i, G):
+    q, r = G >> 3, G & 7
+    x, k = i & 7, i >> 3
+    return x * q + (x if x < r else r) + k
+
+
+@pytest.mark.parametrize("G", list(range(1, 70)) + [256, 257, 1000, 2048, 4095, 4096, 5185])
+def test_xcd_remap_is_a_bijection(G):
+    ids = sorted(xcd_block_id(i, G) for i in range(G))
+    assert ids == list(range(G))
