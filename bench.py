@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--dslash-only", action="store_true",
+                    help="only the timed Dslash loop and the copy kernel (for counter runs)")
     return ap.parse_args()
 
 
@@ -167,6 +169,9 @@ def main():
         },
     }
 
+    if args.dslash_only:
+        print(json.dumps(result))
+        return
     if not args.no_solves:
         result["solves"] = run_solves(ost, lib, L, u, ph, b, uf, phf)
 

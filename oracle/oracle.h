@@ -103,6 +103,22 @@ void or_gen_vec3_z2(const or_geom *g, void *v, unsigned long long seed, unsigned
 void or_gen_conf_cold(const or_geom *g, void *u, double eps, unsigned long long seed);
 void or_gen_conf_expH(const or_geom *g, void *u, double beta, unsigned long long seed);
 
+/* ---- gauge part (gauge.c): stout smearing, its force chain, unitarisation,
+ *      fermion-force building blocks.  tamat/thmat fields are 8 x 64*sizeh bytes. */
+void or_calc_loc_staples(const or_geom *g, const void *u, void *stap, double c_zero);
+void or_rho_times_conf_times_staples_ta(const or_geom *g, const void *u, const void *stap, void *qa,
+                                        double rho, double c_zero);
+void or_exp_minus_QA_times_conf(const or_geom *g, const void *u, const void *qa, void *uout);
+void or_stout_isotropic(const or_geom *g, const void *u, void *uout, double rho, double c_zero);
+void or_compute_lambda(const or_geom *g, void *L, const void *SP, const void *U, const void *QA);
+void or_compute_sigma(const or_geom *g, const void *L, const void *U, void *S, const void *QA, double rho);
+void or_sigma_prime_to_sigma(const or_geom *g, void *Sigma, void *Lambda, void *QA, const void *U, double rho,
+                             double c_zero);
+void or_unitarize_conf(const or_geom *g, void *u);
+void or_direct_product_into_auxmat(const or_geom *g, const void *s, const void *h, void *aux, double a);
+void or_multiply_backfield_times_force(const or_geom *g, const double *phases, const void *aux, void *out);
+void or_conf_times_force_ta(const or_geom *g, const void *u, const void *aux, void *ipdot);
+
 /* ---- rational approximations (RationalApprox/rationalapprox.c:83-236) */
 #define OR_MAX_APPROX_ORDER 25
 typedef struct {

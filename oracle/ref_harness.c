@@ -58,3 +58,46 @@ double ref_real_scal_prod(const void *a, const void *b) { return real_scal_prod_
 /* rational approximations (RationalApprox/rationalapprox.c:83-236) */
 int ref_ratapprox_read(RationalApprox *ra, char *name) { return rationalapprox_read_custom_nomefile(ra, name); }
 double ref_ratapprox_evaluate(RationalApprox *ra, double x) { return rational_approx_evaluate(ra, x); }
+
+/* ---- stout smearing and its force chain (OpenAcc/stouting.c, plaquettes.c,
+ * su3_utilities.c) with the reference's own kernels */
+#include "OpenAcc/stouting.h"
+#include "OpenAcc/su3_utilities.h"
+#include "OpenAcc/plaquettes.h"
+
+void ref_stout_isotropic(const void *u, void *uout, double rho)
+{
+    compute_nnp_and_nnm_openacc();
+    gl_stout_rho = rho;
+    su3_soa *stap = malloc(8 * sizeof(su3_soa)), *aux = malloc(8 * sizeof(su3_soa));
+    tamat_soa *ta = malloc(8 * sizeof(tamat_soa));
+    stout_isotropic((const su3_soa *)u, (su3_soa *)uout, stap, aux, ta, 0);
+    free(stap); free(aux); free(ta);
+}
+
+/* staples + rho*TA(U*staples) (plaquettes.c:196-259, su3_utilities.c:210-239) */
+void ref_staples_and_qa(const void *u, void *stap, void *qa, double rho)
+{
+    compute_nnp_and_nnm_openacc();
+    gl_stout_rho = rho;
+    set_su3_soa_to_zero((su3_soa *)stap);
+    calc_loc_staples_nnptrick_all_onlyferms((const su3_soa *)u, (su3_soa *)stap);
+    RHO_times_conf_times_staples_ta_part((const su3_soa *)u, (const su3_soa *)stap, (tamat_soa *)qa, 0);
+}
+
+/* the body of compute_sigma_from_sigma_prime_backinto_sigma_prime
+ * (fermion_force.c:95-160: staples, QA, compute_lambda, compute_sigma) */
+void ref_sigma_prime_to_sigma(void *Sigma, void *Lambda, void *QA, const void *U, double rho)
+{
+    compute_nnp_and_nnm_openacc();
+    gl_stout_rho = rho;
+    su3_soa *tmp = malloc(8 * sizeof(su3_soa));
+    set_su3_soa_to_zero(tmp);
+    calc_loc_staples_nnptrick_all_onlyferms((const su3_soa *)U, tmp);
+    RHO_times_conf_times_staples_ta_part((const su3_soa *)U, tmp, (tamat_soa *)QA, 0);
+    compute_lambda((thmat_soa *)Lambda, (const su3_soa *)Sigma, (const su3_soa *)U, (const tamat_soa *)QA, tmp);
+    compute_sigma((const thmat_soa *)Lambda, (const su3_soa *)U, (su3_soa *)Sigma, (const tamat_soa *)QA, tmp, 0);
+    free(tmp);
+}
+
+void ref_unitarize(void *u) { unitarize_conf((su3_soa *)u); }
