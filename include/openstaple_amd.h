@@ -184,6 +184,32 @@ int ostp_multishift_solve(const su3_soa *u, ferm_param *pars, RationalApprox *ap
                           vec3_soa *loc_s, vec3_soa *loc_p, vec3_soa *shiftferm, int max_cg, int single,
                           int *cg_return, double *shift_res);
 
+/* ======================================================== stout ========== */
+/* rho of the fermion / topological stouting (OpenAcc/action.h:23) */
+extern double gl_stout_rho;
+extern double gl_topo_rho;
+/* stout parameters: act_params.stout_rho/stout_steps/topo_* of the reference
+ * input (Include/setting_file_parser.c:416) and the staple normalisation
+ * C_ZERO of the gauge action (tlsm != 0: 5/3, tree-level Symanzik; else 1) */
+void ostp_set_stout(double rho, int steps, double topo_rho, int topo_steps, int topo_action, int tlsm);
+/* stouting.h:13-44 */
+void stout_wrapper(const su3_soa *tconf_acc, su3_soa *tstout_conf_acc_arr, const int istopo);
+void stout_isotropic(const su3_soa *u, su3_soa *uprime, su3_soa *local_staples, su3_soa *auxiliary,
+                     tamat_soa *tipdot, const int istopo);
+void exp_minus_QA_times_conf(const su3_soa *tu, const tamat_soa *QA, su3_soa *tu_out, su3_soa *exp_aux);
+void compute_lambda(thmat_soa *L, const su3_soa *SP, const su3_soa *U, const tamat_soa *QA, su3_soa *TMP);
+void compute_sigma(const thmat_soa *L, const su3_soa *U, su3_soa *S, const tamat_soa *QA, su3_soa *TMP,
+                   const int istopo);
+/* fermion_force.h:15-20 */
+void compute_sigma_from_sigma_prime_backinto_sigma_prime(su3_soa *Sigma, thmat_soa *Lambda, tamat_soa *QA,
+                                                          const su3_soa *U, su3_soa *TMP, const int istopo);
+/* plaquettes.h / su3_utilities.h */
+void calc_loc_staples_nnptrick_all_onlyferms(const su3_soa *u, su3_soa *loc_stap);
+void RHO_times_conf_times_staples_ta_part(const su3_soa *u, const su3_soa *loc_stap, tamat_soa *tipdot,
+                                          int istopo);
+/* su3_utilities.h:25 */
+void unitarize_conf(su3_soa *u);
+
 /* ======================================================== generators ===== */
 /* Reference field rules (OpenAcc/random_assignement.c:67-92,106-142,188-250)
  * drawn from a counter-based Philox4x32-10 stream keyed by `seed` instead of

@@ -161,6 +161,16 @@ _SIGS = {
     "ostp_generate_vec3_soa_z2noise": (None, [_P, C.c_uint64, C.c_uint32]),
     "ostp_generate_Conf_cold": (None, [_P, _D, C.c_uint64]),
     "ostp_generate_Conf_expH": (None, [_P, _D, C.c_uint64]),
+    "ostp_set_stout": (None, [_D, _I, _D, _I, _I, _I]),
+    "stout_wrapper": (None, [_P, _P, _I]),
+    "stout_isotropic": (None, [_P, _P, _P, _P, _P, _I]),
+    "exp_minus_QA_times_conf": (None, [_P, _P, _P, _P]),
+    "compute_lambda": (None, [_P, _P, _P, _P, _P]),
+    "compute_sigma": (None, [_P, _P, _P, _P, _P, _I]),
+    "compute_sigma_from_sigma_prime_backinto_sigma_prime": (None, [_P, _P, _P, _P, _P, _I]),
+    "calc_loc_staples_nnptrick_all_onlyferms": (None, [_P, _P]),
+    "RHO_times_conf_times_staples_ta_part": (None, [_P, _P, _P, _I]),
+    "unitarize_conf": (None, [_P]),
     "ostp_time_dslash": (_D, [_P, _P, _P, _P, _I, _I, _I]),
     "ostp_time_copy": (_D, [_P, _P, C.c_size_t, _I]),
 }

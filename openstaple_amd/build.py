@@ -2,7 +2,7 @@
 
 Compiles every openstaple_amd/csrc/*.hip with hipcc for gfx950 in parallel and
 links one shared library exporting the C ABI of include/openstaple_amd.h.
-Run as ``python -m openstaple_amd.build`` or through ``__graft_entry__.build()``.
+Run as ``python openstaple_amd/build.py`` or through ``__graft_entry__.build()``.
 """
 from __future__ import annotations
 

@@ -135,4 +135,90 @@ __device__ __forceinline__ void ch_exp_minus_qa(const Tamat& q, Su3& o) {
     itamat_poly_no3rdrow(f0, f1, f2, q, o);
 }
 
+// ---------------------------------------------------------- 3x3 algebra
+__device__ __forceinline__ void su3_zero(Su3& m) {
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) m.c[r][c] = make_double2(0.0, 0.0);
+}
+// o = a * b
+__device__ __forceinline__ void su3_mul(const Su3& a, const Su3& b, Su3& o) {
+    Su3 t;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double2 s = cmul(a.c[r][0], b.c[0][c]);
+            s = cfma(a.c[r][1], b.c[1][c], s);
+            s = cfma(a.c[r][2], b.c[2][c], s);
+            t.c[r][c] = s;
+        }
+    o = t;
+}
+// o = a * b^+
+__device__ __forceinline__ void su3_mul_dag(const Su3& a, const Su3& b, Su3& o) {
+    Su3 t;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double2 s = cmul(a.c[r][0], cconj(b.c[c][0]));
+            s = cfma(a.c[r][1], cconj(b.c[c][1]), s);
+            s = cfma(a.c[r][2], cconj(b.c[c][2]), s);
+            t.c[r][c] = s;
+        }
+    o = t;
+}
+// o = a^+ * b
+__device__ __forceinline__ void su3_dag_mul(const Su3& a, const Su3& b, Su3& o) {
+    Su3 t;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double2 s = cmulc(a.c[0][r], b.c[0][c]);
+            s = cfmac(a.c[1][r], b.c[1][c], s);
+            s = cfmac(a.c[2][r], b.c[2][c], s);
+            t.c[r][c] = s;
+        }
+    o = t;
+}
+// acc += s * m  (complex scalar)
+__device__ __forceinline__ void su3_axpy(double2 s, const Su3& m, Su3& acc) {
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc.c[r][c] = cfma(s, m.c[r][c], acc.c[r][c]);
+}
+__device__ __forceinline__ void su3_scale_add(double s, const Su3& m, Su3& acc) {
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            acc.c[r][c] = make_double2(fma(s, m.c[r][c].x, acc.c[r][c].x), fma(s, m.c[r][c].y, acc.c[r][c].y));
+}
+// full Hermitian matrix of a thmat
+__device__ __forceinline__ void th_full(const Thmat& l, Su3& o) {
+    o.c[0][0] = creal2(l.rc00);
+    o.c[0][1] = l.c01;
+    o.c[0][2] = l.c02;
+    o.c[1][0] = cconj(l.c01);
+    o.c[1][1] = creal2(l.rc11);
+    o.c[1][2] = l.c12;
+    o.c[2][0] = cconj(l.c02);
+    o.c[2][1] = cconj(l.c12);
+    o.c[2][2] = creal2(-l.rc00 - l.rc11);
+}
+// scaled traceless anti-Hermitian part (RHO_times_mat1_times_mat2_into_tamat3)
+__device__ __forceinline__ void ta_part(const Su3& X, double s, Tamat& q) {
+    const double tr = X.c[0][0].y + X.c[1][1].y + X.c[2][2].y;
+    const double third = 0.33333333333333333333333;
+    q.c01 = cscale(csub(X.c[0][1], cconj(X.c[1][0])), 0.5 * s);
+    q.c02 = cscale(csub(X.c[0][2], cconj(X.c[2][0])), 0.5 * s);
+    q.c12 = cscale(csub(X.c[1][2], cconj(X.c[2][1])), 0.5 * s);
+    q.ic00 = s * (X.c[0][0].y - third * tr);
+    q.ic11 = s * (X.c[1][1].y - third * tr);
+}
+
 }  // namespace ostp

@@ -1,0 +1,734 @@
+// Stout smearing and its force chain on gfx950 (FP64).
+//
+//  smearing   U' = exp(-QA) U, QA = (rho/c0) TA(U * C), C = c0 * sum of the 6
+//             staples (OpenAcc/stouting.c:27-165, plaquettes.c:196-259,
+//             su3_utilities.c:210-239; Morningstar-Peardon hep-lat/0311018)
+//  force      Sigma' -> Lambda -> Sigma (stouting.c:171-1302,
+//             fermion_force.c:52-162; MP eqs. 57-75)
+//
+// One thread per link (8 link arrays x sizeh half-sites); the staple loop
+// re-reads neighbouring links, which the L2 of the owning XCD serves
+// (workgroups are remapped so each XCD walks a contiguous slab).  With about
+// 2.5 kflop per link against 192 B of compulsory link traffic these kernels sit
+// near the FP64 ridge; everything stays in FP64 as the reference does.
+#include "ostp_blas.h"
+#include "ostp_su3.h"
+
+#include <cmath>
+
+extern "C" {
+// action.h:23 and the staple normalisation of the gauge action (common_defines.h:71-84)
+double gl_stout_rho = 0.15;
+double gl_topo_rho = 0.0;
+}
+
+namespace ostp {
+
+static double g_c_zero = 5.0 * 0.33333333333333333333333;  // tree-level Symanzik C_ZERO
+static int g_stout_steps = 2;
+static int g_topo_steps = 0;
+static int g_topo_action = 0;
+
+struct LinkGeom {
+    int nd[4];
+    int sizeh;
+};
+
+struct GaugePtrs {
+    const double2* u[8];  // r0.c0 plane of each link array; rows at +3k*sizeh, cols at +c*sizeh
+};
+
+__device__ __forceinline__ void decode(int idxh, int parity, const LinkGeom& g, int d[4]) {
+    const int nd0h = g.nd[0] >> 1;
+    int t = idxh / nd0h;
+    const int hd0 = idxh - t * nd0h;
+    int t2 = t / g.nd[1];
+    d[1] = t - t2 * g.nd[1];
+    d[3] = t2 / g.nd[2];
+    d[2] = t2 - d[3] * g.nd[2];
+    d[0] = 2 * hd0 + ((d[1] + d[2] + d[3] + parity) & 1);
+}
+
+__device__ __forceinline__ int snum(const int d[4], const LinkGeom& g) {
+    return (d[0] + g.nd[0] * (d[1] + g.nd[1] * (d[2] + g.nd[2] * d[3]))) >> 1;
+}
+
+// half-site index of x + s*e_mu (+ t*e_nu), periodic
+__device__ __forceinline__ int shifted(const int d[4], int mu, int s, int nu, int t, const LinkGeom& g) {
+    int e[4] = {d[0], d[1], d[2], d[3]};
+    e[mu] += s;
+    if (e[mu] < 0) e[mu] += g.nd[mu];
+    if (e[mu] >= g.nd[mu]) e[mu] -= g.nd[mu];
+    if (t != 0) {
+        e[nu] += t;
+        if (e[nu] < 0) e[nu] += g.nd[nu];
+        if (e[nu] >= g.nd[nu]) e[nu] -= g.nd[nu];
+    }
+    return snum(e, g);
+}
+
+__device__ __forceinline__ void load_link(const double2* __restrict__ U, int sizeh, int i, Su3& m) {
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) m.c[r][c] = U[(3 * r + c) * sizeh + i];
+    rebuild3row(m);
+}
+__device__ __forceinline__ void load_gl3(const double2* __restrict__ U, int sizeh, int i, Su3& m) {
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) m.c[r][c] = U[(3 * r + c) * sizeh + i];
+}
+__device__ __forceinline__ void store_rows(double2* __restrict__ U, int sizeh, int i, const Su3& m, int rows) {
+    for (int r = 0; r < rows; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) U[(3 * r + c) * sizeh + i] = m.c[r][c];
+}
+
+// tamat / thmat soa: c01,c02,c12 complex planes then two real planes
+struct TaPtr {
+    double2* c;   // 3 planes
+    double* d;    // 2 planes
+};
+__device__ __forceinline__ TaPtr ta_at(void* base, int link, int sizeh) {
+    char* b = (char*)base + (size_t)link * 64 * sizeh;
+    return TaPtr{(double2*)b, (double*)(b + (size_t)48 * sizeh)};
+}
+__device__ __forceinline__ void ta_load(const void* base, int link, int sizeh, int i, Tamat& q) {
+    TaPtr p = ta_at(const_cast<void*>(base), link, sizeh);
+    q.c01 = p.c[i];
+    q.c02 = p.c[sizeh + i];
+    q.c12 = p.c[2 * sizeh + i];
+    q.ic00 = p.d[i];
+    q.ic11 = p.d[sizeh + i];
+}
+__device__ __forceinline__ void ta_store(void* base, int link, int sizeh, int i, const Tamat& q) {
+    TaPtr p = ta_at(base, link, sizeh);
+    p.c[i] = q.c01;
+    p.c[sizeh + i] = q.c02;
+    p.c[2 * sizeh + i] = q.c12;
+    p.d[i] = q.ic00;
+    p.d[sizeh + i] = q.ic11;
+}
+__device__ __forceinline__ void th_load(const void* base, int link, int sizeh, int i, Thmat& l) {
+    TaPtr p = ta_at(const_cast<void*>(base), link, sizeh);
+    l.c01 = p.c[i];
+    l.c02 = p.c[sizeh + i];
+    l.c12 = p.c[2 * sizeh + i];
+    l.rc00 = p.d[i];
+    l.rc11 = p.d[sizeh + i];
+}
+__device__ __forceinline__ void th_store(void* base, int link, int sizeh, int i, const Thmat& l) {
+    TaPtr p = ta_at(base, link, sizeh);
+    p.c[i] = l.c01;
+    p.c[sizeh + i] = l.c02;
+    p.c[2 * sizeh + i] = l.c12;
+    p.d[i] = l.rc00;
+    p.d[sizeh + i] = l.rc11;
+}
+
+__device__ __constant__ int c_perp[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+
+// sum of the 6 staples of link (x, mu) times c0 (plaquettes.c:196-259)
+__device__ __forceinline__ void staple_sum(const GaugePtrs& G, const LinkGeom& g, const int d[4], int idxh, int mu,
+                                           int parity, double c0, Su3& acc) {
+    su3_zero(acc);
+    const int sh = g.sizeh;
+    const int i_pmu = shifted(d, mu, 1, 0, 0, g);
+    for (int it = 0; it < 3; it++) {
+        const int nu = c_perp[mu][it];
+        Su3 A, B, C, T;
+        // right: U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+        load_link(G.u[2 * nu + !parity], sh, i_pmu, A);
+        load_link(G.u[2 * mu + !parity], sh, shifted(d, nu, 1, 0, 0, g), B);
+        load_link(G.u[2 * nu + parity], sh, idxh, C);
+        su3_mul_dag(A, B, T);
+        su3_mul_dag(T, C, T);
+        rebuild3row(T);
+        su3_scale_add(c0, T, acc);
+        // left: U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
+        const int i_mnu = shifted(d, nu, -1, 0, 0, g);
+        load_link(G.u[2 * nu + parity], sh, shifted(d, mu, 1, nu, -1, g), A);
+        load_link(G.u[2 * mu + !parity], sh, i_mnu, B);
+        load_link(G.u[2 * nu + !parity], sh, i_mnu, C);
+        Su3 AB;
+        su3_mul(B, A, AB);      // A^+ B^+ = (B A)^+
+        su3_dag_mul(AB, C, T);  // (B A)^+ C
+        rebuild3row(T);
+        su3_scale_add(c0, T, acc);
+    }
+}
+
+__device__ __forceinline__ void link_of(int gid, int sizeh, int& dir, int& idxh) {
+    dir = gid / sizeh;
+    idxh = gid - dir * sizeh;
+}
+
+// Fused smearing: U'(rows 0,1) = exp(-QA) U; optional side outputs of the
+// reference's parking arrays (staples, QA, exp rows 0,1).
+__global__ __launch_bounds__(kBlock) void k_stout(GaugePtrs G, double2* __restrict__ uout, size_t su3_stride,
+                                                 LinkGeom g, double rho, double c0, double2* __restrict__ stap_out,
+                                                 void* __restrict__ qa_out, double2* __restrict__ exp_out) {
+    const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
+    if (gid >= 8 * g.sizeh) return;
+    int dir, idxh;
+    link_of(gid, g.sizeh, dir, idxh);
+    const int mu = dir >> 1, parity = dir & 1;
+    int d[4];
+    decode(idxh, parity, g, d);
+    Su3 S, U, X, E, O;
+    staple_sum(G, g, d, idxh, mu, parity, c0, S);
+    load_link(G.u[dir], g.sizeh, idxh, U);
+    su3_mul(U, S, X);
+    Tamat q;
+    ta_part(X, rho / c0, q);
+    ch_exp_minus_qa(q, E);
+    // rows 0,1 of E * U
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double2 s = cmul(E.c[r][0], U.c[0][c]);
+            s = cfma(E.c[r][1], U.c[1][c], s);
+            s = cfma(E.c[r][2], U.c[2][c], s);
+            O.c[r][c] = s;
+        }
+    double2* Uo = (double2*)((char*)uout + dir * su3_stride);
+    store_rows(Uo, g.sizeh, idxh, O, 2);
+    if (stap_out) store_rows((double2*)((char*)stap_out + dir * su3_stride), g.sizeh, idxh, S, 3);
+    if (qa_out) ta_store(qa_out, dir, g.sizeh, idxh, q);
+    if (exp_out) store_rows((double2*)((char*)exp_out + dir * su3_stride), g.sizeh, idxh, E, 2);
+}
+
+// staples accumulated into `stap` (calc_loc_staples_nnptrick_all_onlyferms adds)
+__global__ __launch_bounds__(kBlock) void k_staples(GaugePtrs G, double2* __restrict__ stap, size_t su3_stride,
+                                                   LinkGeom g, double c0) {
+    const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
+    if (gid >= 8 * g.sizeh) return;
+    int dir, idxh;
+    link_of(gid, g.sizeh, dir, idxh);
+    int d[4];
+    decode(idxh, dir & 1, g, d);
+    Su3 S, old;
+    staple_sum(G, g, d, idxh, dir >> 1, dir & 1, c0, S);
+    double2* P = (double2*)((char*)stap + dir * su3_stride);
+    load_gl3(P, g.sizeh, idxh, old);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) old.c[r][c] = cadd(old.c[r][c], S.c[r][c]);
+    store_rows(P, g.sizeh, idxh, old, 3);
+}
+
+// QA = (rho/c0) TA(U * stap) (RHO_times_conf_times_staples_ta_part)
+__global__ __launch_bounds__(kBlock) void k_rho_ta(GaugePtrs G, const double2* __restrict__ stap, size_t su3_stride,
+                                                  void* __restrict__ qa, LinkGeom g, double rho, double c0) {
+    const int gid = blockIdx.x * kBlock + (int)threadIdx.x;
+    if (gid >= 8 * g.sizeh) return;
+    int dir, idxh;
+    link_of(gid, g.sizeh, dir, idxh);
+    Su3 U, S, X;
+    load_link(G.u[dir], g.sizeh, idxh, U);
+    load_gl3((const double2*)((const char*)stap + dir * su3_stride), g.sizeh, idxh, S);
+    su3_mul(U, S, X);
+    Tamat q;
+    ta_part(X, rho / c0, q);
+    ta_store(qa, dir, g.sizeh, idxh, q);
+}
+
+// U_out(rows 0,1) = exp(-QA) U (exp_minus_QA_times_conf); exp rows 0,1 to exp_aux
+__global__ __launch_bounds__(kBlock) void k_exp_times_conf(GaugePtrs G, const void* __restrict__ qa,
+                                                          double2* __restrict__ uout, double2* __restrict__ exp_aux,
+                                                          size_t su3_stride, LinkGeom g) {
+    const int gid = blockIdx.x * kBlock + (int)threadIdx.x;
+    if (gid >= 8 * g.sizeh) return;
+    int dir, idxh;
+    link_of(gid, g.sizeh, dir, idxh);
+    Tamat q;
+    ta_load(qa, dir, g.sizeh, idxh, q);
+    Su3 E, U, O;
+    ch_exp_minus_qa(q, E);
+    load_link(G.u[dir], g.sizeh, idxh, U);
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double2 s = cmul(E.c[r][0], U.c[0][c]);
+            s = cfma(E.c[r][1], U.c[1][c], s);
+            s = cfma(E.c[r][2], U.c[2][c], s);
+            O.c[r][c] = s;
+        }
+    store_rows((double2*)((char*)uout + dir * su3_stride), g.sizeh, idxh, O, 2);
+    if (exp_aux) store_rows((double2*)((char*)exp_aux + dir * su3_stride), g.sizeh, idxh, E, 2);
+}
+
+// ------------------------------------------------------------- force chain
+// Lambda of one link (compute_loc_Lambda, stouting.c:171-512): the derivative
+// coefficients b_{1j}, b_{2j} of Morningstar-Peardon eqs. 57-58 (small-c1
+// series as in NISSA), B_k = b_k0 + b_k1 Q + b_k2 Q^2, tr_k = Tr(Sigma' B_k U),
+// Gamma = tr1 Q + tr2 Q^2 + f1 U Sigma' + f2 (Q U Sigma' + U Sigma' Q),
+// Lambda = (Gamma + Gamma^+)/2 - Tr(Gamma + Gamma^+)/6.
+__device__ void lambda_local(const Tamat& q, const Su3& SP, const Su3& U, Thmat& L) {
+    double c0 = det_i_times_qa(q);
+    const double c1 = 0.5 * tr_i_times_qa_sq(q);
+    const double c0max = 2 * pow(c1 / 3, 1.5);
+    double2 f0, f1, f2, b10, b11, b12, b20, b21, b22;
+    if (c1 < 4e-3) {
+        ch_coefficients(q, f0, f1, f2);
+        b10 = make_double2(0.0, c0 / 120 * (1 - c1 / 21));
+        b11 = make_double2(-c0 / 360 * (1 - 3 * c1 / 56), -1.0 / 6 * (1 - c1 / 10 * (1.0 - c1 / 28)));
+        b12 = make_double2(0.5 * (1.0 / 12 * (1 - 2 * c1 / 30 * (1 - 3 * c1 / 112))), 0.5 * (-c0 / 1260 * (1 - c1 / 24)));
+        b20 = make_double2(-c0 / 360, -1.0 / 6 * (1 - c1 / 20 * (1 - c1 / 42)));
+        b21 = make_double2(1.0 / 24 * (1 - c1 / 15 * (1 - 3 * c1 / 112)), -c0 / 2520);
+        b22 = make_double2(0.5 * c0 / 10080, 0.5 * (1.0 / 60 * (1 - c1 / 21 * (1 - c1 / 48))));
+    } else {
+        int sign = 1;
+        if (c0 < 0) {
+            sign = -1;
+            c0 = -c0;
+        }
+        const double eps = (c0max - c0) / c0max;
+        double theta;
+        if (eps < 0) theta = 0.0;
+        else if (eps < 1e-3)
+            theta = sqrt(2 * eps) *
+                    (1 + (1.0 / 12 + (3.0 / 160 + (5.0 / 896 + (35.0 / 18432 + 63.0 / 90112 * eps) * eps) * eps) * eps) * eps);
+        else theta = acos(c0 / c0max);
+        const double u = sqrt(c1 / 3) * cos(theta / 3);
+        const double w = sqrt(c1) * sin(theta / 3);
+        const double u2 = u * u, w2 = w * w, u2mw2 = u2 - w2, w2p3u2 = w2 + 3 * u2, w2m3u2 = w2 - 3 * u2;
+        const double cu = cos(u), c2u = cos(2 * u), su = sin(u), s2u = sin(2 * u), cw = cos(w);
+        double xi0w, xi1w;
+        if (fabs(w) < 0.05) {
+            const double t0 = w * w, t1 = 1 - t0 / 42, t2 = 1.0 - t0 / 20 * t1;
+            xi0w = 1 - t0 / 6 * t2;
+            xi1w = -(1 - w2 * (1 - w2 * (1 - w2 / 54) / 28) / 10) / 3;
+        } else {
+            xi0w = sin(w) / w;
+            xi1w = cw / w2 - sin(w) / (w2 * w);
+        }
+        const double denom = 1 / (9 * u * u - w * w);
+        f0 = make_double2((u2mw2 * c2u + cu * 8 * u2 * cw + 2 * su * u * w2p3u2 * xi0w) * denom,
+                          (u2mw2 * s2u - su * 8 * u2 * cw + cu * 2 * u * w2p3u2 * xi0w) * denom);
+        f1 = make_double2((2 * u * c2u - cu * 2 * u * cw - su * w2m3u2 * xi0w) * denom,
+                          (2 * u * s2u + su * 2 * u * cw - cu * w2m3u2 * xi0w) * denom);
+        f2 = make_double2((c2u - cu * cw - 3 * su * u * xi0w) * denom, (s2u + su * cw - cu * 3 * u * xi0w) * denom);
+        const double2 r01 = make_double2(
+            2 * c2u * u + s2u * (-2 * u2 + 2 * w2) + 2 * cu * u * (8 * cw + 3 * u2 * xi0w + w2 * xi0w) +
+                su * (-8 * cw * u2 + 18 * u2 * xi0w + 2 * w2 * xi0w),
+            -8 * cw * (2 * su * u + cu * u2) + 2 * (s2u * u + c2u * u2 - c2u * w2) +
+                2 * (9 * cu * u2 - 3 * su * u * u2 + cu * w2 - su * u * w2) * xi0w);
+        const double2 r11 = make_double2(2 * c2u - 4 * s2u * u + su * (2 * cw * u + 6 * u * xi0w) +
+                                             cu * (-2 * cw + 3 * u2 * xi0w - w2 * xi0w),
+                                         2 * s2u + 4 * c2u * u + 2 * cw * (su + cu * u) + (6 * cu * u - 3 * su * u2 + su * w2) * xi0w);
+        const double2 r21 = make_double2(-2 * s2u + cw * su - 3 * (su + cu * u) * xi0w,
+                                         2 * c2u + cu * cw + (-3 * cu + 3 * su * u) * xi0w);
+        const double2 r02 = make_double2(-2 * c2u + 2 * cw * su * u + 2 * su * u * xi0w - 8 * cu * u2 * xi0w +
+                                             6 * su * u * u2 * xi1w,
+                                         2 * (-s2u + 4 * su * u2 * xi0w + cu * u * (cw + xi0w + 3 * u2 * xi1w)));
+        const double2 r12 = make_double2(2 * cu * u * xi0w + su * (-cw - xi0w + 3 * u2 * xi1w),
+                                         -2 * su * u * xi0w - cu * (cw + xi0w - 3 * u2 * xi1w));
+        const double2 r22 = make_double2(cu * xi0w - 3 * su * u * xi1w, -(su * xi0w) - 3 * cu * u * xi1w);
+        const double h = 0.5 * denom * denom;
+        const double a1 = 2.0 * u, a2 = 3.0 * u * u - w * w, a3 = -2.0 * (15.0 * u * u + w * w);
+        auto B1 = [&](double2 r1, double2 r2, double2 f) {
+            return make_double2(h * (a1 * r1.x + a2 * r2.x + a3 * f.x), h * (a1 * r1.y + a2 * r2.y + a3 * f.y));
+        };
+        auto B2 = [&](double2 r1, double2 r2, double2 f) {
+            return make_double2(h * (r1.x - 3.0 * u * r2.x - 24.0 * u * f.x), h * (r1.y - 3.0 * u * r2.y - 24.0 * u * f.y));
+        };
+        b10 = B1(r01, r02, f0);
+        b11 = B1(r11, r12, f1);
+        b12 = B1(r21, r22, f2);
+        b20 = B2(r01, r02, f0);
+        b21 = B2(r11, r12, f1);
+        b22 = B2(r21, r22, f2);
+        if (sign == -1) {
+            b10 = cconj(b10);
+            b11 = make_double2(-b11.x, b11.y);
+            b12 = cconj(b12);
+            b20 = make_double2(-b20.x, b20.y);
+            b21 = cconj(b21);
+            b22 = make_double2(-b22.x, b22.y);
+            f0 = cconj(f0);
+            f1 = make_double2(-f1.x, f1.y);
+            f2 = cconj(f2);
+        }
+    }
+    const double2 z = make_double2(0.0, 0.0), one = make_double2(1.0, 0.0);
+    Su3 Bm, Q, Q2, T, USP, QU, UQ;
+    itamat_poly_no3rdrow(z, one, z, q, Q);
+    itamat_poly_row2(z, one, z, q, Q);
+    itamat_poly_no3rdrow(z, z, one, q, Q2);
+    itamat_poly_row2(z, z, one, q, Q2);
+    // tr_k = Tr(Sigma' B_k U)
+    double2 tr1 = z, tr2 = z;
+    itamat_poly_no3rdrow(b10, b11, b12, q, Bm);
+    itamat_poly_row2(b10, b11, b12, q, Bm);
+    su3_mul(Bm, U, T);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) tr1 = cfma(SP.c[a][b], T.c[b][a], tr1);
+    itamat_poly_no3rdrow(b20, b21, b22, q, Bm);
+    itamat_poly_row2(b20, b21, b22, q, Bm);
+    su3_mul(Bm, U, T);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) tr2 = cfma(SP.c[a][b], T.c[b][a], tr2);
+    su3_mul(U, SP, USP);
+    su3_mul(Q, USP, QU);
+    su3_mul(USP, Q, UQ);
+    Su3 G;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            double2 s = cmul(tr1, Q.c[a][b]);
+            s = cfma(tr2, Q2.c[a][b], s);
+            s = cfma(f1, USP.c[a][b], s);
+            s = cfma(f2, cadd(QU.c[a][b], UQ.c[a][b]), s);
+            G.c[a][b] = s;
+        }
+    const double third = 0.33333333333333333333333;
+    L.rc00 = (2 * G.c[0][0].x - G.c[1][1].x - G.c[2][2].x) * third;
+    L.rc11 = (2 * G.c[1][1].x - G.c[0][0].x - G.c[2][2].x) * third;
+    L.c01 = cscale(cadd(G.c[0][1], cconj(G.c[1][0])), 0.5);
+    L.c02 = cscale(cadd(G.c[0][2], cconj(G.c[2][0])), 0.5);
+    L.c12 = cscale(cadd(G.c[1][2], cconj(G.c[2][1])), 0.5);
+}
+
+// staples -> QA -> Lambda for every link (compute_sigma_from_sigma_prime ... first half)
+__global__ __launch_bounds__(kBlock) void k_qa_lambda(GaugePtrs G, const double2* __restrict__ SP, size_t su3_stride,
+                                                     void* __restrict__ qa, void* __restrict__ lam, LinkGeom g,
+                                                     double rho, double c0, int qa_given) {
+    const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
+    if (gid >= 8 * g.sizeh) return;
+    int dir, idxh;
+    link_of(gid, g.sizeh, dir, idxh);
+    Su3 U, S;
+    load_link(G.u[dir], g.sizeh, idxh, U);
+    Tamat q;
+    if (qa_given) {
+        ta_load(qa, dir, g.sizeh, idxh, q);
+    } else {
+        int d[4];
+        decode(idxh, dir & 1, g, d);
+        Su3 C, X;
+        staple_sum(G, g, d, idxh, dir >> 1, dir & 1, c0, C);
+        su3_mul(U, C, X);
+        ta_part(X, rho / c0, q);
+        ta_store(qa, dir, g.sizeh, idxh, q);
+    }
+    load_gl3((const double2*)((const char*)SP + dir * su3_stride), g.sizeh, idxh, S);
+    Thmat L;
+    lambda_local(q, S, U, L);
+    th_store(lam, dir, g.sizeh, idxh, L);
+}
+
+// Sigma = Sigma' exp(iQ) + i rho (staple terms with Lambda) (compute_sigma, stouting.c:1175-1302)
+__global__ __launch_bounds__(kBlock) void k_sigma(GaugePtrs G, const void* __restrict__ lam,
+                                                 const void* __restrict__ qa, double2* __restrict__ S,
+                                                 size_t su3_stride, LinkGeom g, double rho) {
+    const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
+    if (gid >= 8 * g.sizeh) return;
+    int dir, idxh;
+    link_of(gid, g.sizeh, dir, idxh);
+    const int mu = dir >> 1, parity = dir & 1, sh = g.sizeh;
+    int d[4];
+    decode(idxh, parity, g, d);
+    Tamat q;
+    ta_load(qa, dir, sh, idxh, q);
+    Su3 E, SPm, res;
+    ch_exp_minus_qa(q, E);
+    rebuild3row(E);
+    double2* Sd = (double2*)((char*)S + dir * su3_stride);
+    load_gl3(Sd, sh, idxh, SPm);
+    su3_mul(SPm, E, res);
+    Thmat lt;
+    Su3 LD;
+    th_load(lam, dir, sh, idxh, lt);
+    th_full(lt, LD);
+    const double2 irho = make_double2(0.0, rho), mirho = make_double2(0.0, -rho);
+    const int i_pmu = shifted(d, mu, 1, 0, 0, g);
+    for (int it = 0; it < 3; it++) {
+        const int nu = c_perp[mu][it];
+        const int i_pnu = shifted(d, nu, 1, 0, 0, g);
+        Su3 A, B, C, AB, ABC, X, L1, L2;
+        // right part: A = U_nu(x+mu), B = U_mu(x+nu)^+, C = U_nu(x)^+
+        load_link(G.u[2 * nu + !parity], sh, i_pmu, A);
+        load_link(G.u[2 * mu + !parity], sh, i_pnu, B);
+        load_link(G.u[2 * nu + parity], sh, idxh, C);
+        su3_mul_dag(A, B, AB);
+        su3_mul_dag(AB, C, ABC);
+        // i rho ABC (L_mu(x) - L_nu(x))
+        th_load(lam, 2 * nu + parity, sh, idxh, lt);
+        th_full(lt, L1);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) L1.c[a][b] = csub(LD.c[a][b], L1.c[a][b]);
+        su3_mul(ABC, L1, X);
+        su3_axpy(irho, X, res);
+        // i rho L_nu(x+mu) ABC
+        th_load(lam, 2 * nu + !parity, sh, i_pmu, lt);
+        th_full(lt, L2);
+        su3_mul(L2, ABC, X);
+        su3_axpy(irho, X, res);
+        // -i rho AB L_mu(x+nu) C
+        th_load(lam, 2 * mu + !parity, sh, i_pnu, lt);
+        th_full(lt, L2);
+        su3_mul(AB, L2, X);
+        su3_mul_dag(X, C, X);
+        su3_axpy(mirho, X, res);
+        // left part: A = U_nu(x+mu-nu)^+, B = U_mu(x-nu)^+, C = U_nu(x-nu)
+        const int i_mnu = shifted(d, nu, -1, 0, 0, g);
+        const int i_pmu_mnu = shifted(d, mu, 1, nu, -1, g);
+        load_link(G.u[2 * nu + parity], sh, i_pmu_mnu, A);
+        load_link(G.u[2 * mu + !parity], sh, i_mnu, B);
+        load_link(G.u[2 * nu + !parity], sh, i_mnu, C);
+        su3_mul(B, A, AB);  // (A^+ B^+) = (B A)^+
+        // i rho A^+ B^+ (L_nu(x-nu) - L_mu(x-nu)) C
+        th_load(lam, 2 * nu + !parity, sh, i_mnu, lt);
+        th_full(lt, L1);
+        th_load(lam, 2 * mu + !parity, sh, i_mnu, lt);
+        th_full(lt, L2);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) L1.c[a][b] = csub(L1.c[a][b], L2.c[a][b]);
+        su3_dag_mul(AB, L1, X);   // (BA)^+ (G - E)
+        su3_mul(X, C, X);
+        su3_axpy(irho, X, res);
+        // i rho A^+ B^+ C L_mu(x)
+        su3_dag_mul(AB, C, X);
+        su3_mul(X, LD, X);
+        su3_axpy(irho, X, res);
+        // -i rho A^+ L_nu(x+mu-nu) B^+ C
+        th_load(lam, 2 * nu + parity, sh, i_pmu_mnu, lt);
+        th_full(lt, L2);
+        su3_dag_mul(A, L2, X);
+        su3_dag_mul(B, C, L1);
+        su3_mul(X, L1, X);
+        su3_axpy(mirho, X, res);
+    }
+    store_rows(Sd, sh, idxh, res, 3);
+}
+
+// unitarize_conf (su3_utilities.c:17-40, loc_unitarize_conf su3_utilities.h:68-113)
+__global__ __launch_bounds__(kBlock) void k_unitarize(double2* __restrict__ u, size_t su3_stride, int sizeh) {
+    const int gid = blockIdx.x * kBlock + (int)threadIdx.x;
+    if (gid >= 8 * sizeh) return;
+    int dir, i;
+    link_of(gid, sizeh, dir, i);
+    double2* U = (double2*)((char*)u + dir * su3_stride);
+    double2 a0 = U[i], a1 = U[sizeh + i], a2 = U[2 * sizeh + i];
+    double2 b0 = U[3 * sizeh + i], b1 = U[4 * sizeh + i], b2 = U[5 * sizeh + i];
+    double n = 1.0 / sqrt(cnorm2(a0) + cnorm2(a1) + cnorm2(a2));
+    a0 = cscale(a0, n);
+    a1 = cscale(a1, n);
+    a2 = cscale(a2, n);
+    double2 sp = cmulc(a0, b0);
+    sp = cfmac(a1, b1, sp);
+    sp = cfmac(a2, b2, sp);
+    b0 = csub(b0, cmul(sp, a0));
+    b1 = csub(b1, cmul(sp, a1));
+    b2 = csub(b2, cmul(sp, a2));
+    n = 1.0 / sqrt(cnorm2(b0) + cnorm2(b1) + cnorm2(b2));
+    U[i] = a0;
+    U[sizeh + i] = a1;
+    U[2 * sizeh + i] = a2;
+    U[3 * sizeh + i] = cscale(b0, n);
+    U[4 * sizeh + i] = cscale(b1, n);
+    U[5 * sizeh + i] = cscale(b2, n);
+}
+
+// ---------------------------------------------------------------- launchers
+static LinkGeom link_geom() {
+    const Lattice& L = ctx().lat;
+    LinkGeom g;
+    for (int i = 0; i < 4; i++) g.nd[i] = L.nd[i];
+    g.sizeh = (int)L.sizeh;
+    return g;
+}
+static GaugePtrs gauge_ptrs(const void* u) {
+    GaugePtrs G;
+    const size_t st = ctx().lat.su3_bytes();
+    for (int l = 0; l < 8; l++) G.u[l] = (const double2*)((const char*)u + l * st);
+    return G;
+}
+static int nblk8() { return blocks_for(8 * ctx().lat.sizeh); }
+
+bool launch_stout(const void* u, void* uout, double rho, double c0, void* stap, void* qa, void* expo) {
+    hipLaunchKernelGGL(k_stout, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u), (double2*)uout,
+                       ctx().lat.su3_bytes(), link_geom(), rho, c0, (double2*)stap, qa, (double2*)expo);
+    return hip_ok(hipGetLastError(), "k_stout");
+}
+
+bool launch_sigma_chain(void* sigma, void* lam, void* qa, const void* u, double rho, double c0) {
+    const size_t st = ctx().lat.su3_bytes();
+    hipLaunchKernelGGL(k_qa_lambda, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u),
+                       (const double2*)sigma, st, qa, lam, link_geom(), rho, c0, 0);
+    if (!hip_ok(hipGetLastError(), "k_qa_lambda")) return false;
+    hipLaunchKernelGGL(k_sigma, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u), (const void*)lam,
+                       (const void*)qa, (double2*)sigma, st, link_geom(), rho);
+    return hip_ok(hipGetLastError(), "k_sigma");
+}
+
+double stout_c_zero() { return g_c_zero; }
+
+}  // namespace ostp
+
+using namespace ostp;
+
+// ------------------------------------------------------------ C entry points
+static bool sync(const char* nm) { return hip_ok(hipStreamSynchronize(ctx().stream), nm); }
+static double rho_of(int istopo) { return istopo ? gl_topo_rho : gl_stout_rho; }
+
+extern "C" {
+
+int ostp_stout_steps(void) { return g_stout_steps; }
+
+void ostp_set_stout(double rho, int steps, double topo_rho, int topo_steps, int topo_action, int tlsm) {
+    gl_stout_rho = rho;
+    gl_topo_rho = topo_rho;
+    g_stout_steps = steps;
+    g_topo_steps = topo_steps;
+    g_topo_action = topo_action;
+    g_c_zero = tlsm ? 5.0 * 0.33333333333333333333333 : 1.0;
+}
+
+void stout_isotropic(const su3_soa* u, su3_soa* uprime, su3_soa* local_staples, su3_soa* auxiliary,
+                     tamat_soa* tipdot, const int istopo) {
+    const char* nm = "stout_isotropic";
+    if (!require_init(nm)) return;
+    const Lattice& L = ctx().lat;
+    const size_t gb = 8 * L.su3_bytes(), tb = (size_t)8 * 64 * L.sizeh;
+    const void* du = resolve(u, gb, nm);
+    void* dout = resolve(uprime, gb, nm);
+    void* dst = local_staples ? resolve(local_staples, gb, nm) : nullptr;
+    void* dex = auxiliary ? resolve(auxiliary, gb, nm) : nullptr;
+    void* dqa = tipdot ? resolve(tipdot, tb, nm) : nullptr;
+    if (!du || !dout || (local_staples && !dst) || (auxiliary && !dex) || (tipdot && !dqa)) return;
+    if (launch_stout(du, dout, rho_of(istopo), g_c_zero, dst, dqa, dex)) sync(nm);
+}
+
+void stout_wrapper(const su3_soa* tconf_acc, su3_soa* tstout_conf_acc_arr, const int istopo) {
+    const char* nm = "stout_wrapper";
+    if (!require_init(nm)) return;
+    const int steps = (istopo & g_topo_action) ? g_topo_steps : g_stout_steps;
+    if (steps <= 0) return;
+    const Lattice& L = ctx().lat;
+    const size_t gb = 8 * L.su3_bytes();
+    const void* du = resolve(tconf_acc, gb, nm);
+    char* arr = (char*)resolve(tstout_conf_acc_arr, steps * gb, nm);
+    if (!du || !arr) return;
+    const double rho = rho_of(istopo);
+    if (!launch_stout(du, arr, rho, g_c_zero, nullptr, nullptr, nullptr)) return;
+    for (int level = 1; level < steps; level++)
+        if (!launch_stout(arr + (level - 1) * gb, arr + level * gb, rho, g_c_zero, nullptr, nullptr, nullptr)) return;
+    sync(nm);
+}
+
+void exp_minus_QA_times_conf(const su3_soa* tu, const tamat_soa* QA, su3_soa* tu_out, su3_soa* exp_aux) {
+    const char* nm = "exp_minus_QA_times_conf";
+    if (!require_init(nm)) return;
+    const Lattice& L = ctx().lat;
+    const size_t gb = 8 * L.su3_bytes(), tb = (size_t)8 * 64 * L.sizeh;
+    const void* du = resolve(tu, gb, nm);
+    const void* dq = resolve(QA, tb, nm);
+    void* dout = resolve(tu_out, gb, nm);
+    void* dex = exp_aux ? resolve(exp_aux, gb, nm) : nullptr;
+    if (!du || !dq || !dout || (exp_aux && !dex)) return;
+    hipLaunchKernelGGL(k_exp_times_conf, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(du), dq,
+                       (double2*)dout, (double2*)dex, L.su3_bytes(), link_geom());
+    if (hip_ok(hipGetLastError(), nm)) sync(nm);
+}
+
+void calc_loc_staples_nnptrick_all_onlyferms(const su3_soa* u, su3_soa* loc_stap) {
+    const char* nm = "calc_loc_staples_nnptrick_all_onlyferms";
+    if (!require_init(nm)) return;
+    const Lattice& L = ctx().lat;
+    const void* du = resolve(u, 8 * L.su3_bytes(), nm);
+    void* ds = resolve(loc_stap, 8 * L.su3_bytes(), nm);
+    if (!du || !ds) return;
+    hipLaunchKernelGGL(k_staples, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(du), (double2*)ds,
+                       L.su3_bytes(), link_geom(), g_c_zero);
+    if (hip_ok(hipGetLastError(), nm)) sync(nm);
+}
+
+void RHO_times_conf_times_staples_ta_part(const su3_soa* u, const su3_soa* loc_stap, tamat_soa* tipdot, int istopo) {
+    const char* nm = "RHO_times_conf_times_staples_ta_part";
+    if (!require_init(nm)) return;
+    const Lattice& L = ctx().lat;
+    const void* du = resolve(u, 8 * L.su3_bytes(), nm);
+    const void* ds = resolve(loc_stap, 8 * L.su3_bytes(), nm);
+    void* dq = resolve(tipdot, (size_t)8 * 64 * L.sizeh, nm);
+    if (!du || !ds || !dq) return;
+    hipLaunchKernelGGL(k_rho_ta, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(du), (const double2*)ds,
+                       L.su3_bytes(), dq, link_geom(), rho_of(istopo), g_c_zero);
+    if (hip_ok(hipGetLastError(), nm)) sync(nm);
+}
+
+void compute_lambda(thmat_soa* Lm, const su3_soa* SP, const su3_soa* U, const tamat_soa* QA, su3_soa* TMP) {
+    const char* nm = "compute_lambda";
+    (void)TMP;
+    if (!require_init(nm)) return;
+    const Lattice& L = ctx().lat;
+    const size_t gb = 8 * L.su3_bytes(), tb = (size_t)8 * 64 * L.sizeh;
+    void* dl = resolve(Lm, tb, nm);
+    const void* ds = resolve(SP, gb, nm);
+    const void* du = resolve(U, gb, nm);
+    const void* dq = resolve(QA, tb, nm);
+    if (!dl || !ds || !du || !dq) return;
+    hipLaunchKernelGGL(k_qa_lambda, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(du), (const double2*)ds,
+                       L.su3_bytes(), const_cast<void*>(dq), dl, link_geom(), 0.0, g_c_zero, 1);
+    if (hip_ok(hipGetLastError(), nm)) sync(nm);
+}
+
+void compute_sigma(const thmat_soa* Lm, const su3_soa* U, su3_soa* S, const tamat_soa* QA, su3_soa* TMP,
+                   const int istopo) {
+    const char* nm = "compute_sigma";
+    (void)TMP;
+    if (!require_init(nm)) return;
+    const Lattice& L = ctx().lat;
+    const size_t gb = 8 * L.su3_bytes(), tb = (size_t)8 * 64 * L.sizeh;
+    const void* dl = resolve(Lm, tb, nm);
+    const void* du = resolve(U, gb, nm);
+    void* ds = resolve(S, gb, nm);
+    const void* dq = resolve(QA, tb, nm);
+    if (!dl || !ds || !du || !dq) return;
+    hipLaunchKernelGGL(k_sigma, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(du), dl, dq, (double2*)ds,
+                       L.su3_bytes(), link_geom(), rho_of(istopo));
+    if (hip_ok(hipGetLastError(), nm)) sync(nm);
+}
+
+void compute_sigma_from_sigma_prime_backinto_sigma_prime(su3_soa* Sigma, thmat_soa* Lambda, tamat_soa* QA,
+                                                          const su3_soa* U, su3_soa* TMP, const int istopo) {
+    const char* nm = "compute_sigma_from_sigma_prime_backinto_sigma_prime";
+    (void)TMP;
+    if (!require_init(nm)) return;
+    const Lattice& L = ctx().lat;
+    const size_t gb = 8 * L.su3_bytes(), tb = (size_t)8 * 64 * L.sizeh;
+    void* ds = resolve(Sigma, gb, nm);
+    void* dl = resolve(Lambda, tb, nm);
+    void* dq = resolve(QA, tb, nm);
+    const void* du = resolve(U, gb, nm);
+    if (!ds || !dl || !dq || !du) return;
+    if (launch_sigma_chain(ds, dl, dq, du, rho_of(istopo), g_c_zero)) sync(nm);
+}
+
+void unitarize_conf(su3_soa* u) {
+    const char* nm = "unitarize_conf";
+    if (!require_init(nm)) return;
+    const Lattice& L = ctx().lat;
+    void* du = resolve(u, 8 * L.su3_bytes(), nm);
+    if (!du) return;
+    hipLaunchKernelGGL(k_unitarize, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, (double2*)du, L.su3_bytes(),
+                       (int)L.sizeh);
+    if (hip_ok(hipGetLastError(), nm)) sync(nm);
+}
+
+}  // extern "C"

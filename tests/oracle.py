@@ -225,3 +225,81 @@ def read_remez(path) -> RatApprox:
     if rc != 0:
         raise ValueError(f"or_ratapprox_read({path}) -> {rc}")
     return ra
+
+
+# ---- gauge part (oracle/gauge.c)
+C_ZERO_TLSM = 5.0 * 0.33333333333333333333333  # common_defines.h:71-76
+for name, res, args in [
+    ("or_calc_loc_staples", None, [_G, _P, _P, _D]),
+    ("or_rho_times_conf_times_staples_ta", None, [_G, _P, _P, _P, _D, _D]),
+    ("or_exp_minus_QA_times_conf", None, [_G, _P, _P, _P]),
+    ("or_stout_isotropic", None, [_G, _P, _P, _D, _D]),
+    ("or_compute_lambda", None, [_G, _P, _P, _P, _P]),
+    ("or_compute_sigma", None, [_G, _P, _P, _P, _P, _D]),
+    ("or_sigma_prime_to_sigma", None, [_G, _P, _P, _P, _P, _D, _D]),
+    ("or_unitarize_conf", None, [_G, _P]),
+    ("or_direct_product_into_auxmat", None, [_G, _P, _P, _P, _D]),
+    ("or_multiply_backfield_times_force", None, [_G, _P, _P, _P]),
+    ("or_conf_times_force_ta", None, [_G, _P, _P, _P]),
+]:
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def _ta_zero(o):
+    return np.zeros(8 * 64 * o.sizeh, dtype=np.uint8)
+
+
+def ta_view(o, t):
+    """(c (8,3,sizeh) complex, d (8,2,sizeh) real) views of a tamat/thmat soa[8]."""
+    raw = t.view(np.uint8).reshape(8, 64 * o.sizeh)
+    c = raw[:, : 48 * o.sizeh].view(np.complex128).reshape(8, 3, o.sizeh)
+    d = raw[:, 48 * o.sizeh:].view(np.float64).reshape(8, 2, o.sizeh)
+    return c, d
+
+
+def _bind_gauge(cls):
+    def staples(self, u, c_zero=C_ZERO_TLSM):
+        st = self.gauge()
+        lib.or_calc_loc_staples(self.gp, p(u), p(st), c_zero)
+        return st
+
+    def qa(self, u, st, rho, c_zero=C_ZERO_TLSM):
+        q = _ta_zero(self)
+        lib.or_rho_times_conf_times_staples_ta(self.gp, p(u), p(st), p(q), rho, c_zero)
+        return q
+
+    def stout(self, u, rho, c_zero=C_ZERO_TLSM):
+        out = self.gauge()
+        lib.or_stout_isotropic(self.gp, p(u), p(out), rho, c_zero)
+        return out
+
+    def sigma_prime_to_sigma(self, sigma, u, rho, c_zero=C_ZERO_TLSM):
+        s = sigma.copy()
+        lam, q = _ta_zero(self), _ta_zero(self)
+        lib.or_sigma_prime_to_sigma(self.gp, p(s), p(lam), p(q), p(u), rho, c_zero)
+        return s, lam, q
+
+    def unitarize(self, u):
+        v = u.copy()
+        lib.or_unitarize_conf(self.gp, p(v))
+        return v
+
+    def direct_product(self, s, h, aux, a):
+        lib.or_direct_product_into_auxmat(self.gp, p(s), p(h), p(aux), a)
+
+    def backfield_times_force(self, ph, aux, out):
+        lib.or_multiply_backfield_times_force(self.gp, p(ph), p(aux), p(out))
+
+    def conf_times_force_ta(self, u, aux, ipdot):
+        lib.or_conf_times_force_ta(self.gp, p(u), p(aux), p(ipdot))
+
+    for fn in (staples, qa, stout, sigma_prime_to_sigma, unitarize, direct_product, backfield_times_force,
+               conf_times_force_ta):
+        setattr(cls, fn.__name__, fn)
+    cls.ta_zero = _ta_zero
+    return cls
+
+
+_bind_gauge(Oracle)

@@ -62,3 +62,50 @@ def xcd_block_id(i, G):
 def test_xcd_remap_is_a_bijection(G):
     ids = sorted(xcd_block_id(i, G) for i in range(G))
     assert ids == list(range(G))
+
+
+def decode(idxh, parity, nd):
+    """Restatement of `decode` in openstaple_amd/csrc/stout.hip."""
+    nd0h = nd[0] >> 1
+    t = idxh // nd0h
+    hd0 = idxh - t * nd0h
+    t2 = t // nd[1]
+    d1 = t - t2 * nd[1]
+    d3 = t2 // nd[2]
+    d2 = t2 - d3 * nd[2]
+    d0 = 2 * hd0 + ((d1 + d2 + d3 + parity) & 1)
+    return [d0, d1, d2, d3]
+
+
+def snum(d, nd):
+    return (d[0] + nd[0] * (d[1] + nd[1] * (d[2] + nd[2] * d[3]))) >> 1
+
+
+def shifted(d, mu, s, nu, t, nd):
+    e = list(d)
+    e[mu] = (e[mu] + s) % nd[mu]
+    if t:
+        e[nu] = (e[nu] + t) % nd[nu]
+    return snum(e, nd)
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4, 4), (4, 6, 8, 10), (16, 4, 6, 2)])
+def test_link_kernel_site_decoding(shape):
+    """decode/shifted of the per-link kernels (stout.hip, force.hip) agree with
+    the neighbour tables and stay in bounds."""
+    o = Oracle(*shape)
+    nnp, nnm = o.nn_tables()
+    sizeh = o.sizeh
+    for parity in (0, 1):
+        for i in range(sizeh):
+            d = decode(i, parity, shape)
+            assert snum(d, shape) == i and sum(d) % 2 == parity
+            for mu in range(4):
+                assert shifted(d, mu, 1, 0, 0, shape) == nnp[i, mu, parity]
+                assert shifted(d, mu, -1, 0, 0, shape) == nnm[i, mu, parity]
+                for nu in range(4):
+                    if nu == mu:
+                        continue
+                    j = shifted(d, mu, 1, nu, -1, shape)
+                    assert 0 <= j < sizeh
+                    assert j == nnm[nnp[i, mu, parity], nu, 1 - parity]
