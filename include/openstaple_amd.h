@@ -209,6 +209,21 @@ void RHO_times_conf_times_staples_ta_part(const su3_soa *u, const su3_soa *loc_s
                                           int istopo);
 /* su3_utilities.h:25 */
 void unitarize_conf(su3_soa *u);
+int  ostp_stout_steps(void);
+
+/* ======================================================== fermion force == */
+/* fermion_force_utilities.h:16-42 */
+void direct_product_of_fermions_into_auxmat(const vec3_soa *loc_s, const vec3_soa *loc_h, su3_soa *aux_u,
+                                            const RationalApprox *approx, int iter);
+void multiply_backfield_times_force(ferm_param *tpars, const su3_soa *auxmat, su3_soa *pseudo_ipdot);
+void multiply_conf_times_force_and_take_ta_nophase(const su3_soa *u, const su3_soa *auxmat, tamat_soa *ipdot);
+void ker_openacc_compute_fermion_force(const su3_soa *u, su3_soa *aux_u, const vec3_soa *in_shiftmulti,
+                                       vec3_soa *loc_s, vec3_soa *loc_h, ferm_param *tpars);
+/* fermion_force.h:24-36 (STOUT_FERMIONS build; stout steps from ostp_set_stout) */
+void fermion_force_soloopenacc(su3_soa *tconf_acc, su3_soa *tstout_conf_acc_arr, su3_soa *gl3_aux,
+                               tamat_soa *tipdot_acc, ferm_param *tfermion_parameters, int tNDiffFlavs,
+                               const vec3_soa *ferm_in_acc, double res, su3_soa *taux_conf_acc,
+                               vec3_soa *tferm_shiftmulti_acc, inverter_package ipt, const int max_cg);
 
 /* ======================================================== generators ===== */
 /* Reference field rules (OpenAcc/random_assignement.c:67-92,106-142,188-250)

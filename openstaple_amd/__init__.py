@@ -171,6 +171,12 @@ _SIGS = {
     "calc_loc_staples_nnptrick_all_onlyferms": (None, [_P, _P]),
     "RHO_times_conf_times_staples_ta_part": (None, [_P, _P, _P, _I]),
     "unitarize_conf": (None, [_P]),
+    "ostp_stout_steps": (_I, []),
+    "direct_product_of_fermions_into_auxmat": (None, [_P, _P, _P, _P, _I]),
+    "multiply_backfield_times_force": (None, [_P, _P, _P]),
+    "multiply_conf_times_force_and_take_ta_nophase": (None, [_P, _P, _P]),
+    "ker_openacc_compute_fermion_force": (None, [_P, _P, _P, _P, _P, _P]),
+    "fermion_force_soloopenacc": (None, [_P, _P, _P, _P, _P, _I, _P, _D, _P, _P, InverterPackage, _I]),
     "ostp_time_dslash": (_D, [_P, _P, _P, _P, _I, _I, _I]),
     "ostp_time_copy": (_D, [_P, _P, C.c_size_t, _I]),
 }

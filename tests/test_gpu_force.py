@@ -1,0 +1,148 @@
+"""GPU parity of the fermion-force pieces and of the full
+fermion_force_soloopenacc driver against the oracle composed the way the
+reference composes it (fermion_force.c:166-358).
+
+Bar: the per-link pieces (outer products, phase multiplication, TA(U Sigma))
+to 1e-13 relative on identical inputs; the full driver, whose multishift
+solutions each carry the solver tolerance 1e-10, to 1e-7 relative."""
+import ctypes as C
+import pathlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import openstaple_amd as ost  # noqa: E402
+from openstaple_amd import lib  # noqa: E402
+from gpu_util import Dev, init_lattice  # noqa: E402
+from oracle import Oracle, ta_view  # noqa: E402
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+RHO = 0.15
+
+
+def maxrel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def approx_md(order=9):
+    name = "approx_-1_over_4_order_9_mloglm_6.4.REMEZ" if order == 9 else \
+        "approx_-1_over_4_order_19_mloglm_6.4.REMEZ"
+    ra = ost.read_remez(str(GOLDEN / name))
+    lmax = 32.0
+    eps = lmax ** (ra.exponent_num / ra.exponent_den)
+    ra.RA_a0 *= eps
+    for k in range(ra.approx_order):
+        ra.RA_a[k] *= lmax * eps
+        ra.RA_b[k] *= lmax
+    return ra
+
+
+def test_force_pieces_vs_oracle():
+    shape = (8, 8, 8, 8)
+    o = Oracle(*shape)
+    init_lattice(shape)
+    u = o.gen_cold(0.6, 2)
+    ph = o.phases()
+    ra = approx_md(9)
+    order = ra.approx_order
+    xs = np.stack([o.gen_gauss(40 + k) for k in range(order)])
+    # outer products over all shifts
+    aux = o.gauge()
+    for k in range(order):
+        h = o.doe(u, xs[k], ph)
+        o.direct_product(np.ascontiguousarray(xs[k]), h, aux, ra.RA_a[k])
+    du, dph, dxs = Dev(u), Dev(ph), Dev(xs)
+    daux = Dev(nbytes=aux.nbytes)
+    fp = ost.ferm_param(0.05, dph.p)
+    fp.approx_md = ra
+    s, h = Dev(nbytes=xs[0].nbytes), Dev(nbytes=xs[0].nbytes)
+    lib.ker_openacc_compute_fermion_force(du.p, daux.p, dxs.p, s.p, h.p, C.byref(fp))
+    ost.check(0, "ker_openacc_compute_fermion_force")
+    assert maxrel(o.links(daux.get(aux)), o.links(aux)) < 1e-13
+    # single-shift entry point accumulates the same
+    daux2 = Dev(nbytes=aux.nbytes)
+    dh = Dev(nbytes=xs[0].nbytes)
+    for k in range(order):
+        lib.acc_Doe(du.p, dh.p, dxs.p + k * xs[0].nbytes, dph.p)
+        lib.direct_product_of_fermions_into_auxmat(dxs.p + k * xs[0].nbytes, dh.p, daux2.p, C.byref(ra), k)
+    ost.check(0, "direct_product")
+    assert maxrel(o.links(daux2.get(aux)), o.links(aux)) < 1e-13
+    # phases
+    phx = o.phases(bf=(0.3, 0, 0, 0, 1.0, 0), charge=1.0)
+    dphx = Dev(phx)
+    fpx = ost.ferm_param(0.05, dphx.p)
+    gl3 = o.gauge()
+    o.backfield_times_force(phx, aux, gl3)
+    dgl3 = Dev(nbytes=aux.nbytes)
+    lib.multiply_backfield_times_force(C.byref(fpx), daux.p, dgl3.p)
+    ost.check(0, "backfield")
+    assert maxrel(o.links(dgl3.get(gl3)), o.links(gl3)) < 1e-13
+    # TA(U Sigma)
+    ip = o.ta_zero()
+    o.conf_times_force_ta(u, gl3, ip)
+    dip = Dev(nbytes=ip.nbytes)
+    lib.multiply_conf_times_force_and_take_ta_nophase(du.p, dgl3.p, dip.p)
+    ost.check(0, "conf_times_force")
+    c, d = ta_view(o, dip.get(ip))
+    rc, rd = ta_view(o, ip)
+    assert maxrel(c, rc) < 1e-13 and maxrel(d, rd) < 1e-13
+
+
+@pytest.mark.parametrize("steps", [0, 2])
+def test_fermion_force_driver(steps):
+    shape = (8, 8, 8, 8)
+    o = Oracle(*shape)
+    init_lattice(shape)
+    lib.ostp_set_stout(RHO, steps, 0.0, 0, 0, 1)
+    lib.ostp_set_inverter_tricks(0, 0, 0.1, 10000)
+    u = o.gen_cold(0.5, 7)
+    ph = o.phases()
+    ra = approx_md(9)
+    order = ra.approx_order
+    mass = 0.1
+    nps = 2
+    phis = np.stack([o.gen_gauss(70 + i) for i in range(nps)])
+    # ---- oracle composition (fermion_force.c:166-358)
+    levels = [u]
+    for _ in range(steps):
+        levels.append(o.stout(levels[-1], RHO))
+    v = levels[-1]
+    taux = o.gauge()
+    shifts = [ra.RA_b[k] for k in range(order)]
+    for i in range(nps):
+        st, cg, xs = o.multishift(v, ph, mass, shifts, np.ascontiguousarray(phis[i]), 1e-11)
+        for k in range(order):
+            xk = np.ascontiguousarray(xs[k])
+            o.direct_product(xk, o.doe(v, xk, ph), taux, ra.RA_a[k])
+    gl3 = o.gauge()
+    o.backfield_times_force(ph, taux, gl3)
+    for level in range(steps, 0, -1):
+        gl3, _, _ = o.sigma_prime_to_sigma(gl3, levels[level - 1], RHO)
+    ip = o.ta_zero()
+    o.conf_times_force_ta(u, gl3, ip)
+    # ---- library driver
+    lat = ost.Lattice(*shape)
+    du, dph, dphi = Dev(u), Dev(ph), Dev(phis)
+    darr = Dev(nbytes=max(steps, 1) * u.nbytes)
+    dgl3, dtaux = Dev(nbytes=u.nbytes), Dev(nbytes=u.nbytes)
+    dip = Dev(nbytes=ip.nbytes)
+    dxs, dsf = Dev(nbytes=order * lat.vec3_bytes), Dev(nbytes=order * lat.vec3_bytes)
+    d64 = [Dev(nbytes=lat.vec3_bytes) for _ in range(4)]
+    fps = (ost.FermParam * 1)()
+    fps[0].ferm_mass = mass
+    fps[0].phases = dph.p
+    fps[0].number_of_ps = nps
+    fps[0].index_of_the_first_ps = 0
+    fps[0].approx_md = ra
+    ipk = ost.InverterPackage()
+    ipk.u = du.p
+    ipk.ferm_shift_temp, ipk.nshifts = dsf.p, order
+    ipk.loc_r, ipk.loc_h, ipk.loc_s, ipk.loc_p = (d.p for d in d64)
+    lib.fermion_force_soloopenacc(du.p, darr.p, dgl3.p, dip.p, fps, 1, dphi.p, 1e-11, dtaux.p, dxs.p, ipk, 10000)
+    ost.check(0, "fermion_force_soloopenacc")
+    c, d = ta_view(o, dip.get(ip))
+    rc, rd = ta_view(o, ip)
+    assert maxrel(c, rc) < 1e-7 and maxrel(d, rd) < 1e-7
+    lib.ostp_set_stout(RHO, 2, 0.0, 0, 0, 1)
