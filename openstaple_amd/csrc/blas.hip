@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const C* __restrict__ a, cons
             }
         }
     }
-    grid_reduce_finalize<F, (KIND == 2 ? 2 : 1)>(v0, v1, ctl);
+    block_partial<(KIND == 2 ? 2 : 1)>(v0, v1, ctl);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -114,7 +114,7 @@ bool launch_norm2(const void* a, const SolverCtl& ctl) {
     const int sizeh = (int)ctx().lat.sizeh;
     hipLaunchKernelGGL((k_reduce<C, 0, F>), dim3(nb(sizeh)), dim3(kBlock), 0, S(), (const C*)a, (const C*)nullptr,
                        sizeh, ctl);
-    return hip_ok(hipGetLastError(), "k_reduce");
+    return hip_ok(hipGetLastError(), "k_reduce") && launch_finalize(F, ctl, nb(sizeh));
 }
 template bool launch_norm2<double2, kFinStore>(const void*, const SolverCtl&);
 template bool launch_norm2<float2, kFinStore>(const void*, const SolverCtl&);
@@ -127,14 +127,32 @@ bool launch_dot(const void* a, const void* b, int complex_dot, const SolverCtl& 
     else
         hipLaunchKernelGGL((k_reduce<double2, 1, kFinStore>), dim3(nb(sizeh)), dim3(kBlock), 0, S(),
                            (const double2*)a, (const double2*)b, sizeh, ctl);
-    return hip_ok(hipGetLastError(), "k_reduce");
+    return hip_ok(hipGetLastError(), "k_reduce") &&
+           launch_finalize(complex_dot ? kFinStore2 : kFinStore, ctl, nb(sizeh));
+}
+
+bool launch_finalize(int fin, const SolverCtl& c, int nparts) {
+#define OSTP_F(F, NV) hipLaunchKernelGGL((k_finalize<F, NV>), dim3(1), dim3(kBlock), 0, S(), c, nparts)
+    switch (fin) {
+        case kFinStore: OSTP_F(kFinStore, 1); break;
+        case kFinStore2: OSTP_F(kFinStore2, 2); break;
+        case kFinCgAlpha: OSTP_F(kFinCgAlpha, 1); break;
+        case kFinCgLambda: OSTP_F(kFinCgLambda, 1); break;
+        case kFinCgRestart: OSTP_F(kFinCgRestart, 1); break;
+        case kFinMsAlpha: OSTP_F(kFinMsAlpha, 1); break;
+        case kFinMsLambda: OSTP_F(kFinMsLambda, 1); break;
+        case kFinMxAlpha: OSTP_F(kFinMxAlpha, 1); break;
+        case kFinMxLambda: OSTP_F(kFinMxLambda, 1); break;
+        default: return fail("unknown finalizer %d", fin);
+    }
+#undef OSTP_F
+    return hip_ok(hipGetLastError(), "k_finalize");
 }
 
 SolverCtl plain_ctl() {
     SolverCtl c;
     c.st = ctx().state;
     c.partials = ctx().partials;
-    c.ticket = ctx().tickets;
     c.pred = kAlways;
     return c;
 }

@@ -40,8 +40,15 @@ struct DslashArgs {
     SolverCtl ctl;
 };
 
-__device__ __forceinline__ void sincos_t(double a, double* s, double* c) { sincos(a, s, c); }
-__device__ __forceinline__ void sincos_t(float a, float* s, float* c) { sincosf(a, s, c); }
+// FP32 phases are evaluated in double (eight per site; the kernel is
+// bandwidth-bound) and rounded once.
+__device__ __forceinline__ void sincos_t(double a, double* s, double* c) { phase_sincos(a, s, c); }
+__device__ __forceinline__ void sincos_t(float a, float* s, float* c) {
+    double ds, dc;
+    phase_sincos((double)a, &ds, &dc);
+    *s = (float)ds;
+    *c = (float)dc;
+}
 
 template <typename C> __device__ __forceinline__ C mk(double a, double b);
 template <> __device__ __forceinline__ double2 mk<double2>(double a, double b) { return make_double2(a, b); }
@@ -55,9 +62,9 @@ __device__ __forceinline__ void hop(const typename Cplx<R>::T* __restrict__ U, c
                                     typename Cplx<R>::T& o0, typename Cplx<R>::T& o1,
                                     typename Cplx<R>::T& o2) {
     using C = typename Cplx<R>::T;
-    const C a0 = U[iu], a1 = U[sizeh + iu], a2 = U[2 * sizeh + iu];
-    const C b0 = U[3 * sizeh + iu], b1 = U[4 * sizeh + iu], b2 = U[5 * sizeh + iu];
-    const R arg = P[iu];
+    const C a0 = ld_nt(U + iu), a1 = ld_nt(U + sizeh + iu), a2 = ld_nt(U + 2 * sizeh + iu);
+    const C b0 = ld_nt(U + 3 * sizeh + iu), b1 = ld_nt(U + 4 * sizeh + iu), b2 = ld_nt(U + 5 * sizeh + iu);
+    const R arg = ld_nt(P + iu);
     C v0 = in[iv], v1 = in[sizeh + iv], v2 = in[2 * sizeh + iv];
     R s, c;
     sincos_t(arg, &s, &c);
@@ -87,7 +94,7 @@ template <typename C> __device__ __forceinline__ double re_dot(C a, C b) {
 }
 
 // EO = 0: Deo (even output, odd input); EO = 1: Doe.
-template <typename R, int EO, int MODE, int FIN>
+template <typename R, int EO, int MODE>
 __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> a) {
     using C = typename Cplx<R>::T;
     if (!pred_pass(a.ctl)) return;
@@ -142,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> a) {
             if (MODE == kMdagMDot) dot = re_dot(x0, s0) + re_dot(x1, s1v) + re_dot(x2, s2v);
         }
     }
-    if (MODE == kMdagMDot) grid_reduce_finalize<FIN, 1>(dot, 0.0, a.ctl);
+    if (MODE == kMdagMDot) block_partial<1>(dot, 0.0, a.ctl);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -181,23 +188,20 @@ bool launch_dslash(int eo, int mode, int fin, const void* u, const void* ph, con
     a.ctl = ctl;
     const int nb = blocks_for(ctx().lat.sizeh);
     hipStream_t s = ctx().stream;
-#define OSTP_L(EO, MODE, FIN) hipLaunchKernelGGL((k_dslash<R, EO, MODE, FIN>), dim3(nb), dim3(kBlock), 0, s, a)
+#define OSTP_L(EO, MODE) hipLaunchKernelGGL((k_dslash<R, EO, MODE>), dim3(nb), dim3(kBlock), 0, s, a)
     if (mode == kPlain) {
-        if (eo == 0) OSTP_L(0, kPlain, kFinStore); else OSTP_L(1, kPlain, kFinStore);
+        if (eo == 0) OSTP_L(0, kPlain); else OSTP_L(1, kPlain);
     } else if (mode == kMdagM) {
-        if (eo == 0) OSTP_L(0, kMdagM, kFinStore); else OSTP_L(1, kMdagM, kFinStore);
+        if (eo == 0) OSTP_L(0, kMdagM); else OSTP_L(1, kMdagM);
     } else {
         if (eo != 0) return fail("fused dot epilogue only on Deo");
-        switch (fin) {
-            case kFinStore: OSTP_L(0, kMdagMDot, kFinStore); break;
-            case kFinCgAlpha: OSTP_L(0, kMdagMDot, kFinCgAlpha); break;
-            case kFinMsAlpha: OSTP_L(0, kMdagMDot, kFinMsAlpha); break;
-            case kFinMxAlpha: OSTP_L(0, kMdagMDot, kFinMxAlpha); break;
-            default: return fail("unsupported dslash finalizer %d", fin);
-        }
+        if (fin != kFinStore && fin != kFinCgAlpha && fin != kFinMsAlpha && fin != kFinMxAlpha)
+            return fail("unsupported dslash finalizer %d", fin);
+        OSTP_L(0, kMdagMDot);
     }
 #undef OSTP_L
-    return hip_ok(hipGetLastError(), "k_dslash launch");
+    if (!hip_ok(hipGetLastError(), "k_dslash launch")) return false;
+    return mode == kMdagMDot ? launch_finalize(fin, ctl, nb) : true;
 }
 
 template bool launch_dslash<double>(int, int, int, const void*, const void*, const void*, void*, const void*,

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_backfield_times_force(const double2*
     if (gid >= 8 * sizeh) return;
     const int dir = gid / sizeh, i = gid - dir * sizeh;
     double s, c;
-    sincos(phases[(long)dir * sizeh + i], &s, &c);
+    phase_sincos(phases[(long)dir * sizeh + i], &s, &c);
     const double2 ph = make_double2(c, s);
     const double2* A = (const double2*)((const char*)aux + dir * su3_stride);
     double2* O = (double2*)((char*)out + dir * su3_stride);

@@ -1,10 +1,14 @@
 // Deterministic grid reductions with an on-device scalar epilogue.
 //
-// Every block reduces its partial with a fixed shuffle/LDS tree and publishes
-// it at agent scope; the last block to take a ticket sums the partials in
-// block-index order (so the result does not depend on scheduling) and runs
-// the solver's scalar update ("finalizer") on the device.  The solver
-// recurrences therefore never wait for a host round trip.
+// A reducing kernel sums its values per block with a fixed shuffle/LDS tree
+// and stores the block's partial with a plain store; k_finalize, launched
+// right after it on the same stream, sums the partials in block-index order
+// (so the result does not depend on scheduling) and runs the solver's scalar
+// update ("finalizer") on the device.  The kernel boundary orders the partial
+// stores before the finalizer's loads, so no block issues a device-scope
+// release or atomic (on a multi-XCD part each such release writes back the
+// XCD's L2, which costs more than the reduction itself).  The solver
+// recurrences never wait for a host round trip.
 #pragma once
 
 #include "ostp_runtime.h"
@@ -31,12 +35,12 @@ enum Pred : int {
     kIfRunning = 1,      // status == kRunning
     kIfIter = 2,         // cg == iteration number this launch belongs to
     kIfRunningRU = 3,    // running and reliable update scheduled
+    kIfRunningNoRU = 4,  // running and no reliable update scheduled
 };
 
 struct SolverCtl {
     SolverState* st = nullptr;
     double* partials = nullptr;
-    unsigned int* ticket = nullptr;
     int pred = kAlways;
     int iter = 0;
 };
@@ -47,6 +51,7 @@ __device__ __forceinline__ bool pred_pass(const SolverCtl& c) {
     if (c.pred == kIfRunning) return st->status == kRunning;
     if (c.pred == kIfIter) return st->cg == c.iter;
     if (c.pred == kIfRunningRU) return st->status == kRunning && st->ru != 0;
+    if (c.pred == kIfRunningNoRU) return st->status == kRunning && st->ru == 0;
     return false;
 }
 
@@ -138,38 +143,41 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
     }
 }
 
-// All threads of the block must call this (with 0 contributions from idle
-// threads).  NV = 1 or 2 values per thread.
-template <int F, int NV = 1>
-__device__ __forceinline__ void grid_reduce_finalize(double v0, double v1, const SolverCtl& c) {
+// Block partial of a grid reduction, stored at partials[2*blockIdx.x (+1)].
+// All threads of the block must call this (idle threads contribute 0).
+// NV = 1 or 2 values per thread.
+template <int NV = 1>
+__device__ __forceinline__ void block_partial(double v0, double v1, const SolverCtl& c) {
     __shared__ double lds[2][kBlock / 64];
-    __shared__ int am_last;
-    double b0 = block_sum<kBlock>(v0, lds[0]);
+    const double b0 = block_sum<kBlock>(v0, lds[0]);
     double b1 = 0.0;
     if (NV == 2) b1 = block_sum<kBlock>(v1, lds[1]);
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&c.partials[2 * blockIdx.x], b0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (NV == 2)
-            __hip_atomic_store(&c.partials[2 * blockIdx.x + 1], b1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned int t = __hip_atomic_fetch_add(c.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (t == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    double a0 = 0.0, a1 = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) {
-        a0 += __hip_atomic_load(&c.partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (NV == 2) a1 += __hip_atomic_load(&c.partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    double t0 = block_sum<kBlock>(a0, lds[0]);
-    double t1 = 0.0;
-    if (NV == 2) t1 = block_sum<kBlock>(a1, lds[1]);
-    if (threadIdx.x == 0) {
-        finalize<F>(c.st, t0, t1);
-        __hip_atomic_store(c.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        c.partials[2 * blockIdx.x] = b0;
+        if (NV == 2) c.partials[2 * blockIdx.x + 1] = b1;
     }
 }
+
+// Sum `nparts` block partials in block order and apply finalizer F.  One
+// block; launched on the stream of the reducing kernel, with its SolverCtl
+// (the predicate reads state that only finalizers write, so it evaluates as
+// it did for the reducing kernel).
+template <int F, int NV>
+__global__ __launch_bounds__(kBlock) void k_finalize(SolverCtl c, int nparts) {
+    if (!pred_pass(c)) return;
+    __shared__ double lds[2][kBlock / 64];
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) {
+        a0 += c.partials[2 * i];
+        if (NV == 2) a1 += c.partials[2 * i + 1];
+    }
+    const double t0 = block_sum<kBlock>(a0, lds[0]);
+    double t1 = 0.0;
+    if (NV == 2) t1 = block_sum<kBlock>(a1, lds[1]);
+    if (threadIdx.x == 0) finalize<F>(c.st, t0, t1);
+}
+
+// Host: queue k_finalize<fin> for a reduction over `nparts` blocks.
+bool launch_finalize(int fin, const SolverCtl& c, int nparts);
 
 }  // namespace ostp

@@ -10,8 +10,8 @@ constexpr int kMaxShifts = MAX_APPROX_ORDER;
 constexpr int kBlock = 256;
 
 // Scalar state of a solver, resident in device memory so that iterations can be
-// queued without a host round trip per reduction.  Written only by the last
-// block of a reducing kernel (thread 0).
+// queued without a host round trip per reduction.  Written only by the
+// finalize kernel that closes each reduction (thread 0).
 enum SolverStatus : int { kRunning = 0, kInnerExit = 1, kDone = 2 };
 
 struct SolverState {
@@ -37,9 +37,8 @@ struct Context {
     hipStream_t stream = nullptr;
     int device = 0;
     int abort_on_error = 1;
-    // device scratch: reduction partials + tickets + solver state
+    // device scratch: reduction partials + solver state
     double* partials = nullptr;     // 2 * max_blocks
-    unsigned int* tickets = nullptr;  // 8 counters, zero between launches
     SolverState* state = nullptr;   // device
     SolverState* state_host = nullptr;  // pinned mirror
     int max_blocks = 0;

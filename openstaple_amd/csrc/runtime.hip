@@ -124,11 +124,9 @@ bool push_state() {
 
 static void release_scratch() {
     if (g_ctx.partials) (void)hipFree(g_ctx.partials);
-    if (g_ctx.tickets) (void)hipFree(g_ctx.tickets);
     if (g_ctx.state) (void)hipFree(g_ctx.state);
     if (g_ctx.state_host) (void)hipHostFree(g_ctx.state_host);
     g_ctx.partials = nullptr;
-    g_ctx.tickets = nullptr;
     g_ctx.state = nullptr;
     g_ctx.state_host = nullptr;
 }
@@ -158,11 +156,9 @@ int ostp_init(int nd0, int nd1, int nd2, int nd3, int par_temp) {
     if (!c.partials || blocks > c.max_blocks) {
         release_scratch();
         if (!hip_ok(hipMalloc(&c.partials, sizeof(double) * 4 * (size_t)blocks), "hipMalloc partials")) return 1;
-        if (!hip_ok(hipMalloc(&c.tickets, sizeof(unsigned int) * 16), "hipMalloc tickets")) return 1;
         if (!hip_ok(hipMalloc(&c.state, sizeof(SolverState)), "hipMalloc state")) return 1;
         if (!hip_ok(hipHostMalloc(&c.state_host, sizeof(SolverState), hipHostMallocDefault), "hipHostMalloc"))
             return 1;
-        if (!hip_ok(hipMemset(c.tickets, 0, sizeof(unsigned int) * 16), "hipMemset tickets")) return 1;
         c.max_blocks = blocks;
     }
     c.lat = L;

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(C* __restrict__ x, C* __restri
             v += (double)rn.x * rn.x + (double)rn.y * rn.y;
         }
     }
-    grid_reduce_finalize<kFinCgLambda, 1>(v, 0.0, ctl);
+    block_partial<1>(v, 0.0, ctl);
 }
 
 // p = p*gammag + r   (inverter_full.c:96)
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_restart(const double2* __restrict
             v += rr.x * rr.x + rr.y * rr.y;
         }
     }
-    grid_reduce_finalize<F, 1>(v, 0.0, ctl);
+    block_partial<1>(v, 0.0, ctl);
 }
 
 // r += omega s ; lambda = |r|^2   (inverter_multishift_full.c:141-142)
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_r(C* __restrict__ r, const C* __r
             v += (double)rn.x * rn.x + (double)rn.y * rn.y;
         }
     }
-    grid_reduce_finalize<kFinMsLambda, 1>(v, 0.0, ctl);
+    block_partial<1>(v, 0.0, ctl);
 }
 
 // p = p*gammag + r ; for active shifts k: x_k -= omega_k p_k ; p_k = gamma_k p_k + zeta_k r
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_init(const double2* __restrict__ 
             v += (double)f.x * f.x + (double)f.y * f.y;
         }
     }
-    grid_reduce_finalize<kFinCgRestart, 1>(v, 0.0, ctl);
+    block_partial<1>(v, 0.0, ctl);
 }
 
 // out_f += omega p_f; then either r_f -= omega s_f with lambda = |r_f|^2, or
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_xr(float2* __restrict__ outf, con
             }
         }
     }
-    if (!ru) grid_reduce_finalize<kFinMxLambda, 1>(v, 0.0, ctl);
+    if (!ru) block_partial<1>(v, 0.0, ctl);
 }
 
 // reliable update tail: r = b - s (FP64), r_f = (float) r, lambda = |r_f|^2  (inverter_mixedp.c:121-124,137)
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_resid(const double2* __restrict__
             v += (double)f.x * f.x + (double)f.y * f.y;
         }
     }
-    grid_reduce_finalize<kFinMxLambda, 1>(v, 0.0, ctl);
+    block_partial<1>(v, 0.0, ctl);
 }
 
 // x += (double) out_f   (combine_add_in2_into_in1_mixed_precision, inverter_mixedp.c:24-34)
@@ -304,13 +304,14 @@ int cg_solve(const CgBufs& B, double mass, double shift, double res, int max_cg,
         if (!launch_mdagm<double>(B.u, B.ph, B.x, B.h, B.s, m2, 1, kFinStore, ctl_of(kAlways))) return false;
         hipLaunchKernelGGL((k_cg_restart<kFinCgRestart>), dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b,
                            (const double2*)B.s, (double2*)B.r, (double2*)B.p, (int)sizeh, ctl_of(kAlways));
-        return check_launch("k_cg_restart");
+        return check_launch("k_cg_restart") && launch_finalize(kFinCgRestart, ctl_of(kAlways), nbh);
     };
     auto iteration = [&](int j) -> bool {
         // s = M p ; alpha = (p,s) ; omega = delta/alpha
         if (!launch_mdagm<double>(B.u, B.ph, B.p, B.h, B.s, m2, 2, kFinCgAlpha, ctl_of(kIfRunning))) return false;
         hipLaunchKernelGGL((k_cg_xr<double2>), dim3(nbh), dim3(kBlock), 0, S(), (double2*)B.x, (double2*)B.r,
                            (const double2*)B.p, (const double2*)B.s, (int)sizeh, ctl_of(kIfRunning));
+        if (!launch_finalize(kFinCgLambda, ctl_of(kIfRunning), nbh)) return false;
         hipLaunchKernelGGL((k_cg_p<double2>), dim3(nb3), dim3(kBlock), 0, S(), (double2*)B.p, (const double2*)B.r, n3,
                            ctl_of(kIfIter, j));
         return check_launch("cg iteration");
@@ -331,6 +332,7 @@ int cg_solve(const CgBufs& B, double mass, double shift, double res, int max_cg,
     if (!launch_mdagm<double>(B.u, B.ph, B.x, B.h, B.s, m2, 1, kFinStore, ctl_of(kAlways))) return INVERTER_FAILURE;
     hipLaunchKernelGGL((k_cg_restart<kFinStore>), dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b,
                        (const double2*)B.s, (double2*)B.h, (double2*)nullptr, (int)sizeh, ctl_of(kAlways));
+    if (!launch_finalize(kFinStore, ctl_of(kAlways), nbh)) return INVERTER_FAILURE;
     if (!check_launch("final residual") || !pull_state()) return INVERTER_FAILURE;
     const double current_res = H().sum / sn;
     if (verbosity_lv > 1)
@@ -367,17 +369,19 @@ int mixed_solve(const MxBufs& B, double mass, double shift, double res, int max_
     if (!launch_mdagm<double>(B.u, B.ph, B.x, B.h, B.s, m2, 1, kFinStore, ctl_of(kAlways))) return INVERTER_FAILURE;
     hipLaunchKernelGGL(k_mx_init, dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b, (const double2*)B.s,
                        (double2*)B.r, (float2*)B.rf, (float2*)B.pf, (float2*)B.outf, (int)sizeh, ctl_of(kAlways));
-    if (!check_launch("k_mx_init")) return INVERTER_FAILURE;
+    if (!check_launch("k_mx_init") || !launch_finalize(kFinCgRestart, ctl_of(kAlways), nbh)) return INVERTER_FAILURE;
 
     auto iteration = [&](int j) -> bool {
         if (!launch_mdagm<float>(B.u_f, B.ph_f, B.pf, B.hf, B.sf, m2f, 2, kFinMxAlpha, ctl_of(kIfRunning)))
             return false;
         hipLaunchKernelGGL(k_mx_xr, dim3(nbh), dim3(kBlock), 0, S(), (float2*)B.outf, (const float2*)B.pf,
                            (float2*)B.rf, (const float2*)B.sf, (double2*)B.x, (int)sizeh, ctl_of(kIfRunning));
+        if (!launch_finalize(kFinMxLambda, ctl_of(kIfRunningNoRU), nbh)) return false;
         // reliable update: s = M x (FP64), r = b - s, r_f = (float) r
         if (!launch_mdagm<double>(B.u, B.ph, B.x, B.h, B.s, m2, 1, kFinStore, ctl_of(kIfRunningRU))) return false;
         hipLaunchKernelGGL(k_mx_resid, dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b, (const double2*)B.s,
                            (double2*)B.r, (float2*)B.rf, (int)sizeh, ctl_of(kIfRunningRU));
+        if (!launch_finalize(kFinMxLambda, ctl_of(kIfRunningRU), nbh)) return false;
         hipLaunchKernelGGL((k_cg_p<float2>), dim3(nb3), dim3(kBlock), 0, S(), (float2*)B.pf, (const float2*)B.rf, n3,
                            ctl_of(kIfIter, j));
         return check_launch("mixed iteration");
@@ -389,6 +393,7 @@ int mixed_solve(const MxBufs& B, double mass, double shift, double res, int max_
     if (!launch_mdagm<double>(B.u, B.ph, B.x, B.h, B.s, m2, 1, kFinStore, ctl_of(kAlways))) return INVERTER_FAILURE;
     hipLaunchKernelGGL((k_cg_restart<kFinStore>), dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b,
                        (const double2*)B.s, (double2*)B.h, (double2*)nullptr, (int)sizeh, ctl_of(kAlways));
+    if (!launch_finalize(kFinStore, ctl_of(kAlways), nbh)) return INVERTER_FAILURE;
     if (!check_launch("final residual") || !pull_state()) return INVERTER_FAILURE;
     const double giustoono = H().sum / sn;
     if (verbosity_lv > 1)
@@ -456,6 +461,7 @@ int multishift_solve(const MsBufs& B, double mass, int order, const double* shif
         if (!launch_mdagm<R>(B.u, B.ph, B.p, B.h, B.s, m2, 2, kFinMsAlpha, ctl_of(kIfRunning))) return false;
         hipLaunchKernelGGL((k_ms_r<C>), dim3(nbh), dim3(kBlock), 0, S(), (C*)B.r, (const C*)B.s, (int)sizeh,
                            ctl_of(kIfRunning));
+        if (!launch_finalize(kFinMsLambda, ctl_of(kIfRunning), nbh)) return false;
         hipLaunchKernelGGL((k_ms_update<C>), dim3(nb3), dim3(kBlock), 0, S(), (const C*)B.r, (C*)B.p, (C*)B.out,
                            (C*)B.shiftferm, vstride, n3, ctl_of(kIfIter, j));
         return check_launch("multishift iteration");
@@ -475,6 +481,7 @@ int multishift_solve(const MsBufs& B, double mass, int order, const double* shif
         if (sizeof(R) == 8) {
             hipLaunchKernelGGL((k_cg_restart<kFinStore>), dim3(nbh), dim3(kBlock), 0, S(), (const double2*)B.b,
                                (const double2*)B.s, (double2*)B.h, (double2*)nullptr, (int)sizeh, ctl_of(kAlways));
+            if (!launch_finalize(kFinStore, ctl_of(kAlways), nbh)) return INVERTER_FAILURE;
             if (!check_launch("verify") || !pull_state()) return INVERTER_FAILURE;
             g = H().sum / sn;
         } else {
