@@ -45,6 +45,8 @@ typedef struct RationalApprox_t RationalApprox; /* RationalApprox/rationalapprox
 typedef struct inverter_package_t inverter_package; /* OpenAcc/inverter_package.h:12-29 */
 typedef struct bf_param_t bf_param;         /* OpenAcc/backfield_parameters.h:5-9 */
 typedef struct inv_tricks_t inv_tricks;     /* Include/inverter_tricks.h:4-11 */
+typedef struct rep_info_t rep_info;         /* Include/rep_info.h:4-11 */
+typedef struct defect_info_t defect_info;   /* OpenAcc/HPT_utilities.h:14-32 */
 
 /* Reference globals the replaced code reads or writes.  The library defines
  * them with the reference defaults; a host program that defines them itself
@@ -224,6 +226,56 @@ void fermion_force_soloopenacc(su3_soa *tconf_acc, su3_soa *tstout_conf_acc_arr,
                                tamat_soa *tipdot_acc, ferm_param *tfermion_parameters, int tNDiffFlavs,
                                const vec3_soa *ferm_in_acc, double res, su3_soa *taux_conf_acc,
                                vec3_soa *tferm_shiftmulti_acc, inverter_package ipt, const int max_cg);
+
+/* ============================================ even-odd full inversion ==== */
+/* fermionic_utilities.c:379-401: in2 = fact1*in1 - in2 */
+void combine_in1_x_fact1_minus_in2_back_into_in2(const vec3_soa *in1, double fact1, vec3_soa *in2);
+/* fermionic_utilities.c:403-419: out = fact*(in1 - in2) */
+void combine_in1_minus_in2_allxfact(const vec3_soa *in1, const vec3_soa *in2, double fact, vec3_soa *out);
+/* Meas/ferm_meas.c:50-71: (out_e, out_o) = M^-1 (in_e, in_o) for the full
+ * staggered matrix M = [[m, Deo], [Doe, m]] through the even-odd system
+ * (m^2 - Deo Doe) out_e = m in_e - Deo in_o (inverter_wrapper, critical),
+ * out_o = (in_o - Doe out_e) / m.  phi_e, phi_o are scratch. */
+void eo_inversion(inverter_package ip, ferm_param *tfermions_parameters, double res, int max_cg,
+                  vec3_soa *in_e, vec3_soa *in_o, vec3_soa *out_e, vec3_soa *out_o, vec3_soa *phi_e,
+                  vec3_soa *phi_o);
+
+/* ================================== parallel tempering (PAR_TEMP builds) === */
+/* OpenAcc/HPT_utilities.c:30-310.  K = 1 on every link, K = c_r on the links
+ * of direction def_axis (mapped through ostp_set_direction_map) whose site
+ * lies on the defect: coordinate nd-1 along def_axis, [0, def_vec[i]) along
+ * the three orthogonal directions (x-y-z-t order).  When defect_info_config
+ * is 0 the Delta-S swap ranges in *def are filled as well (single rank).
+ * Needs ostp_init(..., par_temp = 1): K is the plane after row 2 of each
+ * su3_soa. */
+void init_k(su3_soa *conf, double c_r, int def_axis, int *def_vec, defect_info *def, int defect_info_config);
+
+/* Replica label bookkeeping of manage_replica_swaps (HPT_utilities.c:722-1003)
+ * for replicas held by one process.  Labels: replica index r carries label
+ * label[r]; replica r runs with coefficient cr_vec[label[r]].
+ *   action(cr_of_replica, label, S_of_label, user) must store, for every
+ *     replica r, the defect action of replica r evaluated with coefficient
+ *     cr_of_replica[r] at S_of_label[label[r]] (compute_S_of_replicas);
+ *   uniform(user) returns a uniform number in [0,1) (the reference's casuale()).
+ * Random numbers are drawn in the reference's order: one for the swap order,
+ * then one per proposed pair whose Delta S is not negative.  On return every
+ * replica r runs with cr_vec[label[r]] again (accepted pairs exchanged their
+ * labels); swap_num, all_swap_vet[lab] and acceptance_vet[lab] are
+ * incremented as in the reference (indexed by the lower label of a pair). */
+typedef void (*ostp_pt_action_fn)(const double *cr_of_replica, const int *label, double *S_of_label, void *user);
+typedef double (*ostp_pt_uniform_fn)(void *user);
+void ostp_pt_manage_replica_swaps(rep_info *hpt_params, ostp_pt_action_fn action, ostp_pt_uniform_fn uniform,
+                                  void *user, int *swap_num, int *all_swap_vet, int *acceptance_vet);
+/* main.c:341-342: label[r] = r */
+void ostp_pt_init_labels(rep_info *hpt_params);
+/* rep_info.c:8-14 (get_index_of_pbc_replica): replica index carrying label 0, -1 if none */
+int ostp_pt_index_of_label0(const rep_info *hpt_params);
+/* metro_SWAP_worldmaster (HPT_utilities.c:704-719) with the uniform draw
+ * passed in: accept if Delta_S < 0, else if u < exp(-Delta_S). */
+int ostp_pt_metropolis(double delta_s_swap, double u);
+/* Partner label of `lab` in an even/odd stage (HPT_utilities.c:769); -1 if
+ * the label sits out this stage. */
+int ostp_pt_twin_label(int lab, int swap_order, int nreplicas);
 
 /* ======================================================== generators ===== */
 /* Reference field rules (OpenAcc/random_assignement.c:67-92,106-142,188-250)

@@ -87,6 +87,34 @@ struct inv_tricks_t {       /* Include/inverter_tricks.h:4-11 */
 
 extern inv_tricks inverter_tricks; /* Include/inverter_tricks.c:6 */
 
+struct rep_info_t {         /* Include/rep_info.h:4-11 */
+    int replicas_total_number;
+    int is_evenodd;
+    int defect_boundary;
+    int defect_coordinates[3];
+    double *cr_vec;
+    int *label;
+};
+
+/* OpenAcc/HPT_utilities.h:14-32 as compiled with GAUGE_ACT_TLSM (the
+ * tree-level Symanzik action, the library default).  With the Wilson action
+ * (ostp_set_stout(..., tlsm = 0)) the reference struct has no *_TLSM members;
+ * init_k then writes the ostp_defect_info_wilson layout below. */
+struct defect_info_t {
+    int defect_swap_min[4][4];
+    int defect_swap_max[4][4];
+    int defect_swap_min_TLSM[2][4][4];
+    int defect_swap_max_TLSM[2][4][4];
+    int def_axis_mapped;
+    int def_mapped_perp_dir[3];
+};
+typedef struct ostp_defect_info_wilson_t {
+    int defect_swap_min[4][4];
+    int defect_swap_max[4][4];
+    int def_axis_mapped;
+    int def_mapped_perp_dir[3];
+} ostp_defect_info_wilson;
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,9 +92,44 @@ class BfParam(C.Structure):  # OpenAcc/backfield_parameters.h:5-9
     _fields_ = [(n, C.c_double) for n in ("ex", "ey", "ez", "bx", "by", "bz")]
 
 
+class RepInfo(C.Structure):  # Include/rep_info.h:4-11
+    _fields_ = [
+        ("replicas_total_number", C.c_int),
+        ("is_evenodd", C.c_int),
+        ("defect_boundary", C.c_int),
+        ("defect_coordinates", C.c_int * 3),
+        ("cr_vec", C.POINTER(C.c_double)),
+        ("label", C.POINTER(C.c_int)),
+    ]
+
+
+class DefectInfo(C.Structure):  # OpenAcc/HPT_utilities.h:14-32 (GAUGE_ACT_TLSM layout)
+    _fields_ = [
+        ("defect_swap_min", (C.c_int * 4) * 4),
+        ("defect_swap_max", (C.c_int * 4) * 4),
+        ("defect_swap_min_TLSM", ((C.c_int * 4) * 4) * 2),
+        ("defect_swap_max_TLSM", ((C.c_int * 4) * 4) * 2),
+        ("def_axis_mapped", C.c_int),
+        ("def_mapped_perp_dir", C.c_int * 3),
+    ]
+
+
+class DefectInfoWilson(C.Structure):  # the same struct compiled without GAUGE_ACT_TLSM
+    _fields_ = [
+        ("defect_swap_min", (C.c_int * 4) * 4),
+        ("defect_swap_max", (C.c_int * 4) * 4),
+        ("def_axis_mapped", C.c_int),
+        ("def_mapped_perp_dir", C.c_int * 3),
+    ]
+
+
+PT_ACTION_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_void_p)
+PT_UNIFORM_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
+
 assert C.sizeof(RationalApprox) == 456
 assert FermParam.phases.offset == 72 and FermParam.phases_f.offset == 104
 assert C.sizeof(InverterPackage) == 112
+assert C.sizeof(RepInfo) == 40 and C.sizeof(DefectInfo) == 400 and C.sizeof(DefectInfoWilson) == 144
 
 # ------------------------------------------------------------------ signatures
 _P = C.c_void_p
@@ -177,6 +212,16 @@ _SIGS = {
     "multiply_conf_times_force_and_take_ta_nophase": (None, [_P, _P, _P]),
     "ker_openacc_compute_fermion_force": (None, [_P, _P, _P, _P, _P, _P]),
     "fermion_force_soloopenacc": (None, [_P, _P, _P, _P, _P, _I, _P, _D, _P, _P, InverterPackage, _I]),
+    "combine_in1_x_fact1_minus_in2_back_into_in2": (None, [_P, _D, _P]),
+    "combine_in1_minus_in2_allxfact": (None, [_P, _P, _D, _P]),
+    "eo_inversion": (None, [InverterPackage, _P, _D, _I, _P, _P, _P, _P, _P, _P]),
+    "init_k": (None, [_P, _D, _I, C.POINTER(_I), _P, _I]),
+    "ostp_pt_manage_replica_swaps": (None, [C.POINTER(RepInfo), PT_ACTION_FN, PT_UNIFORM_FN, _P, C.POINTER(_I),
+                                            C.POINTER(_I), C.POINTER(_I)]),
+    "ostp_pt_init_labels": (None, [C.POINTER(RepInfo)]),
+    "ostp_pt_index_of_label0": (_I, [C.POINTER(RepInfo)]),
+    "ostp_pt_metropolis": (_I, [_D, _D]),
+    "ostp_pt_twin_label": (_I, [_I, _I, _I]),
     "ostp_time_dslash": (_D, [_P, _P, _P, _P, _I, _I, _I]),
     "ostp_time_copy": (_D, [_P, _P, C.c_size_t, _I]),
 }

@@ -30,6 +30,15 @@ __global__ __launch_bounds__(kBlock) void k_sub(const C* __restrict__ in1, const
 }
 
 // in2 = in1*m2 - in2 (combine_in1xferm_mass_minus_in2)
+// out = fact * (in1 - in2)   (fermionic_utilities.c:403-419)
+__global__ __launch_bounds__(kBlock) void k_sub_scale(const double2* __restrict__ in1, const double2* __restrict__ in2,
+                                                      double fact, double2* __restrict__ out, long n3) {
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n3) return;
+    const double2 a = in1[i], b = in2[i];
+    out[i] = make_double2(fact * (a.x - b.x), fact * (a.y - b.y));
+}
+
 __global__ __launch_bounds__(kBlock) void k_mass_minus(const double2* __restrict__ in1, double m2,
                                                        double2* __restrict__ in2, long n3) {
     const long i = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -240,6 +249,30 @@ void combine_in1_minus_in2(const vec3_soa* in_vect1, const vec3_soa* in_vect2, v
     const void* b = VEC(in_vect2, "combine_in1_minus_in2");
     void* o = VEC(out, "combine_in1_minus_in2");
     if (a && b && o && launch_sub(a, b, o)) finish("combine_in1_minus_in2");
+}
+
+// in2 = fact1*in1 - in2: the same elementwise formula as
+// combine_in1xferm_mass_minus_in2 (a real factor times a complex number
+// commutes exactly), so the same kernel serves both.
+void combine_in1_x_fact1_minus_in2_back_into_in2(const vec3_soa* in1, double fact1, vec3_soa* in2) {
+    const char* nm = "combine_in1_x_fact1_minus_in2_back_into_in2";
+    if (!require_init(nm)) return;
+    const void* a = VEC(in1, nm);
+    void* b = VEC(in2, nm);
+    if (a && b && launch_mass_minus(a, fact1, b)) finish(nm);
+}
+
+void combine_in1_minus_in2_allxfact(const vec3_soa* in1, const vec3_soa* in2, double fact, vec3_soa* out) {
+    const char* nm = "combine_in1_minus_in2_allxfact";
+    if (!require_init(nm)) return;
+    const void* a = VEC(in1, nm);
+    const void* b = VEC(in2, nm);
+    void* o = VEC(out, nm);
+    if (!a || !b || !o) return;
+    const long n3 = 3 * ctx().lat.sizeh;
+    hipLaunchKernelGGL(k_sub_scale, dim3(nb(n3)), dim3(kBlock), 0, S(), (const double2*)a, (const double2*)b, fact,
+                       (double2*)o, n3);
+    if (hip_ok(hipGetLastError(), nm)) finish(nm);
 }
 
 void assign_in_to_out(const vec3_soa* in_vect1, vec3_soa* out) {

@@ -50,6 +50,14 @@ __device__ __forceinline__ void sincos_t(float a, float* s, float* c) {
     *c = (float)dc;
 }
 
+// Links and phases are read once per launch.  FP64: non-temporal loads
+// (measured 85.7 us vs 92.5 us per 32^4 Dslash); FP32: plain loads (36.3 us
+// vs 50.0 us with non-temporal 64-bit loads).
+__device__ __forceinline__ double2 ld_link(const double2* p) { return ld_nt(p); }
+__device__ __forceinline__ double ld_link(const double* p) { return ld_nt(p); }
+__device__ __forceinline__ float2 ld_link(const float2* p) { return *p; }
+__device__ __forceinline__ float ld_link(const float* p) { return *p; }
+
 template <typename C> __device__ __forceinline__ C mk(double a, double b);
 template <> __device__ __forceinline__ double2 mk<double2>(double a, double b) { return make_double2(a, b); }
 template <> __device__ __forceinline__ float2 mk<float2>(double a, double b) { return make_float2((float)a, (float)b); }
@@ -62,9 +70,9 @@ __device__ __forceinline__ void hop(const typename Cplx<R>::T* __restrict__ U, c
                                     typename Cplx<R>::T& o0, typename Cplx<R>::T& o1,
                                     typename Cplx<R>::T& o2) {
     using C = typename Cplx<R>::T;
-    const C a0 = ld_nt(U + iu), a1 = ld_nt(U + sizeh + iu), a2 = ld_nt(U + 2 * sizeh + iu);
-    const C b0 = ld_nt(U + 3 * sizeh + iu), b1 = ld_nt(U + 4 * sizeh + iu), b2 = ld_nt(U + 5 * sizeh + iu);
-    const R arg = ld_nt(P + iu);
+    const C a0 = ld_link(U + iu), a1 = ld_link(U + sizeh + iu), a2 = ld_link(U + 2 * sizeh + iu);
+    const C b0 = ld_link(U + 3 * sizeh + iu), b1 = ld_link(U + 4 * sizeh + iu), b2 = ld_link(U + 5 * sizeh + iu);
+    const R arg = ld_link(P + iu);
     C v0 = in[iv], v1 = in[sizeh + iv], v2 = in[2 * sizeh + iv];
     R s, c;
     sincos_t(arg, &s, &c);

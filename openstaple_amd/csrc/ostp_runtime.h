@@ -64,6 +64,9 @@ inline T* resolve_as(const void* p, size_t bytes, const char* what) {
 // Launch geometry over sizeh half-sites.
 inline int blocks_for(long n, int nt = kBlock) { return (int)((n + nt - 1) / nt); }
 
+// Gauge action selected by ostp_set_stout (true: tree-level Symanzik).
+bool gauge_action_tlsm();
+
 // Read the device solver state into the pinned mirror (synchronises the stream).
 bool pull_state();
 bool push_state();

@@ -664,6 +664,23 @@ int inverter_wrapper(inverter_package ip, ferm_param* pars, vec3_soa* out, const
     return cg_return;
 }
 
+// Meas/ferm_meas.c:50-71
+void eo_inversion(inverter_package ip, ferm_param* tfermions_parameters, double res, int max_cg, vec3_soa* in_e,
+                  vec3_soa* in_o, vec3_soa* out_e, vec3_soa* out_o, vec3_soa* phi_e, vec3_soa* phi_o) {
+    if (!require_init("eo_inversion")) return;
+    if (!tfermions_parameters) {
+        fail("eo_inversion: null ferm_param");
+        return;
+    }
+    const double m = tfermions_parameters->ferm_mass;
+    acc_Deo(ip.u, phi_e, in_o, tfermions_parameters->phases);
+    combine_in1_x_fact1_minus_in2_back_into_in2(in_e, m, phi_e);
+    set_vec3_soa_to_zero(out_e);
+    inverter_wrapper(ip, tfermions_parameters, out_e, phi_e, res, max_cg, 0, CONVERGENCE_CRITICAL);
+    acc_Doe(ip.u, phi_o, out_e, tfermions_parameters->phases);
+    combine_in1_minus_in2_allxfact(in_o, phi_o, (double)1 / m, out_o);
+}
+
 int inverter_multishift_wrapper(inverter_package ip, ferm_param* pars, RationalApprox* approx, vec3_soa* out,
                                 const vec3_soa* in, double res, int max_cg, int convergence_importance) {
     const char* nm = "inverter_multishift_wrapper";

@@ -28,6 +28,9 @@ static double g_c_zero = 5.0 * 0.33333333333333333333333;  // tree-level Symanzi
 static int g_stout_steps = 2;
 static int g_topo_steps = 0;
 static int g_topo_action = 0;
+static bool g_tlsm = true;
+
+bool gauge_action_tlsm() { return g_tlsm; }
 
 struct LinkGeom {
     int nd[4];
@@ -598,6 +601,7 @@ void ostp_set_stout(double rho, int steps, double topo_rho, int topo_steps, int 
     g_topo_steps = topo_steps;
     g_topo_action = topo_action;
     g_c_zero = tlsm ? 5.0 * 0.33333333333333333333333 : 1.0;
+    g_tlsm = tlsm != 0;
 }
 
 void stout_isotropic(const su3_soa* u, su3_soa* uprime, su3_soa* local_staples, su3_soa* auxiliary,

@@ -758,3 +758,23 @@ void or_recombine(const or_geom *g, const void *in_shifted, const void *in, void
         }
     }
 }
+
+/* eo_inversion (Meas/ferm_meas.c:50-71) with the plain FP64 CG of
+ * inverter_wrapper (useMixedPrecision = 0): returns the CG status. */
+int or_eo_inversion(const or_geom *g, const void *u, const double *phases, double mass, double res, int max_cg,
+                    const void *in_e, const void *in_o, void *out_e, void *out_o, void *phi_e, void *phi_o,
+                    void *r, void *h, void *s, void *p, const or_tricks *tr, int *cg_return)
+{
+    const long n3 = 3 * g->sizeh;
+    const or_dc *ie = (const or_dc *)in_e, *io = (const or_dc *)in_o;
+    or_dc *pe = (or_dc *)phi_e, *po = (or_dc *)phi_o, *oe = (or_dc *)out_e, *oo = (or_dc *)out_o;
+    long i;
+    or_acc_Deo(g, u, phi_e, in_o, phases);
+    for (i = 0; i < n3; i++) pe[i] = mass * ie[i] - pe[i];   /* fermionic_utilities.c:393-395 */
+    for (i = 0; i < n3; i++) oe[i] = 0.0;
+    int st = or_ker_invert(g, u, phases, mass, out_e, phi_e, res, r, h, s, p, max_cg, 0.0, tr, cg_return);
+    or_acc_Doe(g, u, phi_o, out_e, phases);
+    const double f = (double)1 / mass;
+    for (i = 0; i < n3; i++) oo[i] = f * (io[i] - po[i]);    /* fermionic_utilities.c:413-415 */
+    return st;
+}

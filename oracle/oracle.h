@@ -132,4 +132,24 @@ int or_ratapprox_read(or_ratapprox *ra, const char *filename);
 double or_ratapprox_evaluate(const or_ratapprox *ra, double x);
 void or_ratapprox_rescale(const or_ratapprox *in, or_ratapprox *out, const double minmax[2]);
 
+/* eo_inversion (Meas/ferm_meas.c:50-71): full-matrix solve through the
+ * even-odd CG; returns the CG status */
+int or_eo_inversion(const or_geom *g, const void *u, const double *phases, double mass, double res, int max_cg,
+                    const void *in_e, const void *in_o, void *out_e, void *out_o, void *phi_e, void *phi_o,
+                    void *r, void *h, void *s, void *p, const or_tricks *tr, int *cg_return);
+
+/* ---- parallel tempering (OpenAcc/HPT_utilities.c), single rank */
+/* init_k (:30-310); conf = su3_soa[8] with K planes (par_temp geometry), map =
+ * {xmap,ymap,zmap,tmap}; def_out = 100 ints in the GAUGE_ACT_TLSM layout of
+ * defect_info (min[4][4], max[4][4], min_TLSM[2][4][4], max_TLSM[2][4][4],
+ * def_axis_mapped, def_mapped_perp_dir[3]), written when defect_info_config == 0 */
+void or_init_k(const or_geom *g, const int map[4], void *conf, double c_r, int def_axis, const int def_vec[3],
+               int *def_out, int defect_info_config);
+int or_metro_swap(double delta_s, double u);
+typedef void (*or_pt_action_fn)(const double *cr_of_replica, const int *label, double *S_of_label, void *user);
+typedef double (*or_pt_uniform_fn)(void *user);
+void or_manage_replica_swaps(int n, int is_evenodd, const double *cr_vec, int *label, or_pt_action_fn action,
+                             or_pt_uniform_fn uniform, void *user, int *swap_num, int *all_swap_vet,
+                             int *acceptance_vet);
+
 #endif

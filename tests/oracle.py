@@ -154,6 +154,29 @@ class Oracle:
                                C.byref(tr), C.byref(cg))
         return st, cg.value, x
 
+    def eo_inversion(self, u, ph, mass, b_e, b_o, res, max_cg=10000):
+        x_e, x_o, phi_e, phi_o, r, h, s, pp = (self.vec3() for _ in range(8))
+        tr = Tricks(10000, 0.1)
+        cg = C.c_int(0)
+        st = lib.or_eo_inversion(self.gp, p(u), p(ph), mass, res, max_cg, p(b_e), p(b_o), p(x_e), p(x_o), p(phi_e),
+                                 p(phi_o), p(r), p(h), p(s), p(pp), C.byref(tr), C.byref(cg))
+        return st, cg.value, x_e, x_o
+
+    def init_k(self, conf, c_r, def_axis, def_vec, map4=(0, 1, 2, 3), defect_info_config=0):
+        """K planes of `conf` (par_temp layout, uint8 view) set in place; returns
+        the 100-int defect_info (GAUGE_ACT_TLSM layout)."""
+        assert self.par_temp
+        out = np.zeros(100, dtype=np.int32)
+        lib.or_init_k(self.gp, (C.c_int * 4)(*map4), p(conf), c_r, def_axis, (C.c_int * 3)(*def_vec), p(out),
+                      defect_info_config)
+        return out
+
+    def k_planes(self, conf):
+        """(8, sizeh) view of the K planes of a par_temp gauge field."""
+        assert self.par_temp
+        per = 152 * self.sizeh
+        return np.stack([conf[l * per + 144 * self.sizeh:(l + 1) * per].view(np.float64) for l in range(8)])
+
     def mixed(self, u, u_f, ph, ph_f, mass, b, res, max_cg=10000, shift=0.0, delta=0.1, x0=None):
         x = self.vec3() if x0 is None else x0.copy()
         tr = Tricks(10000, delta)
@@ -211,6 +234,24 @@ class Oracle:
         return u.view(np.complex128).reshape(8, 3, 3, self.sizeh)
 
 
+PT_ACTION = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_void_p)
+PT_UNIFORM = C.CFUNCTYPE(C.c_double, C.c_void_p)
+lib.or_manage_replica_swaps.argtypes = [_I, _I, _P, _P, PT_ACTION, PT_UNIFORM, _P, C.POINTER(_I), _P, _P]
+
+
+def manage_replica_swaps(n, is_evenodd, cr_vec, label, action, uniform, counters):
+    """Oracle label manager; label (int32 array) and counters = (swap_num
+    c_int, all_swap_vet int32[n], acceptance_vet int32[n]) updated in place."""
+    swap_num, all_swap, acc = counters
+    cr = np.ascontiguousarray(cr_vec, dtype=np.float64)
+    lib.or_manage_replica_swaps(n, int(is_evenodd), p(cr), p(label), action, uniform, None, C.byref(swap_num),
+                                p(all_swap), p(acc))
+
+
+def metro_swap(ds, u):
+    return lib.or_metro_swap(C.c_double(ds), C.c_double(u))
+
+
 def philox(ctr, key):
     c = (C.c_uint * 4)(*ctr)
     k = (C.c_uint * 2)(*key)
@@ -241,6 +282,10 @@ for name, res, args in [
     ("or_direct_product_into_auxmat", None, [_G, _P, _P, _P, _D]),
     ("or_multiply_backfield_times_force", None, [_G, _P, _P, _P]),
     ("or_conf_times_force_ta", None, [_G, _P, _P, _P]),
+    ("or_eo_inversion", _I, [_G, _P, _P, _D, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(Tricks),
+                             C.POINTER(_I)]),
+    ("or_init_k", None, [_G, _P, _P, _D, _I, _P, _P, _I]),
+    ("or_metro_swap", _I, [_D, _D]),
 ]:
     f = getattr(lib, name)
     f.restype = res

@@ -17,6 +17,7 @@ def declared_functions():
     for h in HEADERS:
         txt = h.read_text()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        txt = re.sub(r"^typedef[^;]*;", "", txt, flags=re.M | re.S)  # type and callback typedefs
         for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", txt, flags=re.M):
             name = m.group(1)
             if name in ("if", "while", "for", "return", "sizeof", "defined"):

@@ -1,0 +1,117 @@
+"""GPU parity of the PAR_TEMP K planes (init_k) and of the even-odd full
+inversion (eo_inversion) through the C ABI.
+
+Bar: K planes and defect_info bit-exact against the oracle (which
+tests/test_oracle_hpt.py pins to an independent enumeration of the defect);
+eo_inversion: the full staggered system M x = b with M = [[m, Deo], [Doe, m]]
+solved to the requested tolerance, evaluated with the oracle's Dslash, and the
+solution within the same tolerance of the oracle's eo_inversion.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import openstaple_amd as ost  # noqa: E402
+from openstaple_amd import lib  # noqa: E402
+from gpu_util import Dev, init_lattice  # noqa: E402
+from oracle import Oracle  # noqa: E402
+
+K_CASES = [
+    ((24, 24, 24, 8), (0, 1, 2, 3), 3, (24, 24, 24), 0.6),   # PT replicas of the batched workload
+    ((24, 24, 24, 8), (0, 1, 2, 3), 0, (4, 3, 2), 0.2),
+    ((4, 6, 8, 10), (3, 0, 1, 2), 1, (2, 5, 3), 0.4),
+    ((6, 4, 4, 2), (1, 2, 3, 0), 2, (3, 4, 1), 1.0),
+    ((4, 4, 4, 4), (0, 1, 2, 3), 2, (0, 4, 4), 0.8),          # empty defect
+]
+
+
+@pytest.mark.parametrize("shape,map4,def_axis,def_vec,c_r", K_CASES)
+@pytest.mark.parametrize("tlsm", [1, 0])
+def test_init_k_bitexact(shape, map4, def_axis, def_vec, c_r, tlsm):
+    o = Oracle(*shape, par_temp=True)
+    init_lattice(shape, par_temp=True)
+    ost.check(lib.ostp_set_direction_map(*map4), "map")
+    lib.ostp_set_stout(0.15, 2, 0.0, 0, 0, tlsm)
+    u = o.gen_cold(0.5, 3)
+    ref = u.copy()
+    ref_def = o.init_k(ref, c_r, def_axis, def_vec, map4)
+    du = Dev(u)
+    dvec = (C.c_int * 3)(*def_vec)
+    di = ost.DefectInfo() if tlsm else ost.DefectInfoWilson()
+    lib.init_k(du.p, c_r, def_axis, dvec, C.byref(di), 0)
+    ost.check(0, "init_k")
+    got = du.get(u)
+    np.testing.assert_array_equal(got, ref)  # rows untouched, K planes bit-exact
+    got_def = np.frombuffer(bytes(di), dtype=np.int32)
+    if tlsm:
+        np.testing.assert_array_equal(got_def, ref_def)
+    else:  # Wilson layout: min, max, def_axis_mapped, perp
+        np.testing.assert_array_equal(got_def, np.concatenate([ref_def[:32], ref_def[96:100]]))
+    # defect_info_config = 1 leaves *def alone and rewrites K only
+    di2 = ost.DefectInfo()
+    lib.init_k(du.p, 1.0 - c_r, def_axis, dvec, C.byref(di2), 1)
+    ost.check(0, "init_k config 1")
+    assert not any(bytes(di2))
+    ref2 = u.copy()
+    o.init_k(ref2, 1.0 - c_r, def_axis, def_vec, map4, defect_info_config=1)
+    np.testing.assert_array_equal(du.get(u), ref2)
+    ost.check(lib.ostp_set_direction_map(0, 1, 2, 3), "map")
+    lib.ostp_set_stout(0.15, 2, 0.0, 0, 0, 1)
+
+
+def test_init_k_needs_par_temp_layout():
+    init_lattice((4, 4, 4, 4), par_temp=False)
+    d = Dev(nbytes=8 * 144 * 128)
+    lib.init_k(d.p, 0.5, 3, (C.c_int * 3)(1, 1, 1), None, 1)
+    assert "par_temp" in ost.lib.ostp_last_error().decode()
+    lib.ostp_clear_error()
+
+
+def full_matrix_residual(o, u, ph, m, xe, xo, be, bo):
+    """|M x - b| / |b| for M = [[m, Deo], [Doe, m]] evaluated by the oracle."""
+    re = m * xe + o.deo(u, xo, ph) - be
+    ro = o.doe(u, xe, ph) + m * xo - bo
+    num = np.sqrt(np.linalg.norm(re) ** 2 + np.linalg.norm(ro) ** 2)
+    den = np.sqrt(np.linalg.norm(be) ** 2 + np.linalg.norm(bo) ** 2)
+    return num / den
+
+
+@pytest.mark.parametrize("shape,mass,mixed", [((8, 8, 8, 8), 0.05, 0), ((4, 6, 8, 10), 0.1, 0),
+                                              ((8, 8, 8, 8), 0.05, 1)])
+def test_eo_inversion(shape, mass, mixed):
+    o = Oracle(*shape)
+    init_lattice(shape)
+    u = o.gen_expH(0.3, 1)
+    ph = o.phases()
+    be, bo = o.gen_z2(7, 0), o.gen_z2(7, 1)
+    res = 1e-10
+    du, dph = Dev(u), Dev(ph)
+    uf = o.to_float_gauge(u)
+    duf, dphf = Dev(uf), Dev(ph.astype(np.float32))
+    fp = ost.ferm_param(mass, dph.p, dphf.p)
+    vb = be.nbytes
+    d64 = [Dev(nbytes=vb) for _ in range(4)]
+    d32 = [Dev(nbytes=vb // 2) for _ in range(5)]
+    ip = ost.InverterPackage()
+    ip.u, ip.u_f = du.p, duf.p
+    ip.loc_r, ip.loc_h, ip.loc_s, ip.loc_p = (d.p for d in d64)
+    ip.loc_r_f, ip.loc_h_f, ip.loc_s_f, ip.loc_p_f, ip.out_f = (d.p for d in d32)
+    dbe, dbo = Dev(be), Dev(bo)
+    xe, xo, phe, pho = (Dev(nbytes=vb) for _ in range(4))
+    lib.ostp_set_inverter_tricks(0, mixed, 0.1, 10000)
+    lib.eo_inversion(ip, C.byref(fp), res, 10000, dbe.p, dbo.p, xe.p, xo.p, phe.p, pho.p)
+    ost.check(0, "eo_inversion")
+    lib.ostp_set_inverter_tricks(0, 0, 0.1, 10000)
+    ge, go = xe.get(be), xo.get(bo)
+    # the even system is solved to res; the odd half follows exactly, so the
+    # full residual is bounded by |r_e| / m
+    r = full_matrix_residual(o, u, ph, mass, ge, go, be, bo)
+    assert r <= res / mass, r
+    st, cg, oe, oo = o.eo_inversion(u, ph, mass, be, bo, res)
+    assert st == 1
+    scale = np.sqrt(np.linalg.norm(oe) ** 2 + np.linalg.norm(oo) ** 2)
+    diff = np.sqrt(np.linalg.norm(ge - oe) ** 2 + np.linalg.norm(go - oo) ** 2)
+    assert diff / scale <= 100 * res / mass
