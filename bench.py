@@ -1,20 +1,23 @@
 """Benchmark of the OpenStaPLE fermion hot path on one MI355X.
 
 Workload (32^4, the lattice the headline metric is quoted on): random SU(3)
-links with eps = 0.5 (generate_Conf_cold rule, seed 1), staggered phases with
-antiperiodic time, Gaussian source.  A timed step is one application of the
-FP64 staggered Dslash pair (D_oe then D_eo, the two hops of M = m^2 - D_eo D_oe)
-on HBM-resident fields; `value` is the Dslash bandwidth in GB/s under the
-928 B / output half-site algorithmic model (8 links x 2 stored rows x 3 complex
-x 16 B + 8 phases x 8 B + 48 B in + 48 B out).
+links with eps = 0.5 (generate_Conf_cold rule, seed 1) smeared by 2 stout
+steps with rho = 0.15, staggered phases with antiperiodic time, Gaussian
+source.  A timed step is one application of the FP64 staggered Dslash pair
+(D_oe then D_eo, the two hops of M = m^2 - D_eo D_oe) through the C entry
+points on HBM-resident fields, queued back to back (asynchronous entry mode)
+between two stream synchronisations; `value` is the Dslash bandwidth in GB/s
+under the 928 B / output half-site algorithmic model (8 links x 2 stored rows
+x 3 complex x 16 B + 8 phases x 8 B + 48 B in + 48 B out).
 
-After the timed steps the same fields are solved to ||r||/||b|| < 1e-10 (FP64
-true residual) with (a) the rational multi-shift CG-M over the 19 shifts of the
-order-19 x^(-1/4) approximation rescaled to [m^2, 64], m = 0.01, in FP64, and
-(b) the FP32-inner variant (FP32 CG-M + per-shift FP32 CG with FP64 reliable
-updates), plus (c) the plain FP64 CG at m = 0.01; their wall times are reported
-in "solves".  A bounded sample of the same Dslash is timed on the host with the
-CPU oracle (cpu_baseline).
+After the timed steps, solves to ||r||/||b|| < 1e-10 (FP64 true residual):
+(a) the rational multi-shift CG-M over the 19 shifts of the order-19 x^(-1/4)
+approximation rescaled to [m^2, 64], m = 0.01, in FP64, and (b) its FP32-inner
+variant (FP32 CG-M + per-shift FP32 CG with FP64 reliable updates), on the
+field above; (c) plain FP64 CG and FP32-inner CG with FP64 reliable updates at
+m = 0.005 on a 2-step stout-smeared eps = 0.7 field (seed 2).  Wall times are
+reported in "solves".  A bounded sample of the same Dslash is timed on the
+host with the CPU oracle (cpu_baseline).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solves] [--no-cpu]
 """
@@ -73,17 +76,18 @@ def main():
         lib.ostp_memset(p, 0, n)
         return p
 
-    u = dmalloc(gb)
     ph = dmalloc(pb)
     b = dmalloc(vb)
     h = dmalloc(vb)
     out = dmalloc(vb)
-    lib.ostp_generate_Conf_cold(u, 0.5, 1)
+    lib.ostp_set_stout(0.15, 2, 0.0, 0, 0, 1)
+    u = smeared_field(lib, dmalloc, gb, 0.5, 1)
     lib.calc_u1_phases(ph, ost.BfParam(0, 0, 0, 0, 0, 0), 0.0, 0.0)
     lib.ostp_generate_vec3_soa_gauss(b, 1, 0)
     ost.check(0, "setup")
 
     # ---------------- timed region: K Dslash pairs (Doe then Deo)
+    prev = lib.ostp_set_sync_at_return(0)
     for _ in range(args.warmup):
         lib.acc_Doe(u, h, b, ph)
         lib.acc_Deo(u, out, h, ph)
@@ -94,6 +98,7 @@ def main():
         lib.acc_Deo(u, out, h, ph)
     lib.ostp_synchronize()
     t1 = time.perf_counter()
+    lib.ostp_set_sync_at_return(prev)
     ost.check(0, "dslash loop")
     ms_per_step = 1000.0 * (t1 - t0) / args.steps
     alg_bytes_step = 2 * BYTES_PER_SITE_F64 * sizeh
@@ -148,7 +153,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{'x'.join(map(str, shape))} staggered Dslash pair (Doe+Deo) FP64 on HBM-resident "
-                        "fields; random SU(3) eps=0.5 seed 1; then 19-shift rational CG-M and plain CG to 1e-10",
+                        "fields; random SU(3) eps=0.5 seed 1, 2 stout steps rho=0.15; then 19-shift rational "
+                        "CG-M (m=0.01) and CG at m=0.005 (eps=0.7 smeared) to 1e-10, FP64 vs FP32-inner",
             "lattice": list(shape),
             "sizeh": sizeh,
             "bytes_per_half_site": BYTES_PER_SITE_F64,
@@ -179,6 +185,18 @@ def main():
         result["cpu_baseline"] = cpu_baseline(shape, args.cpu_seconds)
 
     print(json.dumps(result))
+
+
+def smeared_field(lib, dmalloc, gb, eps, seed):
+    """generate_Conf_cold rule with `eps`, then the configured stout steps
+    (stout_wrapper); returns the device pointer of the last level."""
+    raw = dmalloc(gb)
+    lib.ostp_generate_Conf_cold(raw, eps, seed)
+    steps = lib.ostp_stout_steps()
+    arr = dmalloc(steps * gb)
+    lib.stout_wrapper(raw, arr, 0)
+    lib.ostp_free(raw)
+    return arr + (steps - 1) * gb  # the smeared levels stay allocated for the whole run
 
 
 def run_solves(ost, lib, L, u, ph, b, uf, phf):
@@ -246,28 +264,37 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf):
                                "max_shift_true_residual": worst}
     res["multishift_speedup_mixed_vs_fp64"] = round(res["multishift_fp64"]["seconds"] / (t1 - t0), 3)
 
-    # (c) plain FP64 CG and mixed CG at the same mass
+    # (c) plain FP64 CG and FP32-inner CG at m = 0.005 on a 2-step stout-smeared
+    # eps = 0.7 field (seed 2), the same source
+    gb = 8 * L.su3_bytes
+    u4 = smeared_field(lib, dm, gb, 0.7, 2)
+    u4f = dm(8 * L.su3f_bytes)
+    lib.ostp_convert_double_to_float_gauge(u4, u4f)
+    m4 = 0.005
+    fp4 = ost.ferm_param(m4, ph, phf)
+    ip.u, ip.u_f = u4, u4f
     tr = C.c_double(0)
     lib.ostp_memset(x, 0, vb)
     lib.ostp_synchronize()
     t0 = time.perf_counter()
-    st = lib.ostp_cg_solve(u, C.byref(fp), x, b, 1e-10, *d64, 20000, 0.0, 10000, C.byref(cg), C.byref(tr))
+    st = lib.ostp_cg_solve(u4, C.byref(fp4), x, b, 1e-10, *d64, 40000, 0.0, 10000, C.byref(cg), C.byref(tr))
     lib.ostp_synchronize()
     t1 = time.perf_counter()
     res["cg_fp64"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "true_residual": tr.value,
-                      "status": st, "mass": mass}
+                      "status": st, "mass": m4, "field": "eps=0.7 seed 2, 2 stout steps rho=0.15"}
     ru = C.c_int(0)
     lib.ostp_memset(x, 0, vb)
     lib.ostp_synchronize()
     t0 = time.perf_counter()
-    st = lib.ostp_mixed_solve(ip, C.byref(fp), x, b, 1e-10, 20000, 0.0, 0.1, C.byref(cg), C.byref(ru), C.byref(tr))
+    st = lib.ostp_mixed_solve(ip, C.byref(fp4), x, b, 1e-10, 40000, 0.0, 0.1, C.byref(cg), C.byref(ru),
+                              C.byref(tr))
     lib.ostp_synchronize()
     t1 = time.perf_counter()
     res["cg_mixed"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "reliable_updates": ru.value,
                        "true_residual": tr.value, "status": st}
     res["cg_speedup_mixed_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
     ost.check(0, "solves")
-    for p in [*d64, *d32, out, sf, sf32, x]:
+    for p in [*d64, *d32, out, sf, sf32, x, u4f]:
         lib.ostp_free(p)
     return res
 

@@ -88,6 +88,13 @@ int   ostp_memcpy_htod(void *dev, const void *host, size_t bytes);
 int   ostp_memcpy_dtoh(void *host, const void *dev, size_t bytes);
 int   ostp_memset(void *dev, int value, size_t bytes);
 int   ostp_synchronize(void);
+/* Entry points return synchronised by default (the reference's non-async
+ * `acc kernels`).  With 0, Dslash, M^dag M and the vector BLAS entry points
+ * only queue their kernels on the library stream (as the reference's
+ * async(n) variants do) and ostp_synchronize() waits; solvers, reductions
+ * and everything that returns a value still synchronise.  Returns the
+ * previous setting. */
+int   ostp_set_sync_at_return(int on);
 
 /* errors: by default a bad pointer or a HIP failure prints a message and aborts
  * (as the OpenACC runtime does for a missing `present` datum); with 0 the call

@@ -157,6 +157,7 @@ _SIGS = {
     "ostp_set_abort_on_error": (None, [_I]),
     "ostp_last_error": (C.c_char_p, []),
     "ostp_clear_error": (None, []),
+    "ostp_set_sync_at_return": (_I, [_I]),
     "ostp_set_inverter_tricks": (None, [_I, _I, _D, _I]),
     "ostp_compute_nnp_and_nnm": (_I, [_P, _P]),
     "calc_u1_phases": (None, [_P, BfParam, _D, _D]),

@@ -37,6 +37,7 @@ struct Context {
     hipStream_t stream = nullptr;
     int device = 0;
     int abort_on_error = 1;
+    int sync_at_return = 1;   // entry points without a result wait for their kernels
     // device scratch: reduction partials + solver state
     double* partials = nullptr;     // 2 * max_blocks
     SolverState* state = nullptr;   // device
@@ -66,6 +67,9 @@ inline int blocks_for(long n, int nt = kBlock) { return (int)((n + nt - 1) / nt)
 
 // Gauge action selected by ostp_set_stout (true: tree-level Symanzik).
 bool gauge_action_tlsm();
+
+// End of a void entry point: wait for the stream unless asynchronous mode is on.
+bool entry_done(const char* what);
 
 // Read the device solver state into the pinned mirror (synchronises the stream).
 bool pull_state();

@@ -114,6 +114,11 @@ bool pull_state() {
     return hip_ok(hipStreamSynchronize(g_ctx.stream), "stream synchronize");
 }
 
+bool entry_done(const char* what) {
+    if (!g_ctx.sync_at_return) return true;
+    return hip_ok(hipStreamSynchronize(g_ctx.stream), what);
+}
+
 bool push_state() {
     if (!hip_ok(hipMemcpyAsync(g_ctx.state, g_ctx.state_host, sizeof(SolverState), hipMemcpyHostToDevice,
                                g_ctx.stream),
@@ -302,6 +307,12 @@ int ostp_synchronize(void) {
 }
 
 void ostp_set_abort_on_error(int a) { g_ctx.abort_on_error = a; }
+
+int ostp_set_sync_at_return(int on) {
+    const int prev = g_ctx.sync_at_return;
+    g_ctx.sync_at_return = on ? 1 : 0;
+    return prev;
+}
 const char* ostp_last_error(void) { return g_last_error.c_str(); }
 void ostp_clear_error(void) { g_last_error.clear(); }
 

@@ -108,19 +108,26 @@ def test_unitarize():
 
 
 def test_stout_full_size_properties():
-    """32^4 smearing of the benchmark field: links stay SU(3) and the mean
-    plaquette rises (smearing is a smoothing)."""
+    """32^4 smearing of the benchmark field: level 1 equals the oracle on every
+    link, and the level-2 links stay SU(3) (rows 0,1 orthonormal)."""
     shape = (32, 32, 32, 32)
     L = init_lattice(shape)
     lat = ost.Lattice(*shape)
-    u = Dev(nbytes=8 * lat.su3_bytes)
+    gb = 8 * lat.su3_bytes
+    u = Dev(nbytes=gb)
     lib.ostp_generate_Conf_cold(u.p, 0.5, 1)
-    arr = Dev(nbytes=2 * 8 * lat.su3_bytes)
+    arr = Dev(nbytes=2 * gb)
     lib.stout_wrapper(u.p, arr.p, 0)
     ost.check(0, "stout 32^4")
     o = Oracle(*shape)
-    host = arr.get(np.zeros(2 * 8 * lat.su3_bytes, dtype=np.uint8))[8 * lat.su3_bytes:]
-    m = np.transpose(o.links(host)[:, :2], (0, 3, 1, 2))[:, ::97]  # sample
-    rows = m.reshape(-1, 2, 3)
+    host = arr.get(np.zeros(2 * gb, dtype=np.uint8))
+    lev1 = o.links(host[:gb].copy())[:, :2]
+    lev2 = o.links(host[gb:].copy())[:, :2]
+    ref1 = o.links(o.stout(o.gen_cold(0.5, 1), RHO))[:, :2]
+    d1 = np.abs(lev1 - ref1).max(axis=(1, 2))
+    assert d1.max() < 1e-13, (d1.max(), np.unravel_index(np.argmax(d1), d1.shape))
+    rows = np.transpose(lev2, (0, 3, 1, 2)).reshape(-1, 2, 3)
     gram = rows @ np.conj(np.transpose(rows, (0, 2, 1)))
-    assert np.max(np.abs(gram - np.eye(2))) < 1e-12
+    dev = np.abs(gram - np.eye(2)).max(axis=(1, 2))
+    worst = int(np.argmax(dev))
+    assert dev[worst] < 1e-12, (dev[worst], divmod(worst, o.sizeh), rows[worst])

@@ -109,3 +109,32 @@ def test_link_kernel_site_decoding(shape):
                     j = shifted(d, mu, 1, nu, -1, shape)
                     assert 0 <= j < sizeh
                     assert j == nnm[nnp[i, mu, parity], nu, 1 - parity]
+
+
+def dslash_site_of_thread(G, nd3, sizeh, tmajor, block=256):
+    """Restatement of the workgroup -> site mapping of k_dslash (dslash.hip)."""
+    i = np.arange(G)
+    q, r = G >> 3, G & 7
+    x, k = i & 7, i >> 3
+    blk = x * q + np.where(x < r, x, r) + k
+    if tmajor:
+        s3 = sizeh // nd3
+        tile, tt = blk // nd3, blk % nd3
+        base = tt * s3 + tile * block
+    else:
+        base = blk * block
+    return (base[:, None] + np.arange(block)[None, :]).ravel()
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32, 32), (24, 24, 24, 8), (32, 32, 32, 8), (48, 48, 48, 12),
+                                   (16, 16, 16, 16), (4, 4, 4, 4), (4, 6, 8, 10)])
+def test_dslash_workgroup_mapping_is_a_bijection(shape):
+    sizeh = np.prod(shape) // 2
+    s3 = sizeh // shape[3]
+    tmajor = s3 % 256 == 0  # the launcher's condition
+    G = (sizeh + 255) // 256
+    idx = dslash_site_of_thread(G, shape[3], sizeh, tmajor)
+    valid = idx[idx < sizeh]
+    assert len(np.unique(valid)) == sizeh and valid.min() == 0
+    if tmajor:
+        assert idx.max() == sizeh - 1  # no thread beyond the lattice
