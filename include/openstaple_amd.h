@@ -284,6 +284,39 @@ int ostp_pt_metropolis(double delta_s_swap, double u);
  * the label sits out this stage. */
 int ostp_pt_twin_label(int lab, int swap_order, int nreplicas);
 
+/* ============================================= batched solves (new) ===== */
+/* No reference counterpart: the reference solves one right-hand side per
+ * call.  One system of a batch: the even-odd system (m^2 - Deo Doe) x = b on
+ * its own lattice (extents nd, K plane iff par_temp), solved by the
+ * inverter_mixed_precision algorithm (OpenAcc/inverter_mixedp.c:41-181:
+ * FP32 CG, FP64 reliable updates, SAFETY 0.9, final FP64 true residual).
+ * Pointers as everywhere else (registered host or device addresses). */
+typedef struct ostp_batch_system_t {
+    int nd[4];
+    int par_temp;
+    const su3_soa *u;          /* FP64 links: reliable updates, final residual */
+    const su3_soa_f *u_f;      /* FP32 links: inner iterations */
+    const double_soa *phases;
+    const float_soa *phases_f;
+    const vec3_soa *in;        /* b */
+    vec3_soa *out;             /* x: initial guess on entry, solution on return */
+    double mass;
+} ostp_batch_system;
+/* Solve all systems together: every kernel is batched over systems, each
+ * system keeps its scalars on the device, and iters_per_graph iterations are
+ * captured once into a HIP graph that is replayed until all systems are
+ * done.  The Dslash tile is autotuned per batch signature on first use.
+ * cg_return[k], ru_return[k] (reliable updates) and true_res[k] (FP64
+ * ||b - M x|| / ||b||) may be NULL.  Returns the number of systems whose true
+ * residual is <= res (the ostp_init lattice is not used by this call, but
+ * ostp_init must have been called once). */
+int ostp_batched_mixed_solve(int nsys, const ostp_batch_system *systems, double res, int max_cg,
+                             double mixed_delta, int iters_per_graph, int *cg_return, int *ru_return,
+                             double *true_res);
+/* Text of the autotuner's measurements so far (candidates and choice per
+ * batch signature); returns its length, copies up to len-1 bytes into buf. */
+int ostp_autotune_report(char *buf, int len);
+
 /* ======================================================== generators ===== */
 /* Reference field rules (OpenAcc/random_assignement.c:67-92,106-142,188-250)
  * drawn from a counter-based Philox4x32-10 stream keyed by `seed` instead of

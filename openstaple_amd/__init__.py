@@ -92,6 +92,20 @@ class BfParam(C.Structure):  # OpenAcc/backfield_parameters.h:5-9
     _fields_ = [(n, C.c_double) for n in ("ex", "ey", "ez", "bx", "by", "bz")]
 
 
+class BatchSystem(C.Structure):  # ostp_batch_system
+    _fields_ = [
+        ("nd", C.c_int * 4),
+        ("par_temp", C.c_int),
+        ("u", C.c_void_p),
+        ("u_f", C.c_void_p),
+        ("phases", C.c_void_p),
+        ("phases_f", C.c_void_p),
+        ("in_", C.c_void_p),
+        ("out", C.c_void_p),
+        ("mass", C.c_double),
+    ]
+
+
 class RepInfo(C.Structure):  # Include/rep_info.h:4-11
     _fields_ = [
         ("replicas_total_number", C.c_int),
@@ -129,6 +143,7 @@ PT_UNIFORM_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
 assert C.sizeof(RationalApprox) == 456
 assert FermParam.phases.offset == 72 and FermParam.phases_f.offset == 104
 assert C.sizeof(InverterPackage) == 112
+assert C.sizeof(BatchSystem) == 80
 assert C.sizeof(RepInfo) == 40 and C.sizeof(DefectInfo) == 400 and C.sizeof(DefectInfoWilson) == 144
 
 # ------------------------------------------------------------------ signatures
@@ -223,6 +238,9 @@ _SIGS = {
     "ostp_pt_index_of_label0": (_I, [C.POINTER(RepInfo)]),
     "ostp_pt_metropolis": (_I, [_D, _D]),
     "ostp_pt_twin_label": (_I, [_I, _I, _I]),
+    "ostp_batched_mixed_solve": (_I, [_I, C.POINTER(BatchSystem), _D, _I, _D, _I, C.POINTER(_I), C.POINTER(_I),
+                                      C.POINTER(_D)]),
+    "ostp_autotune_report": (_I, [C.c_char_p, _I]),
     "ostp_time_dslash": (_D, [_P, _P, _P, _P, _I, _I, _I]),
     "ostp_time_copy": (_D, [_P, _P, C.c_size_t, _I]),
 }

@@ -1,0 +1,578 @@
+// Batched even-odd solves: many right-hand sides and lattices in one HIP graph.
+//
+// Each system is the mixed-precision CG of inverter_mixed_precision
+// (OpenAcc/inverter_mixedp.c:41-181): FP32 CG on m^2 - D_eo D_oe with FP64
+// reliable updates of the true residual when delta < mixed_delta * max delta
+// (SAFETY 0.9, exit on sqrt(lambda/|b|^2) <= 0.9 res, final FP64 true residual).
+// The reference solves one system per call; here S systems -- noise sources
+// on one gauge field, parallel-tempering replicas on their own fields, and
+// lattices of different extents -- advance together:
+//
+//  * every kernel is batched over systems (blockIdx.y = system); a system
+//    whose lattice needs fewer tiles than the grid has exits those tiles;
+//  * each system carries its own device-resident SolverState; kernels and
+//    scalar epilogues are predicated on that state (running / reliable
+//    update due), never on a host-side iteration count, so a captured
+//    sequence of iterations stays valid however often it is replayed;
+//  * `iters_per_graph` iterations are captured once into a HIP graph, which
+//    is replayed until every system has left the running state; the host
+//    reads the S states once per replay;
+//  * the Dslash tile (workgroup size, t-fastest or contiguous tile order) is
+//    chosen per batch signature by timing the candidates on the batch's own
+//    data the first time that signature is seen (autotune table).
+#include "ostp_dslash.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+static_assert(sizeof(ostp_batch_system) == 80 && offsetof(ostp_batch_system, mass) == 72,
+              "ostp_batch_system layout");
+
+namespace ostp {
+
+struct BSys {
+    const double2* u64[8];
+    const double* ph64[8];
+    const float2* u32[8];
+    const float* ph32[8];
+    const double2* b;
+    double2* x;
+    double2 *r64, *h64, *s64;
+    float2 *rf, *hf, *sf, *pf, *outf;
+    SiteGeom g;
+    double m2, m2f;
+    SolverState* st;
+    double* partials;
+};
+
+enum BRole : int {
+    kRoleF32Doe = 0,   // hf = D_oe pf                          (FP32)
+    kRoleF32DeoDot,    // sf = m2f pf - D_eo hf, partial <pf,sf> (FP32)
+    kRoleF64Doe,       // h64 = D_oe x                          (FP64)
+    kRoleF64Deo,       // s64 = m2 x - D_eo h64                 (FP64)
+};
+
+__device__ __forceinline__ SolverCtl bctl(const BSys& S, int pred) {
+    SolverCtl c;
+    c.st = S.st;
+    c.partials = S.partials;
+    c.pred = pred;
+    return c;
+}
+
+// logical tile of this workgroup for a system with `ntiles` tiles; -1 if idle
+__device__ __forceinline__ int tile_of(int ntiles) {
+    const int blk = xcd_block_id();
+    return blk < ntiles ? blk : -1;
+}
+
+template <int ROLE, int BLK>
+__global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, int pred, int tmajor) {
+    using R = typename std::conditional<(ROLE <= kRoleF32DeoDot), float, double>::type;
+    using C = typename Cplx<R>::T;
+    const BSys& S = sys[blockIdx.y];
+    if (!pred_pass(bctl(S, pred))) return;
+    const SiteGeom& g = S.g;
+    const int ntiles = (g.sizeh + BLK - 1) / BLK;
+    const int tile = tile_of(ntiles);
+    if (tile < 0) return;
+    const int s3 = g.sizeh / g.nd3;
+    int idx;
+    if (tmajor && s3 % BLK == 0) {
+        const int q = tile / g.nd3, tt = tile - q * g.nd3;
+        idx = tt * s3 + q * BLK + (int)threadIdx.x;
+    } else {
+        idx = tile * BLK + (int)threadIdx.x;
+    }
+    constexpr int EO = (ROLE == kRoleF32Doe || ROLE == kRoleF64Doe) ? 1 : 0;
+    const C* const* u;
+    const R* const* ph;
+    const C* in;
+    C* out;
+    const C* x = nullptr;
+    double m2 = 0.0;
+    if constexpr (ROLE == kRoleF32Doe) {
+        u = S.u32; ph = S.ph32; in = S.pf; out = S.hf;
+    } else if constexpr (ROLE == kRoleF32DeoDot) {
+        u = S.u32; ph = S.ph32; in = S.hf; out = S.sf; x = S.pf; m2 = S.m2f;
+    } else if constexpr (ROLE == kRoleF64Doe) {
+        u = S.u64; ph = S.ph64; in = S.x; out = S.h64;
+    } else {
+        u = S.u64; ph = S.ph64; in = S.h64; out = S.s64; x = S.x; m2 = S.m2;
+    }
+    double dot = 0.0;
+    if (idx < g.sizeh) {
+        C o0, o1, o2;
+        dslash_site<R, EO>(u, ph, in, g, idx, o0, o1, o2);
+        if constexpr (EO == 1) {
+            out[idx] = o0;
+            out[g.sizeh + idx] = o1;
+            out[2 * g.sizeh + idx] = o2;
+        } else {
+            const C x0 = x[idx], x1 = x[g.sizeh + idx], x2 = x[2 * g.sizeh + idx];
+            const C s0 = mk<C>((double)x0.x * m2 - (double)o0.x, (double)x0.y * m2 - (double)o0.y);
+            const C s1 = mk<C>((double)x1.x * m2 - (double)o1.x, (double)x1.y * m2 - (double)o1.y);
+            const C s2 = mk<C>((double)x2.x * m2 - (double)o2.x, (double)x2.y * m2 - (double)o2.y);
+            out[idx] = s0;
+            out[g.sizeh + idx] = s1;
+            out[2 * g.sizeh + idx] = s2;
+            if constexpr (ROLE == kRoleF32DeoDot) dot = re_dot(x0, s0) + re_dot(x1, s1) + re_dot(x2, s2);
+        }
+    }
+    if constexpr (ROLE == kRoleF32DeoDot) block_partial_at<BLK, 1>(dot, 0.0, S.partials, tile);
+}
+
+// Elementwise kernels: one thread per half-site, 3 colours each, kBlock threads.
+#define BATCH_SITE_PROLOGUE(pred)                                     \
+    const BSys& S = sys[blockIdx.y];                                  \
+    if (!pred_pass(bctl(S, pred))) return;                            \
+    const int ntiles = (S.g.sizeh + kBlock - 1) / kBlock;             \
+    if ((int)blockIdx.x >= ntiles) return;                            \
+    const int i = blockIdx.x * kBlock + threadIdx.x;                  \
+    const long sh = S.g.sizeh;
+
+// r = b - s (FP64); r_f = (float) r; p_f = r_f; out_f = 0; partial |r_f|^2  (inverter_mixedp.c:74-88)
+__global__ __launch_bounds__(kBlock) void kb_mx_init(const BSys* __restrict__ sys) {
+    BATCH_SITE_PROLOGUE(kAlways)
+    double v = 0.0;
+    if (i < sh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = c * sh + i;
+            const double2 bb = S.b[k], ss = S.s64[k];
+            const double2 rr = make_double2(bb.x - ss.x, bb.y - ss.y);
+            S.r64[k] = rr;
+            const float2 f = make_float2((float)rr.x, (float)rr.y);
+            S.rf[k] = f;
+            S.pf[k] = f;
+            S.outf[k] = make_float2(0.f, 0.f);
+            v += (double)f.x * f.x + (double)f.y * f.y;
+        }
+    }
+    block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
+}
+
+// out_f += omega p_f; then r_f -= omega s_f with partial |r_f|^2, or (reliable
+// update due) x += out_f, out_f = 0   (inverter_mixedp.c:107-134)
+__global__ __launch_bounds__(kBlock) void kb_mx_xr(const BSys* __restrict__ sys) {
+    BATCH_SITE_PROLOGUE(kIfRunning)
+    const double om = S.st->omega;
+    const int ru = S.st->ru;
+    double v = 0.0;
+    if (i < sh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = c * sh + i;
+            const float2 pp = S.pf[k], oo = S.outf[k];
+            const float2 on =
+                make_float2((float)((double)pp.x * om + (double)oo.x), (float)((double)pp.y * om + (double)oo.y));
+            if (!ru) {
+                S.outf[k] = on;
+                const float2 ss = S.sf[k], rr = S.rf[k];
+                const float2 rn = make_float2((float)((double)ss.x * (-om) + (double)rr.x),
+                                              (float)((double)ss.y * (-om) + (double)rr.y));
+                S.rf[k] = rn;
+                v += (double)rn.x * rn.x + (double)rn.y * rn.y;
+            } else {
+                const double2 xx = S.x[k];
+                S.x[k] = make_double2(xx.x + (double)on.x, xx.y + (double)on.y);
+                S.outf[k] = make_float2(0.f, 0.f);
+            }
+        }
+    }
+    if (!ru) block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
+}
+
+// reliable update tail: r = b - s (FP64), r_f = (float) r, partial |r_f|^2  (inverter_mixedp.c:121-124,137)
+__global__ __launch_bounds__(kBlock) void kb_mx_resid(const BSys* __restrict__ sys) {
+    BATCH_SITE_PROLOGUE(kIfRunningRU)
+    double v = 0.0;
+    if (i < sh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = c * sh + i;
+            const double2 bb = S.b[k], ss = S.s64[k];
+            const double2 rr = make_double2(bb.x - ss.x, bb.y - ss.y);
+            S.r64[k] = rr;
+            const float2 f = make_float2((float)rr.x, (float)rr.y);
+            S.rf[k] = f;
+            v += (double)f.x * f.x + (double)f.y * f.y;
+        }
+    }
+    block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
+}
+
+// p_f = p_f * gammag + r_f   (inverter_mixedp.c:150)
+__global__ __launch_bounds__(kBlock) void kb_cg_p(const BSys* __restrict__ sys) {
+    BATCH_SITE_PROLOGUE(kIfRunning)
+    if (i >= sh) return;
+    const double g = S.st->gammag;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const long k = c * sh + i;
+        const float2 pp = S.pf[k], rr = S.rf[k];
+        S.pf[k] = make_float2((float)((double)pp.x * g + (double)rr.x), (float)((double)pp.y * g + (double)rr.y));
+    }
+}
+
+// x += (double) out_f   (inverter_mixedp.c:24-34,165)
+__global__ __launch_bounds__(kBlock) void kb_add_f_to_d(const BSys* __restrict__ sys) {
+    BATCH_SITE_PROLOGUE(kAlways)
+    if (i >= sh) return;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const long k = c * sh + i;
+        const double2 xx = S.x[k];
+        const float2 ff = S.outf[k];
+        S.x[k] = make_double2(xx.x + (double)ff.x, xx.y + (double)ff.y);
+    }
+}
+
+// partial of |b - s|^2 (true residual, s = M x) or of |b|^2 (with_s = 0)
+__global__ __launch_bounds__(kBlock) void kb_resid_norm(const BSys* __restrict__ sys, int with_s) {
+    BATCH_SITE_PROLOGUE(kAlways)
+    double v = 0.0;
+    if (i < sh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = c * sh + i;
+            double2 rr = S.b[k];
+            if (with_s) {
+                const double2 ss = S.s64[k];
+                rr = make_double2(rr.x - ss.x, rr.y - ss.y);
+            }
+            v += rr.x * rr.x + rr.y * rr.y;
+        }
+    }
+    block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
+}
+
+// one block per system: sum its partials in tile order, apply finalizer F
+template <int F>
+__global__ __launch_bounds__(kBlock) void kb_finalize(const BSys* __restrict__ sys, int pred, int tile_size) {
+    const BSys& S = sys[blockIdx.x];
+    const SolverCtl c = bctl(S, pred);
+    if (!pred_pass(c)) return;
+    const int nparts = (S.g.sizeh + tile_size - 1) / tile_size;
+    __shared__ double lds[kBlock / 64];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) a += S.partials[2 * i];
+    const double t = block_sum<kBlock>(a, lds);
+    if (threadIdx.x == 0) finalize<F>(S.st, t, 0.0);
+}
+
+// ------------------------------------------------------------------ host side
+struct Tile {
+    int blk = 256;
+    int tmajor = 1;
+    float us = 0.f;
+};
+
+static std::map<std::string, Tile> g_tuned;  // batch signature -> chosen Dslash tile
+static std::string g_last_report;
+
+struct Batch {
+    int n = 0;
+    int max_tiles = 0;   // tiles of the largest system at kBlock threads
+    int max_tiles64 = 0; // same at 64 threads (partials capacity)
+    BSys* dsys = nullptr;
+    SolverState* st = nullptr;
+    SolverState* st_host = nullptr;
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+    size_t st_cap = 0;
+};
+static Batch g_batch;
+
+static hipStream_t S() { return ctx().stream; }
+
+template <int ROLE>
+static void launch_role(const Tile& t, int nsys, int max_sizeh, int pred) {
+    const int tiles = (max_sizeh + t.blk - 1) / t.blk;
+    if (t.blk == 128)
+        hipLaunchKernelGGL((kb_dslash<ROLE, 128>), dim3(tiles, nsys), dim3(128), 0, S(), g_batch.dsys, pred, t.tmajor);
+    else
+        hipLaunchKernelGGL((kb_dslash<ROLE, 256>), dim3(tiles, nsys), dim3(256), 0, S(), g_batch.dsys, pred, t.tmajor);
+}
+
+template <int F>
+static void fin(int nsys, int pred, int tile_size) {
+    hipLaunchKernelGGL((kb_finalize<F>), dim3(nsys), dim3(kBlock), 0, S(), g_batch.dsys, pred, tile_size);
+}
+
+static bool pull_states(int n) {
+    if (!hip_ok(hipMemcpyAsync(g_batch.st_host, g_batch.st, sizeof(SolverState) * n, hipMemcpyDeviceToHost, S()),
+                "state download"))
+        return false;
+    return hip_ok(hipStreamSynchronize(S()), "stream synchronize");
+}
+static bool push_states(int n) {
+    if (!hip_ok(hipMemcpyAsync(g_batch.st, g_batch.st_host, sizeof(SolverState) * n, hipMemcpyHostToDevice, S()),
+                "state upload"))
+        return false;
+    return hip_ok(hipStreamSynchronize(S()), "stream synchronize");
+}
+
+static std::string signature(const std::vector<BSys>& v) {
+    std::string s;
+    for (const auto& b : v) {
+        char buf[64];
+        snprintf(buf, sizeof buf, "%dx%dx%dx%d;", b.g.nd0, b.g.nd1, b.g.nd2, b.g.nd3);
+        s += buf;
+    }
+    return s;
+}
+
+// Time each candidate tile on the batch's FP32 D_oe (the kernel of every
+// inner iteration), three launches each after one warm-up launch.
+static Tile autotune(const std::string& sig, int nsys, int max_sizeh) {
+    auto it = g_tuned.find(sig);
+    if (it != g_tuned.end()) return it->second;
+    const Tile cands[4] = {{256, 1, 0.f}, {256, 0, 0.f}, {128, 1, 0.f}, {128, 0, 0.f}};
+    hipEvent_t e0, e1;
+    Tile best;
+    best.us = 1e30f;
+    std::string rep = "batch " + sig + ":";
+    if (!hip_ok(hipEventCreate(&e0), "event") || !hip_ok(hipEventCreate(&e1), "event")) return Tile{};
+    for (const Tile& c0 : cands) {
+        Tile c = c0;
+        launch_role<kRoleF32Doe>(c, nsys, max_sizeh, kAlways);
+        (void)hipEventRecord(e0, S());
+        for (int r = 0; r < 3; r++) launch_role<kRoleF32Doe>(c, nsys, max_sizeh, kAlways);
+        (void)hipEventRecord(e1, S());
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        c.us = 1000.f * ms / 3;
+        char buf[96];
+        snprintf(buf, sizeof buf, " [%d thr, %s] %.1f us", c.blk, c.tmajor ? "t-fastest" : "contiguous", c.us);
+        rep += buf;
+        if (c.us < best.us) best = c;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    char buf[96];
+    snprintf(buf, sizeof buf, " -> %d thr %s\n", best.blk, best.tmajor ? "t-fastest" : "contiguous");
+    rep += buf;
+    g_last_report += rep;
+    if (verbosity_lv > 1) printf("%s", rep.c_str());
+    g_tuned[sig] = best;
+    return best;
+}
+
+}  // namespace ostp
+
+using namespace ostp;
+
+extern "C" {
+
+int ostp_batched_mixed_solve(int nsys, const ostp_batch_system* sys, double res, int max_cg, double mixed_delta,
+                             int iters_per_graph, int* cg_return, int* ru_return, double* true_res) {
+    const char* nm = "ostp_batched_mixed_solve";
+    if (!require_init(nm)) return 0;
+    if (nsys <= 0 || !sys || iters_per_graph <= 0) {
+        fail("%s: invalid arguments", nm);
+        return 0;
+    }
+    // ---- descriptors
+    std::vector<BSys> hs(nsys);
+    std::vector<long> sizeh(nsys);
+    size_t arena = 0;
+    int max_sizeh = 0, max_t64 = 0;
+    for (int k = 0; k < nsys; k++) {
+        const ostp_batch_system& q = sys[k];
+        const int* nd = q.nd;
+        if (nd[0] <= 0 || nd[1] <= 0 || nd[2] <= 0 || nd[3] <= 0 || (nd[0] & 1)) {
+            fail("%s: system %d has invalid extents", nm, k);
+            return 0;
+        }
+        sizeh[k] = (long)nd[0] * nd[1] * nd[2] * nd[3] / 2;
+        if (sizeh[k] > (1L << 29)) {
+            fail("%s: system %d too large", nm, k);
+            return 0;
+        }
+        max_sizeh = std::max(max_sizeh, (int)sizeh[k]);
+        max_t64 = std::max(max_t64, (int)((sizeh[k] + 63) / 64));
+        arena += (size_t)(3 * 48 + 5 * 24) * sizeh[k] + 256 * 8;  // vectors (+ alignment slack)
+    }
+    const size_t part_per = (size_t)2 * max_t64;
+    arena += (sizeof(double) * part_per + 256) * nsys;
+    Batch& B = g_batch;
+    if (arena > B.arena_bytes) {
+        if (B.arena) (void)hipFree(B.arena);
+        B.arena = nullptr;
+        B.arena_bytes = 0;
+        if (!hip_ok(hipMalloc(&B.arena, arena), "batch arena")) return 0;
+        B.arena_bytes = arena;
+    }
+    if ((size_t)nsys > B.st_cap) {
+        if (B.st) (void)hipFree(B.st);
+        if (B.st_host) (void)hipHostFree(B.st_host);
+        if (B.dsys) (void)hipFree(B.dsys);
+        B.st = nullptr;
+        B.st_host = nullptr;
+        B.dsys = nullptr;
+        B.st_cap = 0;
+        if (!hip_ok(hipMalloc(&B.st, sizeof(SolverState) * nsys), "batch states") ||
+            !hip_ok(hipHostMalloc(&B.st_host, sizeof(SolverState) * nsys, hipHostMallocDefault), "batch states") ||
+            !hip_ok(hipMalloc(&B.dsys, sizeof(BSys) * nsys), "batch descriptors"))
+            return 0;
+        B.st_cap = nsys;
+    }
+    char* cur = (char*)B.arena;
+    auto take = [&](size_t bytes) {
+        void* p = cur;
+        cur += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    for (int k = 0; k < nsys; k++) {
+        const ostp_batch_system& q = sys[k];
+        BSys& b = hs[k];
+        const long sh = sizeh[k];
+        const size_t su3 = (size_t)(144 + (q.par_temp ? 8 : 0)) * sh, su3f = (size_t)(72 + (q.par_temp ? 4 : 0)) * sh;
+        const char* u = (const char*)resolve(q.u, 8 * su3, nm);
+        const char* uf = (const char*)resolve(q.u_f, 8 * su3f, nm);
+        const char* ph = (const char*)resolve(q.phases, 8 * 8 * sh, nm);
+        const char* phf = (const char*)resolve(q.phases_f, 8 * 4 * sh, nm);
+        b.b = (const double2*)resolve(q.in, 48 * sh, nm);
+        b.x = (double2*)resolve(q.out, 48 * sh, nm);
+        if (!u || !uf || !ph || !phf || !b.b || !b.x) return 0;
+        for (int l = 0; l < 8; l++) {
+            b.u64[l] = (const double2*)(u + l * su3);
+            b.u32[l] = (const float2*)(uf + l * su3f);
+            b.ph64[l] = (const double*)(ph + (size_t)l * 8 * sh);
+            b.ph32[l] = (const float*)(phf + (size_t)l * 4 * sh);
+        }
+        b.r64 = (double2*)take(48 * sh);
+        b.h64 = (double2*)take(48 * sh);
+        b.s64 = (double2*)take(48 * sh);
+        b.rf = (float2*)take(24 * sh);
+        b.hf = (float2*)take(24 * sh);
+        b.sf = (float2*)take(24 * sh);
+        b.pf = (float2*)take(24 * sh);
+        b.outf = (float2*)take(24 * sh);
+        b.g.nd0 = q.nd[0];
+        b.g.nd0h = q.nd[0] / 2;
+        b.g.nd1 = q.nd[1];
+        b.g.nd2 = q.nd[2];
+        b.g.nd3 = q.nd[3];
+        b.g.sizeh = (int)sh;
+        b.m2 = q.mass * q.mass;
+        b.m2f = q.mass * q.mass;  // inverter_mixedp.c passes shift 0
+        b.st = B.st + k;
+        b.partials = nullptr;
+    }
+    for (int k = 0; k < nsys; k++) hs[k].partials = (double*)take(sizeof(double) * part_per);
+    if (!hip_ok(hipMemcpyAsync(B.dsys, hs.data(), sizeof(BSys) * nsys, hipMemcpyHostToDevice, S()), "descriptors"))
+        return 0;
+    const int nb_site = (max_sizeh + kBlock - 1) / kBlock;
+    const dim3 gsite(nb_site, nsys);
+
+    // ---- source norms, states
+    hipLaunchKernelGGL(kb_resid_norm, gsite, dim3(kBlock), 0, S(), B.dsys, 0);
+    fin<kFinStore>(nsys, kAlways, kBlock);
+    if (!hip_ok(hipGetLastError(), nm) || !pull_states(nsys)) return 0;
+    for (int k = 0; k < nsys; k++) {
+        SolverState& s = B.st_host[k];
+        const double sn = s.sum;
+        std::memset(&s, 0, sizeof s);
+        s.res = res;
+        s.safety = 0.9;
+        s.max_cg = max_cg;
+        s.mixed_delta = mixed_delta;
+        s.source_norm = sn;
+        s.status = kRunning;
+    }
+    if (!push_states(nsys)) return 0;
+    const std::string sig = signature(hs);
+    const Tile t = autotune(sig, nsys, max_sizeh);
+
+    // r = b - M x (FP64), r_f = p_f = (float) r, out_f = 0, delta = |r_f|^2
+    launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kAlways);
+    launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kAlways);
+    hipLaunchKernelGGL(kb_mx_init, gsite, dim3(kBlock), 0, S(), B.dsys);
+    fin<kFinCgRestart>(nsys, kAlways, kBlock);
+    if (!hip_ok(hipGetLastError(), nm)) return 0;
+
+    // ---- capture iters_per_graph iterations
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (!hip_ok(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal), "begin capture")) return 0;
+    for (int j = 0; j < iters_per_graph; j++) {
+        launch_role<kRoleF32Doe>(t, nsys, max_sizeh, kIfRunning);
+        launch_role<kRoleF32DeoDot>(t, nsys, max_sizeh, kIfRunning);
+        fin<kFinMxAlpha>(nsys, kIfRunning, t.blk);
+        hipLaunchKernelGGL(kb_mx_xr, gsite, dim3(kBlock), 0, S(), B.dsys);
+        fin<kFinMxLambda>(nsys, kIfRunningNoRU, kBlock);
+        launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kIfRunningRU);
+        launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kIfRunningRU);
+        hipLaunchKernelGGL(kb_mx_resid, gsite, dim3(kBlock), 0, S(), B.dsys);
+        fin<kFinMxLambda>(nsys, kIfRunningRU, kBlock);
+        hipLaunchKernelGGL(kb_cg_p, gsite, dim3(kBlock), 0, S(), B.dsys);
+    }
+    const hipError_t ce = hipStreamEndCapture(S(), &graph);
+    if (!hip_ok(ce, "end capture")) return 0;
+    if (!hip_ok(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate")) {
+        (void)hipGraphDestroy(graph);
+        return 0;
+    }
+    bool ok = true;
+    long last_total = -1;
+    for (;;) {
+        if (!pull_states(nsys)) { ok = false; break; }
+        long total = 0;
+        bool running = false;
+        for (int k = 0; k < nsys; k++) {
+            total += B.st_host[k].cg;
+            running = running || B.st_host[k].status == kRunning;
+        }
+        if (!running) break;
+        if (total == last_total) {
+            ok = fail("%s: solver states did not advance", nm);
+            break;
+        }
+        last_total = total;
+        if (!hip_ok(hipGraphLaunch(exec, S()), "graph launch")) { ok = false; break; }
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (!ok) return 0;
+
+    // ---- x += out_f; FP64 true residual of every system
+    hipLaunchKernelGGL(kb_add_f_to_d, gsite, dim3(kBlock), 0, S(), B.dsys);
+    launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kAlways);
+    launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kAlways);
+    hipLaunchKernelGGL(kb_resid_norm, gsite, dim3(kBlock), 0, S(), B.dsys, 1);
+    fin<kFinStore>(nsys, kAlways, kBlock);
+    if (!hip_ok(hipGetLastError(), nm) || !pull_states(nsys)) return 0;
+    int converged = 0;
+    for (int k = 0; k < nsys; k++) {
+        const SolverState& s = B.st_host[k];
+        const double tr = std::sqrt(s.sum / s.source_norm);
+        if (cg_return) cg_return[k] = s.cg;
+        if (ru_return) ru_return[k] = s.ru_count;
+        if (true_res) true_res[k] = tr;
+        if (tr <= res) converged++;
+        else if (verbosity_lv > 0)
+            printf("WARNING: batched system %d: true residual %e above %e after %d iterations\n", k, tr, res, s.cg);
+    }
+    return converged;
+}
+
+int ostp_autotune_report(char* buf, int len) {
+    const int n = (int)g_last_report.size();
+    if (buf && len > 0) {
+        const int m = std::min(n, len - 1);
+        std::memcpy(buf, g_last_report.data(), m);
+        buf[m] = 0;
+    }
+    return n;
+}
+
+}  // extern "C"

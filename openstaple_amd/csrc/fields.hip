@@ -265,8 +265,7 @@ int ostp_compute_nnp_and_nnm(int* nnp, int* nnm) {
     const long vol = 2 * L.sizeh;
     hipLaunchKernelGGL(k_nn_tables, dim3(blocks_for(vol)), dim3(kBlock), 0, ctx().stream, dp, dm, geom_args());
     bool ok = hip_ok(hipGetLastError(), "k_nn_tables") && hip_ok(hipStreamSynchronize(ctx().stream), "sync") &&
-              hip_ok(hipMemcpy(nnp, dp, bytes, hipMemcpyDeviceToHost), "nnp") &&
-              hip_ok(hipMemcpy(nnm, dm, bytes, hipMemcpyDeviceToHost), "nnm");
+              ostp_memcpy_dtoh(nnp, dp, bytes) == 0 && ostp_memcpy_dtoh(nnm, dm, bytes) == 0;
     (void)hipFree(dp);
     (void)hipFree(dm);
     return ok ? 0 : 1;
@@ -282,7 +281,7 @@ void calc_u1_phases(double_soa* phases, bf_param bf_pars, double im_chem_pot, do
     if (!hip_ok(hipGetLastError(), "k_u1_phases") || !hip_ok(hipStreamSynchronize(ctx().stream), "sync")) return;
     // a registered host array is updated too, as the reference computes it on the host
     if (ostp_acc_is_present(phases, 8 * L.dsoa_bytes()) && (void*)d != (void*)phases)
-        hip_ok(hipMemcpy(phases, d, 8 * L.dsoa_bytes(), hipMemcpyDeviceToHost), "calc_u1_phases readback");
+        ostp_memcpy_dtoh(phases, d, 8 * L.dsoa_bytes());
 }
 
 void ostp_phases_to_float(const double_soa* phases, float_soa* phases_f) {

@@ -158,6 +158,21 @@ __device__ __forceinline__ void block_partial(double v0, double v1, const Solver
     }
 }
 
+// The same for a block of BLK threads with an explicit partial slot (batched
+// kernels, whose hardware workgroup id is remapped to a logical tile of one
+// system).
+template <int BLK, int NV = 1>
+__device__ __forceinline__ void block_partial_at(double v0, double v1, double* partials, int slot) {
+    __shared__ double lds[2][BLK / 64];
+    const double b0 = block_sum<BLK>(v0, lds[0]);
+    double b1 = 0.0;
+    if (NV == 2) b1 = block_sum<BLK>(v1, lds[1]);
+    if (threadIdx.x == 0) {
+        partials[2 * slot] = b0;
+        if (NV == 2) partials[2 * slot + 1] = b1;
+    }
+}
+
 // Sum `nparts` block partials in block order and apply finalizer F.  One
 // block; launched on the stream of the reducing kernel, with its SolverCtl
 // (the predicate reads state that only finalizers write, so it evaluates as

@@ -229,10 +229,21 @@ int ostp_acc_create(void* host, size_t bytes) {
     return 0;
 }
 
+// Host<->device transfers and fills run on the library stream and wait for it:
+// the stream is non-blocking, so a null-stream hipMemcpy/hipMemset would not be
+// ordered against the kernels queued on it (a large fill could still be
+// running while a kernel writes the same buffer).
+static hipStream_t lib_stream() { return g_ctx.stream; }
+static int stream_copy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, const char* what) {
+    hipStream_t s = lib_stream();
+    if (!hip_ok(hipMemcpyAsync(dst, src, bytes, kind, s), what)) return 1;
+    return hip_ok(s ? hipStreamSynchronize(s) : hipDeviceSynchronize(), what) ? 0 : 1;
+}
+
 int ostp_acc_update_device(const void* host, size_t bytes) {
     void* dev = resolve(host, bytes, "ostp_acc_update_device");
     if (!dev) return 1;
-    return hip_ok(hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice), "update device") ? 0 : 1;
+    return stream_copy(dev, host, bytes, hipMemcpyHostToDevice, "update device");
 }
 
 int ostp_acc_copyin(void* host, size_t bytes) {
@@ -243,8 +254,7 @@ int ostp_acc_copyin(void* host, size_t bytes) {
 int ostp_acc_update_self(void* host, size_t bytes) {
     void* dev = resolve(host, bytes, "ostp_acc_update_self");
     if (!dev) return 1;
-    if (!hip_ok(hipStreamSynchronize(g_ctx.stream ? g_ctx.stream : 0), "synchronize")) return 1;
-    return hip_ok(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost), "update self") ? 0 : 1;
+    return stream_copy(host, dev, bytes, hipMemcpyDeviceToHost, "update self");
 }
 
 int ostp_acc_delete(void* host) {
@@ -289,16 +299,17 @@ void ostp_free(void* dev) {
 }
 
 int ostp_memcpy_htod(void* dev, const void* host, size_t bytes) {
-    return hip_ok(hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice), "ostp_memcpy_htod") ? 0 : 1;
+    return stream_copy(dev, host, bytes, hipMemcpyHostToDevice, "ostp_memcpy_htod");
 }
 
 int ostp_memcpy_dtoh(void* host, const void* dev, size_t bytes) {
-    if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
-    return hip_ok(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost), "ostp_memcpy_dtoh") ? 0 : 1;
+    return stream_copy(host, dev, bytes, hipMemcpyDeviceToHost, "ostp_memcpy_dtoh");
 }
 
 int ostp_memset(void* dev, int value, size_t bytes) {
-    return hip_ok(hipMemset(dev, value, bytes), "ostp_memset") ? 0 : 1;
+    hipStream_t s = lib_stream();
+    if (!hip_ok(hipMemsetAsync(dev, value, bytes, s), "ostp_memset")) return 1;
+    return hip_ok(s ? hipStreamSynchronize(s) : hipDeviceSynchronize(), "ostp_memset") ? 0 : 1;
 }
 
 int ostp_synchronize(void) {
