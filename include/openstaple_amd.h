@@ -317,6 +317,39 @@ int ostp_batched_mixed_solve(int nsys, const ostp_batch_system *systems, double 
  * batch signature); returns its length, copies up to len-1 bytes into buf. */
 int ostp_autotune_report(char *buf, int len);
 
+/* ================================== learned preconditioner (new) ======= */
+/* No reference counterpart.  A gauge-equivariant network of two
+ * parallel-transport layers with OSTP_PTC_CHANNELS channels approximating
+ * A^-1 for A = m^2 - Deo Doe on even sites:
+ *   z = N r = w0 r + sum_c B_c^dag B_c r,  B_c = sum_nu w[c][nu] T_nu,
+ * T_nu the 8 parallel transports of D_oe (forward nu = 0..3, backward 4..7).
+ * N is Hermitian, positive for w0 > 0.  Parameters: params[0] = w0,
+ * params[1 + 2(8c + nu)] = Re w[c][nu], params[2 + 2(8c + nu)] = Im w[c][nu].
+ * FP32 links/phases (the FP32 copies of the solver), BF16 channel mixing on
+ * the matrix cores with FP32 accumulation; FP64 input and output vectors. */
+#define OSTP_PTC_CHANNELS 12
+#define OSTP_PTC_NPARAM (1 + 2 * 8 * OSTP_PTC_CHANNELS)
+int ostp_ptc_apply(const float *params, const su3_soa_f *u_f, const float_soa *phases_f, const vec3_soa *in,
+                   vec3_soa *out);
+/* Training on a pair (b, x = A^-1 b): returns L = |e|^2 (a_norm = 0) or
+ * L = <e, A e> (a_norm = 1, the error norm CG minimises; needs the FP64
+ * links u and pars->phases), e = N b - x, and writes dL/dparams
+ * (OSTP_PTC_NPARAM doubles, same order as params; Re/Im derivatives
+ * separately).  pars->ferm_mass gives A; pars->phases_f the FP32 phases. */
+double ostp_ptc_loss_grad(const float *params, const su3_soa *u, const su3_soa_f *u_f, ferm_param *pars,
+                          const vec3_soa *b, const vec3_soa *x, int a_norm, double *grad);
+/* Flexible CG (Polak-Ribiere beta, device-resident scalars) on
+ * m^2 - Deo Doe preconditioned by N; the exit test and the final check use
+ * the FP64 residual as ker_invert_openacc does (SAFETY 0.95, success iff
+ * the true residual <= res).  If flexible CG ends above res, plain CG
+ * (ker_invert_openacc semantics) continues from its iterate.  *cg_return =
+ * flexible-CG iterations, *fallback_cg = plain-CG iterations (may be NULL),
+ * *true_res = final ||b - A x|| / ||b|| (may be NULL).  u_f: FP32 copy of u;
+ * pars->phases_f: FP32 phases. */
+int ostp_fcg_solve(const su3_soa *u, ferm_param *pars, const su3_soa_f *u_f, const float *params,
+                   vec3_soa *solution, const vec3_soa *in, double res, int max_cg, int *cg_return,
+                   int *fallback_cg, double *true_res);
+
 /* ======================================================== generators ===== */
 /* Reference field rules (OpenAcc/random_assignement.c:67-92,106-142,188-250)
  * drawn from a counter-based Philox4x32-10 stream keyed by `seed` instead of

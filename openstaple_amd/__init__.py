@@ -137,6 +137,9 @@ class DefectInfoWilson(C.Structure):  # the same struct compiled without GAUGE_A
     ]
 
 
+PTC_CHANNELS = 12
+PTC_NPARAM = 1 + 2 * 8 * PTC_CHANNELS
+
 PT_ACTION_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_void_p)
 PT_UNIFORM_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
 
@@ -241,6 +244,9 @@ _SIGS = {
     "ostp_batched_mixed_solve": (_I, [_I, C.POINTER(BatchSystem), _D, _I, _D, _I, C.POINTER(_I), C.POINTER(_I),
                                       C.POINTER(_D)]),
     "ostp_autotune_report": (_I, [C.c_char_p, _I]),
+    "ostp_ptc_apply": (_I, [_P, _P, _P, _P, _P]),
+    "ostp_ptc_loss_grad": (_D, [_P, _P, _P, _P, _P, _P, _I, _P]),
+    "ostp_fcg_solve": (_I, [_P, _P, _P, _P, _P, _P, _D, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "ostp_time_dslash": (_D, [_P, _P, _P, _P, _I, _I, _I]),
     "ostp_time_copy": (_D, [_P, _P, C.c_size_t, _I]),
 }

@@ -152,6 +152,10 @@ bool launch_finalize(int fin, const SolverCtl& c, int nparts) {
         case kFinMsLambda: OSTP_F(kFinMsLambda, 1); break;
         case kFinMxAlpha: OSTP_F(kFinMxAlpha, 1); break;
         case kFinMxLambda: OSTP_F(kFinMxLambda, 1); break;
+        case kFinFcgAlpha: OSTP_F(kFinFcgAlpha, 1); break;
+        case kFinFcgRes: OSTP_F(kFinFcgRes, 1); break;
+        case kFinFcgBeta: OSTP_F(kFinFcgBeta, 2); break;
+        case kFinFcgInit: OSTP_F(kFinFcgInit, 2); break;
         default: return fail("unknown finalizer %d", fin);
     }
 #undef OSTP_F

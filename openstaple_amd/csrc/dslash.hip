@@ -123,7 +123,8 @@ bool launch_dslash(int eo, int mode, int fin, const void* u, const void* ph, con
         if (eo == 0) OSTP_L(0, kMdagM); else OSTP_L(1, kMdagM);
     } else {
         if (eo != 0) return fail("fused dot epilogue only on Deo");
-        if (fin != kFinStore && fin != kFinCgAlpha && fin != kFinMsAlpha && fin != kFinMxAlpha)
+        if (fin != kFinStore && fin != kFinCgAlpha && fin != kFinMsAlpha && fin != kFinMxAlpha &&
+            fin != kFinFcgAlpha)
             return fail("unsupported dslash finalizer %d", fin);
         OSTP_L(0, kMdagMDot);
     }

@@ -128,4 +128,33 @@ __device__ __forceinline__ void dslash_site(const typename Cplx<R>::T* const* u,
     o0.x *= half; o0.y *= half; o1.x *= half; o1.y *= half; o2.x *= half; o2.y *= half;
 }
 
+// Neighbour half-site indices of output half-site idx of parity EO (0 even,
+// 1 odd): ip[mu] = idx + mu, im[mu] = idx - mu (periodic), as in dslash_site.
+struct Nbrs {
+    int ip[4], im[4];
+};
+template <int EO>
+__device__ __forceinline__ Nbrs site_neighbours(const SiteGeom& a, int idx) {
+    const int nd0h = a.nd0h, nd1 = a.nd1, nd2 = a.nd2, nd3 = a.nd3;
+    int t = idx / nd0h;
+    const int hd0 = idx - t * nd0h;
+    int t2 = t / nd1;
+    const int d1 = t - t2 * nd1;
+    const int d3 = t2 / nd2;
+    const int d2 = t2 - d3 * nd2;
+    const int d0 = 2 * hd0 + ((d1 + d2 + d3 + EO) & 1);
+    const int row = idx - hd0;
+    const int s1 = nd0h, s2 = nd0h * nd1, s3 = s2 * nd2;
+    Nbrs n;
+    n.ip[0] = row + ((d0 + 1 == a.nd0) ? 0 : ((d0 + 1) >> 1));
+    n.im[0] = row + ((d0 == 0) ? ((a.nd0 - 1) >> 1) : ((d0 - 1) >> 1));
+    n.ip[1] = idx + (d1 == nd1 - 1 ? -(nd1 - 1) * s1 : s1);
+    n.im[1] = idx + (d1 == 0 ? (nd1 - 1) * s1 : -s1);
+    n.ip[2] = idx + (d2 == nd2 - 1 ? -(nd2 - 1) * s2 : s2);
+    n.im[2] = idx + (d2 == 0 ? (nd2 - 1) * s2 : -s2);
+    n.ip[3] = idx + (d3 == nd3 - 1 ? -(nd3 - 1) * s3 : s3);
+    n.im[3] = idx + (d3 == 0 ? (nd3 - 1) * s3 : -s3);
+    return n;
+}
+
 }  // namespace ostp

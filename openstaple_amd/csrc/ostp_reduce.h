@@ -27,6 +27,10 @@ enum Fin : int {
     kFinMxAlpha,        // mixed: omega + reliable-update test (inverter_mixedp.c:103-114)
     kFinMxLambda,       // mixed: lambda, gammag, exit test     (inverter_mixedp.c:137-155)
     kFinStore2,         // st->sum, st->sum_im (complex dot)
+    kFinFcgAlpha,       // flexible CG: alpha = <z,r> / <p,q>
+    kFinFcgRes,         // flexible CG: |r|^2, counters, exit test
+    kFinFcgBeta,        // flexible CG: beta = -alpha <z,q> / <z,r>_old ; <z,r> (two values)
+    kFinFcgInit,        // flexible CG start: <z,r> (second value), running
 };
 
 // Predicates evaluated by every block at kernel entry (uniform).
@@ -123,6 +127,20 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
         st->delta = lambda;
         st->cg += 1;
         if (!(maxiter > 0 && st->cg < st->max_cg)) st->status = kDone;
+    } else if (F == kFinFcgAlpha) {
+        st->alpha = st->delta / tot;
+        st->omega = st->alpha;  // the x/r update kernel reads omega
+    } else if (F == kFinFcgRes) {
+        st->lambda = tot;
+        st->cg += 1;
+        const bool cont = (sqrt(tot / st->source_norm) > st->res * st->safety) && st->cg < st->max_cg;
+        if (!cont) st->status = kDone;
+    } else if (F == kFinFcgBeta) {
+        st->gammag = -st->alpha * tot / st->delta;
+        st->delta = tot_im;
+    } else if (F == kFinFcgInit) {
+        st->delta = tot_im;
+        st->status = kRunning;
     } else if (F == kFinMxAlpha) {
         st->alpha = tot;
         st->omega = st->delta / tot;

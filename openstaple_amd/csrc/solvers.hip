@@ -344,6 +344,33 @@ int cg_solve(const CgBufs& B, double mass, double shift, double res, int max_cg,
     return (std::sqrt(current_res) <= res) ? INVERTER_SUCCESS : INVERTER_FAILURE;
 }
 
+// Pieces reused by the flexible CG of the learned preconditioner (ptc.hip).
+bool launch_cg_xr(void* x, void* r, const void* p, const void* s, const SolverCtl& ctl) {
+    const long sizeh = ctx().lat.sizeh;
+    hipLaunchKernelGGL((k_cg_xr<double2>), dim3(blocks_for(sizeh)), dim3(kBlock), 0, S(), (double2*)x, (double2*)r,
+                       (const double2*)p, (const double2*)s, (int)sizeh, ctl);
+    return check_launch("k_cg_xr");
+}
+bool launch_cg_p(void* p, const void* r, const SolverCtl& ctl) {
+    const long n3 = 3 * ctx().lat.sizeh;
+    hipLaunchKernelGGL((k_cg_p<double2>), dim3(blocks_for(n3)), dim3(kBlock), 0, S(), (double2*)p, (const double2*)r,
+                       n3, ctl);
+    return check_launch("k_cg_p");
+}
+int cg_solve_from(const void* u, const void* ph, const void* b, void* x, void* r, void* h, void* s, void* p,
+                  double mass, double res, int max_cg, int* cg_out, double* true_res) {
+    CgBufs B;
+    B.u = u;
+    B.ph = ph;
+    B.b = b;
+    B.x = x;
+    B.r = r;
+    B.h = h;
+    B.s = s;
+    B.p = p;
+    return cg_solve(B, mass, 0.0, res, max_cg, inverter_tricks.restartingEvery, cg_out, true_res);
+}
+
 // Mixed precision CG (inverter_mixed_precision) on device pointers.
 struct MxBufs {
     const void *u, *u_f, *ph, *ph_f, *b;

@@ -152,4 +152,12 @@ void or_manage_replica_swaps(int n, int is_evenodd, const double *cr_vec, int *l
                              or_pt_uniform_fn uniform, void *user, int *swap_num, int *all_swap_vet,
                              int *acceptance_vet);
 
+/* ---- parallel-transport preconditioner (oracle/ptc.c): w = C x 8 complex, row c */
+void or_ptc_transports(const or_geom *g, const void *u, const double *ph, const void *in, void *out);
+void or_ptc_adjoint_add(const or_geom *g, const void *u, const double *ph, const void *in, void *out);
+void or_ptc_apply(const or_geom *g, const void *u, const double *ph, int C, double w0, const or_dc *w,
+                  const void *r, void *z);
+double or_ptc_loss_grad(const or_geom *g, const void *u, const double *ph, int C, double w0, const or_dc *w,
+                        const void *b, const void *x, double *grad);
+
 #endif

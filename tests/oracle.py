@@ -162,6 +162,25 @@ class Oracle:
                                  p(phi_o), p(r), p(h), p(s), p(pp), C.byref(tr), C.byref(cg))
         return st, cg.value, x_e, x_o
 
+    # ---- parallel-transport preconditioner (oracle/ptc.c); w: (C, 8) complex
+    def ptc_transports(self, u, ph, r):
+        t = np.zeros((8, 3, self.sizeh), dtype=np.complex128)
+        lib.or_ptc_transports(self.gp, p(u), p(ph), p(np.ascontiguousarray(r)), p(t))
+        return t
+
+    def ptc_apply(self, u, ph, w0, w, r):
+        w = np.ascontiguousarray(w, dtype=np.complex128)
+        z = self.vec3()
+        lib.or_ptc_apply(self.gp, p(u), p(ph), w.shape[0], w0, p(w), p(np.ascontiguousarray(r)), p(z))
+        return z
+
+    def ptc_loss_grad(self, u, ph, w0, w, b, x):
+        w = np.ascontiguousarray(w, dtype=np.complex128)
+        g = np.zeros(1 + 2 * w.size)
+        L = lib.or_ptc_loss_grad(self.gp, p(u), p(ph), w.shape[0], w0, p(w), p(np.ascontiguousarray(b)),
+                                 p(np.ascontiguousarray(x)), p(g))
+        return L, g
+
     def init_k(self, conf, c_r, def_axis, def_vec, map4=(0, 1, 2, 3), defect_info_config=0):
         """K planes of `conf` (par_temp layout, uint8 view) set in place; returns
         the 100-int defect_info (GAUGE_ACT_TLSM layout)."""
@@ -285,6 +304,10 @@ for name, res, args in [
     ("or_eo_inversion", _I, [_G, _P, _P, _D, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(Tricks),
                              C.POINTER(_I)]),
     ("or_init_k", None, [_G, _P, _P, _D, _I, _P, _P, _I]),
+    ("or_ptc_transports", None, [_G, _P, _P, _P, _P]),
+    ("or_ptc_adjoint_add", None, [_G, _P, _P, _P, _P]),
+    ("or_ptc_apply", None, [_G, _P, _P, _I, _D, _P, _P, _P]),
+    ("or_ptc_loss_grad", _D, [_G, _P, _P, _I, _D, _P, _P, _P, _P]),
     ("or_metro_swap", _I, [_D, _D]),
 ]:
     f = getattr(lib, name)

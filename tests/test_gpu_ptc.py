@@ -1,0 +1,163 @@
+"""Learned parallel-transport preconditioner on the GPU.
+
+Bar (BF16 channel mixing with FP32 accumulation, FP32 transports):
+  * N r against the FP64 oracle (oracle/ptc.c, pinned by tests/test_oracle_ptc.py)
+    within 2e-2 relative (norm-wise);
+  * loss within 2e-2 and gradient within 5e-2 relative of the oracle, for the
+    Euclidean loss and for the A-norm loss (oracle composition);
+  * flexible CG reaches the FP64 true residual; when it is stopped early the
+    plain-CG fallback finishes the solve;
+  * training with the library kernels lowers the loss and the trained network
+    cuts the flexible-CG iteration count below plain CG's (the gain is printed).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import openstaple_amd as ost  # noqa: E402
+from openstaple_amd import learned, lib  # noqa: E402
+from gpu_util import Dev, init_lattice  # noqa: E402
+from oracle import Oracle  # noqa: E402
+
+NC = ost.PTC_CHANNELS
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+class Sys:
+    def __init__(self, shape=(8, 8, 8, 8), mass=0.005, eps=0.7, seed=2):
+        self.o = o = Oracle(*shape)
+        init_lattice(shape)
+        self.u = o.stout(o.stout(o.gen_cold(eps, seed), 0.15), 0.15)
+        self.ph = o.phases()
+        self.uf = o.to_float_gauge(self.u)
+        self.du, self.duf = Dev(self.u), Dev(self.uf)
+        self.dph, self.dphf = Dev(self.ph), Dev(self.ph.astype(np.float32))
+        self.mass = mass
+        self.fp = ost.ferm_param(mass, self.dph.p, self.dphf.p)
+        self.op = learned.Operator(self.du.p, self.duf.p, self.fp)
+
+
+def unpack(p):
+    return p[0], (p[1::2] + 1j * p[2::2]).reshape(NC, 8)
+
+
+@pytest.fixture(scope="module")
+def s8():
+    return Sys()
+
+
+def random_params(seed, w0=0.3, scale=0.3):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(ost.PTC_NPARAM)
+    p[0] = w0
+    p[1:] = scale * rng.standard_normal(ost.PTC_NPARAM - 1)
+    return p.astype(np.float32).astype(np.float64)  # the values the kernels see
+
+
+def test_apply_vs_oracle(s8):
+    o = s8.o
+    for seed in (1, 2):
+        p = random_params(seed)
+        r = o.gen_gauss(20 + seed)
+        dr, dz = Dev(r), Dev(nbytes=r.nbytes)
+        p32 = p.astype(np.float32)
+        assert lib.ostp_ptc_apply(p32.ctypes.data, s8.duf.p, s8.dphf.p, dr.p, dz.p) == 0
+        ost.check(0, "ptc apply")
+        w0, w = unpack(p)
+        ref = o.ptc_apply(s8.u, s8.ph, w0, w, r)
+        assert rel(dz.get(r), ref) < 2e-2
+    # the closed form: channel 0 = +-1 (exact in BF16) gives 0.7 - 4 Deo Doe
+    p = np.zeros(ost.PTC_NPARAM)
+    p[0] = 0.7
+    for nu in range(8):
+        p[1 + 2 * nu] = 1.0 if nu < 4 else -1.0
+    p32 = p.astype(np.float32)
+    r = o.gen_gauss(30)
+    dr, dz = Dev(r), Dev(nbytes=r.nbytes)
+    lib.ostp_ptc_apply(p32.ctypes.data, s8.duf.p, s8.dphf.p, dr.p, dz.p)
+    ref = 0.7 * r - 4.0 * o.deo(s8.u, o.doe(s8.u, r, s8.ph), s8.ph)
+    assert rel(dz.get(r), ref) < 1e-2
+
+
+def oracle_anorm_grad(s, p, b, x):
+    """A-norm loss and gradient composed from the oracle's pieces."""
+    o = s.o
+    w0, w = unpack(p)
+    e = o.ptc_apply(s.u, s.ph, w0, w, b) - x
+    f = o.mdagm(s.u, e, s.ph, s.mass)
+    t = o.ptc_transports(s.u, s.ph, b)
+    a = o.ptc_transports(s.u, s.ph, f)
+    P = np.einsum("nci,mci->nm", np.conj(a), t)
+    S = np.einsum("cm,nm->cn", np.conj(w), np.conj(P)) + np.einsum("cm,mn->cn", np.conj(w), P)
+    G = 2 * np.conj(S)
+    g = np.zeros(ost.PTC_NPARAM)
+    g[0] = 2 * np.vdot(f, b).real
+    g[1::2], g[2::2] = G.real.ravel(), G.imag.ravel()
+    return np.vdot(e, f).real, g
+
+
+@pytest.mark.parametrize("a_norm", [0, 1])
+def test_loss_grad_vs_oracle(s8, a_norm):
+    o = s8.o
+    p = random_params(5, w0=-0.2, scale=0.1)
+    b = o.gen_gauss(40)
+    st, cg, x = o.cg(s8.u, s8.ph, s8.mass, b, 1e-10)
+    db, dx = Dev(b), Dev(x)
+    L, g = learned.loss_grad(s8.op, p, db.p, dx.p, a_norm=bool(a_norm))
+    w0, w = unpack(p)
+    if a_norm:
+        Lr, gr = oracle_anorm_grad(s8, p, b, x)
+    else:
+        Lr, gr = o.ptc_loss_grad(s8.u, s8.ph, w0, w, b, x)
+    assert abs(L - Lr) <= 2e-2 * abs(Lr), (L, Lr)
+    assert np.linalg.norm(g - gr) <= 5e-2 * np.linalg.norm(gr), (np.linalg.norm(g - gr), np.linalg.norm(gr))
+
+
+def plain_cg(s, b):
+    d64 = [Dev(nbytes=b.nbytes) for _ in range(4)]
+    db, x = Dev(b), Dev(nbytes=b.nbytes)
+    cg, tr = C.c_int(0), C.c_double(0)
+    st = lib.ostp_cg_solve(s.du.p, C.byref(s.fp), x.p, db.p, 1e-10, *(d.p for d in d64), 40000, 0.0, 10000,
+                           C.byref(cg), C.byref(tr))
+    assert st == 1
+    return cg.value
+
+
+def test_fcg_converges_and_falls_back(s8):
+    o = s8.o
+    b = o.gen_gauss(50)
+    db, x = Dev(b), Dev(nbytes=b.nbytes)
+    p = learned.init_params()
+    st, its, fb, tr = learned.solve(s8.op, p, x.p, db.p, 1e-10)
+    assert st == 1 and tr <= 1e-10 and fb == 0
+    r = b - o.mdagm(s8.u, x.get(b), s8.ph, s8.mass)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+    # stopped after 5 iterations: the plain-CG fallback finishes from the iterate
+    x.put(np.zeros_like(b))
+    st, its, fb, tr = learned.solve(s8.op, p, x.p, db.p, 1e-10, max_cg=5)
+    assert st == 1 and its == 5 and fb > 0 and tr <= 1e-10
+
+
+def test_training_gains_over_plain_cg(s8):
+    """Train on 16 pairs with the library kernels; report and check the gain."""
+    pairs = learned.Pairs(s8.op, s8.o.vec3().nbytes, 16, seed=77)
+    try:
+        p0 = learned.init_params()
+        p, hist = learned.train(s8.op, pairs, p0, steps=250, lr=0.02, batch=8)
+        assert hist[-20:].mean() < 0.8 * hist[:5].mean()
+        b = s8.o.gen_gauss(999)
+        cg_its = plain_cg(s8, b)
+        db, x = Dev(b), Dev(nbytes=b.nbytes)
+        st, its, fb, tr = learned.solve(s8.op, p, x.p, db.p, 1e-10)
+        print(f"\n8^4 m={s8.mass}: plain CG {cg_its} iterations, flexible CG with the trained network {its} "
+              f"(+{fb} fallback), loss {hist[:5].mean():.3f} -> {hist[-20:].mean():.3f}")
+        assert st == 1 and tr <= 1e-10 and fb == 0
+        assert its < 0.8 * cg_its
+    finally:
+        pairs.free()
