@@ -49,6 +49,10 @@ def parse():
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--learned-pairs", type=int, default=512,
+                    help="(b, x) pairs from plain FP64 CG to train the preconditioner on (0: skip)")
+    ap.add_argument("--learned-steps", type=int, default=300)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve workload")
     ap.add_argument("--dslash-only", action="store_true",
                     help="only the timed Dslash loop and the copy kernel (for counter runs)")
     return ap.parse_args()
@@ -179,7 +183,9 @@ def main():
         print(json.dumps(result))
         return
     if not args.no_solves:
-        result["solves"] = run_solves(ost, lib, L, u, ph, b, uf, phf)
+        result["solves"] = run_solves(ost, lib, L, u, ph, b, uf, phf, args)
+    if not args.no_batch:
+        result["batched"] = run_batched(ost, lib)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(shape, args.cpu_seconds)
@@ -199,7 +205,7 @@ def smeared_field(lib, dmalloc, gb, eps, seed):
     return arr + (steps - 1) * gb  # the smeared levels stay allocated for the whole run
 
 
-def run_solves(ost, lib, L, u, ph, b, uf, phf):
+def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     """Wall time of each solve to ||b - M x|| / ||b|| < 1e-10 (FP64 true residual)."""
     sizeh = L.sizeh
     vb = L.vec3_bytes
@@ -293,10 +299,171 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf):
     res["cg_mixed"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "reliable_updates": ru.value,
                        "true_residual": tr.value, "status": st}
     res["cg_speedup_mixed_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
+
+    # the same mixed-precision algorithm as one system of the batched solver: its
+    # iterations captured in a HIP graph, scalars on the device
+    sysd = ost.BatchSystem()
+    sysd.nd[:] = L.nd
+    sysd.par_temp = 0
+    sysd.u, sysd.u_f, sysd.phases, sysd.phases_f = u4, u4f, ph, phf
+    sysd.in_, sysd.out, sysd.mass = b, x, m4
+    arr = (ost.BatchSystem * 1)(sysd)
+    cgs, rus, trs = (C.c_int * 1)(), (C.c_int * 1)(), (C.c_double * 1)()
+    lib.ostp_memset(x, 0, vb)
+    lib.ostp_batched_mixed_solve(1, arr, 1e-10, 40000, 0.1, 16, cgs, rus, trs)  # warm-up: autotune, graph
+    lib.ostp_memset(x, 0, vb)
+    lib.ostp_synchronize()
+    t0 = time.perf_counter()
+    conv = lib.ostp_batched_mixed_solve(1, arr, 1e-10, 40000, 0.1, 16, cgs, rus, trs)
+    lib.ostp_synchronize()
+    t1 = time.perf_counter()
+    res["cg_mixed_graph"] = {"seconds": round(t1 - t0, 4), "iterations": cgs[0], "reliable_updates": rus[0],
+                             "true_residual": trs[0], "status": int(conv == 1)}
+    res["cg_speedup_mixed_graph_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
+
+    # learned preconditioner: trained on (b, x) pairs from the plain FP64 CG with
+    # the library's kernels, used in flexible CG (FP64 true residual, CG fallback)
+    if args.learned_pairs > 0:
+        from openstaple_amd import learned
+        op = learned.Operator(u4, u4f, fp4)
+        t0 = time.perf_counter()
+        pairs = learned.Pairs(op, vb, args.learned_pairs, seed=1234)
+        t1 = time.perf_counter()
+        params, hist = learned.train(op, pairs, learned.init_params(), steps=args.learned_steps, lr=0.02, batch=8)
+        t2 = time.perf_counter()
+        pairs.free()
+        lib.ostp_memset(x, 0, vb)
+        learned.solve(op, params, x, b, 1e-10)  # warm-up (scratch allocation)
+        lib.ostp_memset(x, 0, vb)
+        lib.ostp_synchronize()
+        t3 = time.perf_counter()
+        st, its, fb, trr = learned.solve(op, params, x, b, 1e-10)
+        lib.ostp_synchronize()
+        t4 = time.perf_counter()
+        res["cg_learned_fcg"] = {"seconds": round(t4 - t3, 4), "iterations": its, "fallback_iterations": fb,
+                                 "true_residual": trr, "status": st,
+                                 "training": {"pairs": args.learned_pairs, "pair_seconds": round(t1 - t0, 2),
+                                              "steps": args.learned_steps, "train_seconds": round(t2 - t1, 2),
+                                              "loss_first": round(float(hist[:5].mean()), 4),
+                                              "loss_last": round(float(hist[-20:].mean()), 4),
+                                              "loss": "A-norm of N b - x over <x, b>"}}
+        res["cg_speedup_learned_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t4 - t3), 3)
+        res["cg_iteration_ratio_learned_vs_fp64"] = round(res["cg_fp64"]["iterations"] / max(1, its + fb), 3)
     ost.check(0, "solves")
     for p in [*d64, *d32, out, sf, sf32, x, u4f]:
         lib.ostp_free(p)
     return res
+
+
+def run_batched(ost, lib):
+    """24 Z2 sources on a 32^3x8 field plus 6 parallel-tempering replicas on
+    24^3x8 fields (K = c_r on the t-boundary defect, c_r = 0, 0.2, ..., 1),
+    m = 0.02, to 1e-9: one batched solve (HIP graph) against the same 30
+    mixed-precision solves run one after the other."""
+    def dm(n):
+        p = lib.ostp_malloc(n)
+        lib.ostp_memset(p, 0, n)
+        return p
+
+    mass, res = 0.02, 1e-9
+    groups = []  # (shape, par_temp, [(u, uf, ph, phf, b, x)])
+    # 32^3 x 8, one field, 24 sources
+    shape = (32, 32, 32, 8)
+    La = ost.Lattice(*shape)
+    La.init()
+    lib.ostp_set_stout(0.15, 2, 0.0, 0, 0, 1)
+    gb = 8 * La.su3_bytes
+    u = smeared_field(lib, dm, gb, 0.5, 3)
+    uf = dm(8 * La.su3f_bytes)
+    lib.ostp_convert_double_to_float_gauge(u, uf)
+    ph, phf = dm(8 * 8 * La.sizeh), dm(4 * 8 * La.sizeh)
+    lib.calc_u1_phases(ph, ost.BfParam(0, 0, 0, 0, 0, 0), 0.0, 0.0)
+    lib.ostp_phases_to_float(ph, phf)
+    items = []
+    for k in range(24):
+        bz, xz = dm(La.vec3_bytes), dm(La.vec3_bytes)
+        lib.ostp_generate_vec3_soa_z2noise(bz, 5, k)
+        items.append((u, uf, ph, phf, bz, xz))
+    groups.append((shape, 0, items))
+    # 6 replicas on 24^3 x 8 with the K plane
+    shape = (24, 24, 24, 8)
+    Lb = ost.Lattice(*shape, par_temp=True)
+    Lb.init()
+    gb = 8 * Lb.su3_bytes
+    ph, phf = dm(8 * 8 * Lb.sizeh), dm(4 * 8 * Lb.sizeh)
+    lib.calc_u1_phases(ph, ost.BfParam(0, 0, 0, 0, 0, 0), 0.0, 0.0)
+    lib.ostp_phases_to_float(ph, phf)
+    items = []
+    for r in range(6):
+        ur = smeared_field(lib, dm, gb, 0.5, 20 + r)
+        lib.init_k(ur, 0.2 * r, 3, (C.c_int * 3)(24, 24, 24), None, 1)
+        urf = dm(8 * Lb.su3f_bytes)
+        lib.ostp_convert_double_to_float_gauge(ur, urf)
+        bz, xz = dm(Lb.vec3_bytes), dm(Lb.vec3_bytes)
+        lib.ostp_generate_vec3_soa_z2noise(bz, 6, r)
+        items.append((ur, urf, ph, phf, bz, xz))
+    groups.append((shape, 1, items))
+    ost.check(0, "batched setup")
+
+    systems = []
+    for shape, pt, items in groups:
+        for (uu, uuf, pp, ppf, bz, xz) in items:
+            sy = ost.BatchSystem()
+            sy.nd[:] = shape
+            sy.par_temp = pt
+            sy.u, sy.u_f, sy.phases, sy.phases_f, sy.in_, sy.out, sy.mass = uu, uuf, pp, ppf, bz, xz, mass
+            systems.append(sy)
+    n = len(systems)
+    arr = (ost.BatchSystem * n)(*systems)
+    cg, ru, tr = (C.c_int * n)(), (C.c_int * n)(), (C.c_double * n)()
+    lib.ostp_batched_mixed_solve(n, arr, res, 40000, 0.1, 16, cg, ru, tr)  # warm-up: autotune
+    for sy in systems:
+        lib.ostp_memset(sy.out, 0, 48 * sy.nd[0] * sy.nd[1] * sy.nd[2] * sy.nd[3] // 2)
+    lib.ostp_synchronize()
+    t0 = time.perf_counter()
+    conv = lib.ostp_batched_mixed_solve(n, arr, res, 40000, 0.1, 16, cg, ru, tr)
+    lib.ostp_synchronize()
+    t1 = time.perf_counter()
+    # algorithmic Dslash bytes: FP32 pairs per iteration, FP64 pairs per reliable update and at start/end
+    dbytes = 0
+    for k, sy in enumerate(systems):
+        sizeh = sy.nd[0] * sy.nd[1] * sy.nd[2] * sy.nd[3] // 2
+        dbytes += 2 * sizeh * (BYTES_PER_SITE_F32 * cg[k] + BYTES_PER_SITE_F64 * (ru[k] + 2))
+    out = {"systems": n, "converged": conv, "seconds": round(t1 - t0, 4), "iterations_total": sum(cg),
+           "iterations_max": max(cg), "max_true_residual": max(tr[k] for k in range(n)),
+           "dslash_GBs_aggregate": round(dbytes / (t1 - t0) / 1e9, 1), "mass": mass, "tolerance": res}
+    # the same systems one by one with the single-system mixed solver
+    t_seq, it_seq = 0.0, 0
+    for (shape, pt, items) in groups:
+        Lx = ost.Lattice(*shape, par_temp=bool(pt))
+        Lx.init()
+        vb = Lx.vec3_bytes
+        d64 = [dm(vb) for _ in range(4)]
+        d32 = [dm(vb // 2) for _ in range(5)]
+        for (uu, uuf, pp, ppf, bz, xz) in items:
+            fp = ost.ferm_param(mass, pp, ppf)
+            ip = ost.InverterPackage()
+            ip.u, ip.u_f = uu, uuf
+            ip.loc_r, ip.loc_h, ip.loc_s, ip.loc_p = d64
+            ip.loc_r_f, ip.loc_h_f, ip.loc_s_f, ip.loc_p_f, ip.out_f = d32
+            lib.ostp_memset(xz, 0, vb)
+            c1, r1, tr1 = C.c_int(0), C.c_int(0), C.c_double(0)
+            lib.ostp_synchronize()
+            ta = time.perf_counter()
+            lib.ostp_mixed_solve(ip, C.byref(fp), xz, bz, res, 40000, 0.0, 0.1, C.byref(c1), C.byref(r1), C.byref(tr1))
+            lib.ostp_synchronize()
+            t_seq += time.perf_counter() - ta
+            it_seq += c1.value
+        for p_ in d64 + d32:
+            lib.ostp_free(p_)
+    out["sequential_seconds"] = round(t_seq, 4)
+    out["sequential_iterations_total"] = it_seq
+    out["speedup_batched_vs_sequential"] = round(t_seq / (t1 - t0), 3)
+    rep = C.create_string_buffer(4096)
+    lib.ostp_autotune_report(rep, 4096)
+    out["autotune"] = rep.value.decode().strip().splitlines()[-1] if rep.value else ""
+    ost.check(0, "batched")
+    return out
 
 
 def cpu_baseline(shape, seconds):

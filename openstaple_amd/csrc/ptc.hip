@@ -2,20 +2,22 @@
 // A = m^2 - D_eo D_oe, and the flexible CG that uses it.
 //
 // Network (2 parallel-transport layers, C = 12 channels, complex weights
-// w[c][nu], real w0 > 0):
+// w[c][nu], real w0):
 //   layer 1 (even -> odd)  h_c = sum_nu w[c][nu] T_nu r
 //   layer 2 (odd -> even)  z   = w0 r + sum_nu T_nu^dag g_nu,  g_nu = sum_c conj(w[c][nu]) h_c
 // with T_nu the eight parallel transports of D_oe (fermion_matrix.c:101-157,
-// before its 1/2 and signs): N = w0 + sum_c B_c^dag B_c is Hermitian positive,
-// as conjugate-gradient methods need.  Links and transports are FP32; the
+// before its 1/2 and signs): N = w0 + sum_c B_c^dag B_c is Hermitian (positive
+// for w0 > 0; trained networks have w0 < 0 with N still positive on the
+// spectrum), as conjugate-gradient methods need.  Links and transports are FP32; the
 // channel mixing (h = W t, g = W^dag h) runs on the matrix cores as two
 // chained BF16 MFMAs with FP32 accumulation (32x32x8: the 32x32 result of the
 // first product has exactly the lane layout of the B operand of the second,
 // so h never leaves registers); the transported vectors reach the MFMA
 // operand layout through LDS.
 //
-// Training kernels: the loss |N b - x|^2 over (b, x) pairs, its gradient via
-// the 8x8 Gram matrix P[nu][mu] = <T_nu e, T_mu b> (e = N b - x).  The CPU
+// Training kernels: the loss |e|^2 or <e, A e> (e = N b - x) over (b, x)
+// pairs, its gradient via the 8x8 Gram matrix P[nu][mu] = <T_nu a, T_mu b>
+// with a = e or A e.  The CPU
 // oracle (oracle/ptc.c) restates forward, loss and gradient in FP64.
 #include "ostp_dslash.h"
 
