@@ -320,6 +320,18 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     res["cg_mixed_graph"] = {"seconds": round(t1 - t0, 4), "iterations": cgs[0], "reliable_updates": rus[0],
                              "true_residual": trs[0], "status": int(conv == 1)}
     res["cg_speedup_mixed_graph_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
+    # FP16 storage in the inner solver (same graph-captured driver)
+    lib.ostp_memset(x, 0, vb)
+    lib.ostp_batched_half_solve(1, arr, 1e-10, 40000, 0.1, 16, cgs, rus, trs)  # warm-up
+    lib.ostp_memset(x, 0, vb)
+    lib.ostp_synchronize()
+    t0 = time.perf_counter()
+    conv = lib.ostp_batched_half_solve(1, arr, 1e-10, 40000, 0.1, 16, cgs, rus, trs)
+    lib.ostp_synchronize()
+    t1 = time.perf_counter()
+    res["cg_half_graph"] = {"seconds": round(t1 - t0, 4), "iterations": cgs[0], "reliable_updates": rus[0],
+                            "true_residual": trs[0], "status": int(conv == 1)}
+    res["cg_speedup_half_graph_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
 
     # learned preconditioner: trained on (b, x) pairs from the plain FP64 CG with
     # the library's kernels, used in flexible CG (FP64 true residual, CG fallback)
@@ -429,9 +441,23 @@ def run_batched(ost, lib):
     for k, sy in enumerate(systems):
         sizeh = sy.nd[0] * sy.nd[1] * sy.nd[2] * sy.nd[3] // 2
         dbytes += 2 * sizeh * (BYTES_PER_SITE_F32 * cg[k] + BYTES_PER_SITE_F64 * (ru[k] + 2))
+    # FP16-storage inner solver on the same batch
+    for sy in systems:
+        lib.ostp_memset(sy.out, 0, 48 * sy.nd[0] * sy.nd[1] * sy.nd[2] * sy.nd[3] // 2)
+    ch, rh, trh = (C.c_int * n)(), (C.c_int * n)(), (C.c_double * n)()
+    lib.ostp_batched_half_solve(n, arr, res, 40000, 0.1, 16, ch, rh, trh)  # warm-up: autotune
+    for sy in systems:
+        lib.ostp_memset(sy.out, 0, 48 * sy.nd[0] * sy.nd[1] * sy.nd[2] * sy.nd[3] // 2)
+    lib.ostp_synchronize()
+    th0 = time.perf_counter()
+    convh = lib.ostp_batched_half_solve(n, arr, res, 40000, 0.1, 16, ch, rh, trh)
+    lib.ostp_synchronize()
+    th1 = time.perf_counter()
     out = {"systems": n, "converged": conv, "seconds": round(t1 - t0, 4), "iterations_total": sum(cg),
            "iterations_max": max(cg), "max_true_residual": max(tr[k] for k in range(n)),
-           "dslash_GBs_aggregate": round(dbytes / (t1 - t0) / 1e9, 1), "mass": mass, "tolerance": res}
+           "dslash_GBs_aggregate": round(dbytes / (t1 - t0) / 1e9, 1), "mass": mass, "tolerance": res,
+           "half": {"converged": convh, "seconds": round(th1 - th0, 4), "iterations_total": sum(ch),
+                    "max_true_residual": max(trh[k] for k in range(n))}}
     # the same systems one by one with the single-system mixed solver
     t_seq, it_seq = 0.0, 0
     for (shape, pt, items) in groups:

@@ -313,6 +313,17 @@ typedef struct ostp_batch_system_t {
 int ostp_batched_mixed_solve(int nsys, const ostp_batch_system *systems, double res, int max_cg,
                              double mixed_delta, int iters_per_graph, int *cg_return, int *ru_return,
                              double *true_res);
+/* The same with FP16 storage in the inner solver (links rows 0,1 and phases
+ * as (cos, sin), inner vectors; FP32 arithmetic, FP32 solution accumulator,
+ * FP64 reliable updates).  The inner vectors are scaled so that their
+ * entries stay O(1): at the start and at every reliable update the FP64
+ * residual is scaled to mean |entry|^2 = 1 and the search direction by the
+ * ratio of the new and old scales; delta carries the same factor, so the
+ * recurrences are the reference's in exact arithmetic.  No reference
+ * counterpart (the reference's inner precision is FP32). */
+int ostp_batched_half_solve(int nsys, const ostp_batch_system *systems, double res, int max_cg,
+                            double mixed_delta, int iters_per_graph, int *cg_return, int *ru_return,
+                            double *true_res);
 /* Text of the autotuner's measurements so far (candidates and choice per
  * batch signature); returns its length, copies up to len-1 bytes into buf. */
 int ostp_autotune_report(char *buf, int len);

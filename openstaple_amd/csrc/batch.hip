@@ -36,6 +36,15 @@ static_assert(sizeof(ostp_batch_system) == 80 && offsetof(ostp_batch_system, mas
 
 namespace ostp {
 
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 h2f(h2 v) { return make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ h2 f2h(float x, float y) {
+    h2 r;
+    r.x = (_Float16)x;
+    r.y = (_Float16)y;
+    return r;
+}
+
 struct BSys {
     const double2* u64[8];
     const double* ph64[8];
@@ -49,6 +58,11 @@ struct BSys {
     double m2, m2f;
     SolverState* st;
     double* partials;
+    // FP16 inner solver: links rows 0,1 and phases (cos, sin) in half precision,
+    // inner vectors in half precision (out_f stays FP32)
+    const h2* uh[8];
+    const h2* phh[8];
+    h2 *rh, *hh, *sh, *pph;
 };
 
 enum BRole : int {
@@ -56,7 +70,49 @@ enum BRole : int {
     kRoleF32DeoDot,    // sf = m2f pf - D_eo hf, partial <pf,sf> (FP32)
     kRoleF64Doe,       // h64 = D_oe x                          (FP64)
     kRoleF64Deo,       // s64 = m2 x - D_eo h64                 (FP64)
+    kRoleH16Doe,       // hh = D_oe pph                         (FP16 storage)
+    kRoleH16DeoDot,    // sh = m2f pph - D_eo hh, partial <pph,sh> (FP16 storage)
 };
+
+// One hop with half-precision storage (FP32 arithmetic); the phase is stored
+// as (cos, sin).  o += U e^{i th} in  or  o -= U^dag e^{-i th} in.
+template <bool DAG>
+__device__ __forceinline__ void hop_h(const h2* __restrict__ U, const h2* __restrict__ P, const h2* __restrict__ in,
+                                      int sizeh, int iu, int iv, float2& o0, float2& o1, float2& o2) {
+    const float2 a0 = h2f(U[iu]), a1 = h2f(U[sizeh + iu]), a2 = h2f(U[2 * sizeh + iu]);
+    const float2 b0 = h2f(U[3 * sizeh + iu]), b1 = h2f(U[4 * sizeh + iu]), b2 = h2f(U[5 * sizeh + iu]);
+    const float2 cs = h2f(P[iu]);
+    const float2 ph = DAG ? make_float2(cs.x, -cs.y) : cs;
+    const float2 v0 = cmul(h2f(in[iv]), ph), v1 = cmul(h2f(in[sizeh + iv]), ph), v2 = cmul(h2f(in[2 * sizeh + iv]), ph);
+    const float2 x0 = csub(cmul(a1, b2), cmul(a2, b1));
+    const float2 x1 = csub(cmul(a2, b0), cmul(a0, b2));
+    const float2 x2 = csub(cmul(a0, b1), cmul(a1, b0));
+    if (!DAG) {
+        float2 r0 = cmul(a0, v0); r0 = cfma(a1, v1, r0); r0 = cfma(a2, v2, r0);
+        float2 r1 = cmul(b0, v0); r1 = cfma(b1, v1, r1); r1 = cfma(b2, v2, r1);
+        float2 r2 = cmul(cconj(x0), v0); r2 = cfmac(x1, v1, r2); r2 = cfmac(x2, v2, r2);
+        o0 = cadd(o0, r0); o1 = cadd(o1, r1); o2 = cadd(o2, r2);
+    } else {
+        float2 r0 = cmul(cconj(a0), v0); r0 = cfmac(b0, v1, r0); r0 = cfma(x0, v2, r0);
+        float2 r1 = cmul(cconj(a1), v0); r1 = cfmac(b1, v1, r1); r1 = cfma(x1, v2, r1);
+        float2 r2 = cmul(cconj(a2), v0); r2 = cfmac(b2, v1, r2); r2 = cfma(x2, v2, r2);
+        o0 = csub(o0, r0); o1 = csub(o1, r1); o2 = csub(o2, r2);
+    }
+}
+
+template <int EO>
+__device__ __forceinline__ void dslash_site_h(const h2* const* u, const h2* const* ph, const h2* __restrict__ in,
+                                              const SiteGeom& g, int idx, float2& o0, float2& o1, float2& o2) {
+    const Nbrs nb = site_neighbours<EO>(g, idx);
+    constexpr int F = (EO == 0) ? 0 : 1;
+    constexpr int B = 1 - F;
+    o0 = o1 = o2 = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) hop_h<true>(u[2 * mu + B], ph[2 * mu + B], in, g.sizeh, nb.im[mu], nb.im[mu], o0, o1, o2);
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) hop_h<false>(u[2 * mu + F], ph[2 * mu + F], in, g.sizeh, idx, nb.ip[mu], o0, o1, o2);
+    o0.x *= 0.5f; o0.y *= 0.5f; o1.x *= 0.5f; o1.y *= 0.5f; o2.x *= 0.5f; o2.y *= 0.5f;
+}
 
 __device__ __forceinline__ SolverCtl bctl(const BSys& S, int pred) {
     SolverCtl c;
@@ -74,7 +130,7 @@ __device__ __forceinline__ int tile_of(int ntiles) {
 
 template <int ROLE, int BLK>
 __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, int pred, int tmajor) {
-    using R = typename std::conditional<(ROLE <= kRoleF32DeoDot), float, double>::type;
+    using R = typename std::conditional<(ROLE <= kRoleF32DeoDot || ROLE >= kRoleH16Doe), float, double>::type;
     using C = typename Cplx<R>::T;
     const BSys& S = sys[blockIdx.y];
     if (!pred_pass(bctl(S, pred))) return;
@@ -90,6 +146,32 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
     } else {
         idx = tile * BLK + (int)threadIdx.x;
     }
+    if constexpr (ROLE >= kRoleH16Doe) {
+        constexpr int EOH = (ROLE == kRoleH16Doe) ? 1 : 0;
+        double dot = 0.0;
+        if (idx < g.sizeh) {
+            float2 o0, o1, o2;
+            if constexpr (EOH == 1) {
+                dslash_site_h<1>(S.uh, S.phh, S.pph, g, idx, o0, o1, o2);
+                S.hh[idx] = f2h(o0.x, o0.y);
+                S.hh[g.sizeh + idx] = f2h(o1.x, o1.y);
+                S.hh[2 * g.sizeh + idx] = f2h(o2.x, o2.y);
+            } else {
+                dslash_site_h<0>(S.uh, S.phh, S.hh, g, idx, o0, o1, o2);
+                const float m2 = (float)S.m2f;
+                const float2 x0 = h2f(S.pph[idx]), x1 = h2f(S.pph[g.sizeh + idx]), x2 = h2f(S.pph[2 * g.sizeh + idx]);
+                const h2 s0 = f2h(x0.x * m2 - o0.x, x0.y * m2 - o0.y);
+                const h2 s1 = f2h(x1.x * m2 - o1.x, x1.y * m2 - o1.y);
+                const h2 s2 = f2h(x2.x * m2 - o2.x, x2.y * m2 - o2.y);
+                S.sh[idx] = s0;
+                S.sh[g.sizeh + idx] = s1;
+                S.sh[2 * g.sizeh + idx] = s2;
+                dot = re_dot(x0, h2f(s0)) + re_dot(x1, h2f(s1)) + re_dot(x2, h2f(s2));
+            }
+        }
+        if constexpr (EOH == 0) block_partial_at<BLK, 1>(dot, 0.0, S.partials, tile);
+        return;
+    } else {
     constexpr int EO = (ROLE == kRoleF32Doe || ROLE == kRoleF64Doe) ? 1 : 0;
     const C* const* u;
     const R* const* ph;
@@ -126,6 +208,7 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
         }
     }
     if constexpr (ROLE == kRoleF32DeoDot) block_partial_at<BLK, 1>(dot, 0.0, S.partials, tile);
+    }
 }
 
 // Elementwise kernels: one thread per half-site, 3 colours each, kBlock threads.
@@ -253,6 +336,135 @@ __global__ __launch_bounds__(kBlock) void kb_resid_norm(const BSys* __restrict__
     block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
 }
 
+// ---- FP16-storage inner solver (FP32 arithmetic, FP64 reliable updates).
+// Inner vectors hold scale x (true vector); the scale makes the residual's
+// entries O(1) after every reliable update (see kFinHScale).
+
+// r64 = b - s64 (FP64), partial |r64|^2
+__global__ __launch_bounds__(kBlock) void kb_h_resid64(const BSys* __restrict__ sys, int pred) {
+    BATCH_SITE_PROLOGUE(pred)
+    double v = 0.0;
+    if (i < sh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = c * sh + i;
+            const double2 bb = S.b[k], ss = S.s64[k];
+            const double2 rr = make_double2(bb.x - ss.x, bb.y - ss.y);
+            S.r64[k] = rr;
+            v += rr.x * rr.x + rr.y * rr.y;
+        }
+    }
+    block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
+}
+
+// r_h = (half) scale r64; start: p_h = r_h, out_f = 0; otherwise p_h *= rho.
+// Partial |r_h|^2 of the stored (rounded) values.
+__global__ __launch_bounds__(kBlock) void kb_h_rescale(const BSys* __restrict__ sys, int pred, int start) {
+    BATCH_SITE_PROLOGUE(pred)
+    const double sc = S.st->scale;
+    const float rho = (float)S.st->rho;
+    double v = 0.0;
+    if (i < sh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = c * sh + i;
+            const double2 rr = S.r64[k];
+            const h2 rh = f2h((float)(sc * rr.x), (float)(sc * rr.y));
+            S.rh[k] = rh;
+            const float2 rf = h2f(rh);
+            v += (double)rf.x * rf.x + (double)rf.y * rf.y;
+            if (start) {
+                S.pph[k] = rh;
+                S.outf[k] = make_float2(0.f, 0.f);
+            } else {
+                const float2 pp = h2f(S.pph[k]);
+                S.pph[k] = f2h(pp.x * rho, pp.y * rho);
+            }
+        }
+    }
+    block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
+}
+
+// out_f += omega p_h; then r_h -= omega s_h with partial |r_h|^2, or (reliable
+// update due) x += out_f / scale, out_f = 0
+__global__ __launch_bounds__(kBlock) void kb_h_xr(const BSys* __restrict__ sys) {
+    BATCH_SITE_PROLOGUE(kIfRunning)
+    const float om = (float)S.st->omega;
+    const int ru = S.st->ru;
+    const double inv = 1.0 / S.st->scale;
+    double v = 0.0;
+    if (i < sh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = c * sh + i;
+            const float2 pp = h2f(S.pph[k]), oo = S.outf[k];
+            const float2 on = make_float2(fmaf(pp.x, om, oo.x), fmaf(pp.y, om, oo.y));
+            if (!ru) {
+                S.outf[k] = on;
+                const float2 ss = h2f(S.sh[k]), rr = h2f(S.rh[k]);
+                const h2 rn = f2h(fmaf(ss.x, -om, rr.x), fmaf(ss.y, -om, rr.y));
+                S.rh[k] = rn;
+                const float2 rf = h2f(rn);
+                v += (double)rf.x * rf.x + (double)rf.y * rf.y;
+            } else {
+                const double2 xx = S.x[k];
+                S.x[k] = make_double2(xx.x + inv * (double)on.x, xx.y + inv * (double)on.y);
+                S.outf[k] = make_float2(0.f, 0.f);
+            }
+        }
+    }
+    if (!ru) block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
+}
+
+// p_h = p_h gammag + r_h
+__global__ __launch_bounds__(kBlock) void kb_h_p(const BSys* __restrict__ sys) {
+    BATCH_SITE_PROLOGUE(kIfRunning)
+    if (i >= sh) return;
+    const float g = (float)S.st->gammag;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const long k = c * sh + i;
+        const float2 pp = h2f(S.pph[k]), rr = h2f(S.rh[k]);
+        S.pph[k] = f2h(fmaf(pp.x, g, rr.x), fmaf(pp.y, g, rr.y));
+    }
+}
+
+// x += out_f / scale
+__global__ __launch_bounds__(kBlock) void kb_h_final(const BSys* __restrict__ sys) {
+    BATCH_SITE_PROLOGUE(kAlways)
+    if (i >= sh) return;
+    const double inv = 1.0 / S.st->scale;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const long k = c * sh + i;
+        const double2 xx = S.x[k];
+        const float2 ff = S.outf[k];
+        S.x[k] = make_double2(xx.x + inv * (double)ff.x, xx.y + inv * (double)ff.y);
+    }
+}
+
+// half copies of one gauge field: rows 0,1 of each link array and its phases as (cos, sin)
+struct HLinks {
+    const double2* u[8];
+    const double* ph[8];
+    h2* uh[8];
+    h2* phh[8];
+    int sizeh;
+};
+__global__ __launch_bounds__(kBlock) void kb_h_links(HLinks a) {
+    const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (gid >= 8L * a.sizeh) return;
+    const int l = (int)(gid / a.sizeh), i = (int)(gid - (long)l * a.sizeh);
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+        const double2 v = a.u[l][(long)p * a.sizeh + i];
+        a.uh[l][(long)p * a.sizeh + i] = f2h((float)v.x, (float)v.y);
+    }
+    double sn, cs;
+    phase_sincos(a.ph[l][i], &sn, &cs);
+    a.phh[l][i] = f2h((float)cs, (float)sn);
+}
+
 // one block per system: sum its partials in tile order, apply finalizer F
 template <int F>
 __global__ __launch_bounds__(kBlock) void kb_finalize(const BSys* __restrict__ sys, int pred, int tile_size) {
@@ -331,7 +543,7 @@ static std::string signature(const std::vector<BSys>& v) {
 
 // Time each candidate tile on the batch's FP32 D_oe (the kernel of every
 // inner iteration), three launches each after one warm-up launch.
-static Tile autotune(const std::string& sig, int nsys, int max_sizeh) {
+static Tile autotune(const std::string& sig, int nsys, int max_sizeh, bool half) {
     auto it = g_tuned.find(sig);
     if (it != g_tuned.end()) return it->second;
     const Tile cands[4] = {{256, 1, 0.f}, {256, 0, 0.f}, {128, 1, 0.f}, {128, 0, 0.f}};
@@ -342,9 +554,13 @@ static Tile autotune(const std::string& sig, int nsys, int max_sizeh) {
     if (!hip_ok(hipEventCreate(&e0), "event") || !hip_ok(hipEventCreate(&e1), "event")) return Tile{};
     for (const Tile& c0 : cands) {
         Tile c = c0;
-        launch_role<kRoleF32Doe>(c, nsys, max_sizeh, kAlways);
+        auto one = [&]() {
+            if (half) launch_role<kRoleH16Doe>(c, nsys, max_sizeh, kAlways);
+            else launch_role<kRoleF32Doe>(c, nsys, max_sizeh, kAlways);
+        };
+        one();
         (void)hipEventRecord(e0, S());
-        for (int r = 0; r < 3; r++) launch_role<kRoleF32Doe>(c, nsys, max_sizeh, kAlways);
+        for (int r = 0; r < 3; r++) one();
         (void)hipEventRecord(e1, S());
         (void)hipEventSynchronize(e1);
         float ms = 0.f;
@@ -370,11 +586,11 @@ static Tile autotune(const std::string& sig, int nsys, int max_sizeh) {
 
 using namespace ostp;
 
-extern "C" {
+namespace ostp {
 
-int ostp_batched_mixed_solve(int nsys, const ostp_batch_system* sys, double res, int max_cg, double mixed_delta,
-                             int iters_per_graph, int* cg_return, int* ru_return, double* true_res) {
-    const char* nm = "ostp_batched_mixed_solve";
+static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int max_cg, double mixed_delta,
+                         int iters_per_graph, int* cg_return, int* ru_return, double* true_res, bool half) {
+    const char* nm = half ? "ostp_batched_half_solve" : "ostp_batched_mixed_solve";
     if (!require_init(nm)) return 0;
     if (nsys <= 0 || !sys || iters_per_graph <= 0) {
         fail("%s: invalid arguments", nm);
@@ -400,6 +616,7 @@ int ostp_batched_mixed_solve(int nsys, const ostp_batch_system* sys, double res,
         max_sizeh = std::max(max_sizeh, (int)sizeh[k]);
         max_t64 = std::max(max_t64, (int)((sizeh[k] + 63) / 64));
         arena += (size_t)(3 * 48 + 5 * 24) * sizeh[k] + 256 * 8;  // vectors (+ alignment slack)
+        if (half) arena += (size_t)(4 * 12 + 8 * 28) * sizeh[k] + 256 * 20;  // half vectors + a half link copy
     }
     const size_t part_per = (size_t)2 * max_t64;
     arena += (sizeof(double) * part_per + 256) * nsys;
@@ -469,6 +686,42 @@ int ostp_batched_mixed_solve(int nsys, const ostp_batch_system* sys, double res,
         b.partials = nullptr;
     }
     for (int k = 0; k < nsys; k++) hs[k].partials = (double*)take(sizeof(double) * part_per);
+    if (half) {
+        // half vectors, and one half copy of each distinct gauge field (systems sharing links share it)
+        std::map<std::pair<const void*, const void*>, int> seen;
+        for (int k = 0; k < nsys; k++) {
+            BSys& b = hs[k];
+            const long sh = sizeh[k];
+            b.rh = (h2*)take(12 * sh);
+            b.hh = (h2*)take(12 * sh);
+            b.sh = (h2*)take(12 * sh);
+            b.pph = (h2*)take(12 * sh);
+            const auto key = std::make_pair((const void*)b.u64[0], (const void*)b.ph64[0]);
+            auto it = seen.find(key);
+            if (it != seen.end()) {
+                for (int l = 0; l < 8; l++) {
+                    b.uh[l] = hs[it->second].uh[l];
+                    b.phh[l] = hs[it->second].phh[l];
+                }
+                continue;
+            }
+            seen[key] = k;
+            HLinks a;
+            h2* ub = (h2*)take((size_t)8 * 6 * 4 * sh);
+            h2* pb = (h2*)take((size_t)8 * 4 * sh);
+            for (int l = 0; l < 8; l++) {
+                a.u[l] = b.u64[l];
+                a.ph[l] = b.ph64[l];
+                a.uh[l] = ub + (size_t)l * 6 * sh;
+                a.phh[l] = pb + (size_t)l * sh;
+                b.uh[l] = a.uh[l];
+                b.phh[l] = a.phh[l];
+            }
+            a.sizeh = (int)sh;
+            hipLaunchKernelGGL(kb_h_links, dim3(blocks_for(8 * sh)), dim3(kBlock), 0, S(), a);
+        }
+        if (!hip_ok(hipGetLastError(), "half links")) return 0;
+    }
     if (!hip_ok(hipMemcpyAsync(B.dsys, hs.data(), sizeof(BSys) * nsys, hipMemcpyHostToDevice, S()), "descriptors"))
         return 0;
     const int nb_site = (max_sizeh + kBlock - 1) / kBlock;
@@ -487,24 +740,46 @@ int ostp_batched_mixed_solve(int nsys, const ostp_batch_system* sys, double res,
         s.max_cg = max_cg;
         s.mixed_delta = mixed_delta;
         s.source_norm = sn;
+        s.nentries = 3.0 * (double)sizeh[k];
         s.status = kRunning;
     }
     if (!push_states(nsys)) return 0;
-    const std::string sig = signature(hs);
-    const Tile t = autotune(sig, nsys, max_sizeh);
+    const std::string sig = signature(hs) + (half ? " fp16" : " fp32");
+    const Tile t = autotune(sig, nsys, max_sizeh, half);
 
-    // r = b - M x (FP64), r_f = p_f = (float) r, out_f = 0, delta = |r_f|^2
+    // r = b - M x (FP64); inner r = p = r, out = 0, delta = |r_inner|^2
     launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kAlways);
     launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kAlways);
-    hipLaunchKernelGGL(kb_mx_init, gsite, dim3(kBlock), 0, S(), B.dsys);
-    fin<kFinCgRestart>(nsys, kAlways, kBlock);
+    if (half) {
+        hipLaunchKernelGGL(kb_h_resid64, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kAlways);
+        fin<kFinHScale>(nsys, kAlways, kBlock);
+        hipLaunchKernelGGL(kb_h_rescale, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kAlways, 1);
+        fin<kFinHInit>(nsys, kAlways, kBlock);
+    } else {
+        hipLaunchKernelGGL(kb_mx_init, gsite, dim3(kBlock), 0, S(), B.dsys);
+        fin<kFinCgRestart>(nsys, kAlways, kBlock);
+    }
     if (!hip_ok(hipGetLastError(), nm)) return 0;
 
     // ---- capture iters_per_graph iterations
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (!hip_ok(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal), "begin capture")) return 0;
-    for (int j = 0; j < iters_per_graph; j++) {
+    for (int j = 0; j < iters_per_graph && half; j++) {
+        launch_role<kRoleH16Doe>(t, nsys, max_sizeh, kIfRunning);
+        launch_role<kRoleH16DeoDot>(t, nsys, max_sizeh, kIfRunning);
+        fin<kFinHAlpha>(nsys, kIfRunning, t.blk);
+        hipLaunchKernelGGL(kb_h_xr, gsite, dim3(kBlock), 0, S(), B.dsys);
+        fin<kFinHLambda>(nsys, kIfRunningNoRU, kBlock);
+        launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kIfRunningRU);
+        launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kIfRunningRU);
+        hipLaunchKernelGGL(kb_h_resid64, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kIfRunningRU);
+        fin<kFinHScale>(nsys, kIfRunningRU, kBlock);
+        hipLaunchKernelGGL(kb_h_rescale, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kIfRunningRU, 0);
+        fin<kFinHLambda>(nsys, kIfRunningRU, kBlock);
+        hipLaunchKernelGGL(kb_h_p, gsite, dim3(kBlock), 0, S(), B.dsys);
+    }
+    for (int j = 0; j < iters_per_graph && !half; j++) {
         launch_role<kRoleF32Doe>(t, nsys, max_sizeh, kIfRunning);
         launch_role<kRoleF32DeoDot>(t, nsys, max_sizeh, kIfRunning);
         fin<kFinMxAlpha>(nsys, kIfRunning, t.blk);
@@ -545,7 +820,10 @@ int ostp_batched_mixed_solve(int nsys, const ostp_batch_system* sys, double res,
     if (!ok) return 0;
 
     // ---- x += out_f; FP64 true residual of every system
-    hipLaunchKernelGGL(kb_add_f_to_d, gsite, dim3(kBlock), 0, S(), B.dsys);
+    if (half)
+        hipLaunchKernelGGL(kb_h_final, gsite, dim3(kBlock), 0, S(), B.dsys);
+    else
+        hipLaunchKernelGGL(kb_add_f_to_d, gsite, dim3(kBlock), 0, S(), B.dsys);
     launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kAlways);
     launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kAlways);
     hipLaunchKernelGGL(kb_resid_norm, gsite, dim3(kBlock), 0, S(), B.dsys, 1);
@@ -563,6 +841,20 @@ int ostp_batched_mixed_solve(int nsys, const ostp_batch_system* sys, double res,
             printf("WARNING: batched system %d: true residual %e above %e after %d iterations\n", k, tr, res, s.cg);
     }
     return converged;
+}
+
+}  // namespace ostp
+
+extern "C" {
+
+int ostp_batched_mixed_solve(int nsys, const ostp_batch_system* sys, double res, int max_cg, double mixed_delta,
+                             int iters_per_graph, int* cg_return, int* ru_return, double* true_res) {
+    return batched_solve(nsys, sys, res, max_cg, mixed_delta, iters_per_graph, cg_return, ru_return, true_res, false);
+}
+
+int ostp_batched_half_solve(int nsys, const ostp_batch_system* sys, double res, int max_cg, double mixed_delta,
+                            int iters_per_graph, int* cg_return, int* ru_return, double* true_res) {
+    return batched_solve(nsys, sys, res, max_cg, mixed_delta, iters_per_graph, cg_return, ru_return, true_res, true);
 }
 
 int ostp_autotune_report(char* buf, int len) {

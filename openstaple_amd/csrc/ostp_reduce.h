@@ -31,6 +31,12 @@ enum Fin : int {
     kFinFcgRes,         // flexible CG: |r|^2, counters, exit test
     kFinFcgBeta,        // flexible CG: beta = -alpha <z,q> / <z,r>_old ; <z,r> (two values)
     kFinFcgInit,        // flexible CG start: <z,r> (second value), running
+    // FP16-storage inner CG with FP64 reliable updates; inner vectors carry a
+    // scale so that their entries stay O(1) (rescaled at every reliable update)
+    kFinHScale,         // scale = sqrt(n / |r|^2) from the FP64 residual, rho = new / old
+    kFinHInit,          // delta = |r_h|^2, running
+    kFinHAlpha,         // as kFinMxAlpha, rho = 1
+    kFinHLambda,        // as kFinMxLambda with delta rescaled by rho^2, exit test in true units
 };
 
 // Predicates evaluated by every block at kernel entry (uniform).
@@ -141,6 +147,34 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
     } else if (F == kFinFcgInit) {
         st->delta = tot_im;
         st->status = kRunning;
+    } else if (F == kFinHScale) {
+        const double n = st->nentries;
+        const double sc = tot > 0.0 ? sqrt(n / tot) : 1.0;
+        st->rho = st->scale > 0.0 ? sc / st->scale : 1.0;
+        st->scale = sc;
+    } else if (F == kFinHInit) {
+        st->delta = tot;
+        st->rho = 1.0;
+        st->status = kRunning;
+    } else if (F == kFinHAlpha) {
+        st->alpha = tot;
+        st->omega = st->delta / tot;
+        st->rho = 1.0;
+        if (st->last_max_res < st->delta) st->last_max_res = st->delta;
+        st->ru = (st->delta < st->mixed_delta * st->last_max_res) ? 1 : 0;
+    } else if (F == kFinHLambda) {
+        const double lambda = tot;
+        st->lambda = lambda;
+        st->gammag = lambda / (st->delta * st->rho * st->rho);
+        st->delta = lambda;
+        st->cg += 1;
+        if (st->ru) {
+            st->last_max_res = 0;
+            st->ru_count += 1;
+        }
+        const double true2 = lambda / (st->scale * st->scale);
+        const bool cont = (sqrt(true2 / st->source_norm) > st->res * st->safety) && st->cg < st->max_cg;
+        if (!cont) st->status = kDone;
     } else if (F == kFinMxAlpha) {
         st->alpha = tot;
         st->omega = st->delta / tot;

@@ -20,6 +20,8 @@ struct SolverState {
     double res, safety, last_max_res, mixed_delta, shift_m2;
     double sum;               // result slot of stand-alone reductions
     double sum_im;
+    double scale, rho;        // FP16 inner solver: inner vectors = scale x true; rescale at the last update
+    double nentries;          // FP16 inner solver: complex entries of one vector (3 sizeh)
     int cg, cg_restarted, restart_every, max_cg;
     int status;               // SolverStatus
     int ru;                   // mixed precision: reliable update in this iteration

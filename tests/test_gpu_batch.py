@@ -120,3 +120,33 @@ def test_batched_max_cg_reports_unconverged():
     conv = lib.ostp_batched_mixed_solve(1, arr, 1e-12, 20, 0.1, 8, cg, None, tr)
     ost.check(0, "batched max_cg")
     assert conv == 0 and cg[0] == 20 and tr[0] > 1e-12
+
+
+def test_batched_half_precision_inner():
+    """FP16 storage in the inner solver: every system still reaches its FP64
+    true residual (reliable updates), within 1.5x the FP32-inner iterations."""
+    init_lattice((8, 8, 8, 8))
+    main = Field((8, 8, 8, 8), False, "expH", 1)
+    reps = [Field((4, 6, 8, 10), True, "cold", s) for s in (2, 3)]
+    cases = [(main, main.o.gen_z2(11, k), 0.05) for k in range(3)]
+    cases += [(f, f.o.gen_z2(12, k), 0.1 + 0.05 * k) for k, f in enumerate(reps)]
+    n = len(cases)
+    keep, arr = [], (ost.BatchSystem * n)()
+    for k, (f, b, m) in enumerate(cases):
+        s, db, dx = system(f, b, m)
+        arr[k] = s
+        keep.append((db, dx))
+    cg32, tr32 = (C.c_int * n)(), (C.c_double * n)()
+    assert lib.ostp_batched_mixed_solve(n, arr, RES, 10000, 0.1, 8, cg32, None, tr32) == n
+    for k in range(n):
+        keep[k][1].put(np.zeros_like(cases[k][1]))
+    cg, ru, tr = (C.c_int * n)(), (C.c_int * n)(), (C.c_double * n)()
+    conv = lib.ostp_batched_half_solve(n, arr, RES, 10000, 0.1, 8, cg, ru, tr)
+    ost.check(0, "batched half solve")
+    print("\nFP16-inner iterations", list(cg), "FP32-inner", list(cg32), "reliable updates", list(ru))
+    assert conv == n, list(tr)
+    for k, (f, b, m) in enumerate(cases):
+        x = keep[k][1].get(b)
+        r = b - f.o.mdagm(f.u, x, f.ph, m)
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= RES, k
+        assert cg[k] <= 1.5 * cg32[k], (k, cg[k], cg32[k])
