@@ -305,7 +305,9 @@ typedef struct ostp_batch_system_t {
 /* Solve all systems together: every kernel is batched over systems, each
  * system keeps its scalars on the device, and iters_per_graph iterations are
  * captured once into a HIP graph that is replayed until all systems are
- * done.  The Dslash tile is autotuned per batch signature on first use.
+ * done (iters_per_graph = 0: the same kernels launched directly, for
+ * profilers and debuggers that do not follow stream capture).  The Dslash
+ * tile is autotuned per batch signature on first use.
  * cg_return[k], ru_return[k] (reliable updates) and true_res[k] (FP64
  * ||b - M x|| / ||b||) may be NULL.  Returns the number of systems whose true
  * residual is <= res (the ostp_init lattice is not used by this call, but

@@ -70,6 +70,7 @@ enum BRole : int {
     kRoleF32DeoDot,    // sf = m2f pf - D_eo hf, partial <pf,sf> (FP32)
     kRoleF64Doe,       // h64 = D_oe x                          (FP64)
     kRoleF64Deo,       // s64 = m2 x - D_eo h64                 (FP64)
+    kRoleF64DeoResid,  // s64 = m2 x - D_eo h64, r64 = b - s64, partial |r64|^2 (FP64)
     kRoleH16Doe,       // hh = D_oe pph                         (FP16 storage)
     kRoleH16DeoDot,    // sh = m2f pph - D_eo hh, partial <pph,sh> (FP16 storage)
 };
@@ -84,6 +85,44 @@ __device__ __forceinline__ void hop_h(const h2* __restrict__ U, const h2* __rest
     const float2 cs = h2f(P[iu]);
     const float2 ph = DAG ? make_float2(cs.x, -cs.y) : cs;
     const float2 v0 = cmul(h2f(in[iv]), ph), v1 = cmul(h2f(in[sizeh + iv]), ph), v2 = cmul(h2f(in[2 * sizeh + iv]), ph);
+    const float2 x0 = csub(cmul(a1, b2), cmul(a2, b1));
+    const float2 x1 = csub(cmul(a2, b0), cmul(a0, b2));
+    const float2 x2 = csub(cmul(a0, b1), cmul(a1, b0));
+    if (!DAG) {
+        float2 r0 = cmul(a0, v0); r0 = cfma(a1, v1, r0); r0 = cfma(a2, v2, r0);
+        float2 r1 = cmul(b0, v0); r1 = cfma(b1, v1, r1); r1 = cfma(b2, v2, r1);
+        float2 r2 = cmul(cconj(x0), v0); r2 = cfmac(x1, v1, r2); r2 = cfmac(x2, v2, r2);
+        o0 = cadd(o0, r0); o1 = cadd(o1, r1); o2 = cadd(o2, r2);
+    } else {
+        float2 r0 = cmul(cconj(a0), v0); r0 = cfmac(b0, v1, r0); r0 = cfma(x0, v2, r0);
+        float2 r1 = cmul(cconj(a1), v0); r1 = cfmac(b1, v1, r1); r1 = cfma(x1, v2, r1);
+        float2 r2 = cmul(cconj(a2), v0); r2 = cfmac(b2, v1, r2); r2 = cfma(x2, v2, r2);
+        o0 = csub(o0, r0); o1 = csub(o1, r1); o2 = csub(o2, r2);
+    }
+}
+
+// p' = (half)(p gamma + r): the search-direction update of the previous
+// iteration, formed where it is read (identical rounding everywhere)
+__device__ __forceinline__ h2 pnext(const h2* __restrict__ p, const h2* __restrict__ r, long k, float g) {
+    const float2 pp = h2f(p[k]), rr = h2f(r[k]);
+    return f2h(fmaf(pp.x, g, rr.x), fmaf(pp.y, g, rr.y));
+}
+
+// hop_h whose input is p' at the neighbour
+template <bool DAG>
+__device__ __forceinline__ void hop_hp(const h2* __restrict__ U, const h2* __restrict__ P, const h2* __restrict__ pin,
+                                       const h2* __restrict__ rin, float g, int sizeh, int iu, int iv, float2& o0,
+                                       float2& o1, float2& o2) {
+    h2 v[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[c] = pnext(pin, rin, (long)c * sizeh + iv, g);
+    // reuse hop_h on a 3-entry local vector laid out with stride 1
+    const h2 loc[3] = {v[0], v[1], v[2]};
+    const float2 a0 = h2f(U[iu]), a1 = h2f(U[sizeh + iu]), a2 = h2f(U[2 * sizeh + iu]);
+    const float2 b0 = h2f(U[3 * sizeh + iu]), b1 = h2f(U[4 * sizeh + iu]), b2 = h2f(U[5 * sizeh + iu]);
+    const float2 cs = h2f(P[iu]);
+    const float2 ph = DAG ? make_float2(cs.x, -cs.y) : cs;
+    const float2 v0 = cmul(h2f(loc[0]), ph), v1 = cmul(h2f(loc[1]), ph), v2 = cmul(h2f(loc[2]), ph);
     const float2 x0 = csub(cmul(a1, b2), cmul(a2, b1));
     const float2 x1 = csub(cmul(a2, b0), cmul(a0, b2));
     const float2 x2 = csub(cmul(a0, b1), cmul(a1, b0));
@@ -152,14 +191,31 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
         if (idx < g.sizeh) {
             float2 o0, o1, o2;
             if constexpr (EOH == 1) {
-                dslash_site_h<1>(S.uh, S.phh, S.pph, g, idx, o0, o1, o2);
+                // D_oe of p' = p gammag + r (the previous iteration's direction update)
+                const float gm = (float)S.st->gammag;
+                const Nbrs nb = site_neighbours<1>(g, idx);
+                o0 = o1 = o2 = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int mu = 0; mu < 4; mu++)
+                    hop_hp<true>(S.uh[2 * mu], S.phh[2 * mu], S.pph, S.rh, gm, g.sizeh, nb.im[mu], nb.im[mu], o0, o1, o2);
+#pragma unroll
+                for (int mu = 0; mu < 4; mu++)
+                    hop_hp<false>(S.uh[2 * mu + 1], S.phh[2 * mu + 1], S.pph, S.rh, gm, g.sizeh, idx, nb.ip[mu], o0, o1, o2);
+                o0.x *= 0.5f; o0.y *= 0.5f; o1.x *= 0.5f; o1.y *= 0.5f; o2.x *= 0.5f; o2.y *= 0.5f;
                 S.hh[idx] = f2h(o0.x, o0.y);
                 S.hh[g.sizeh + idx] = f2h(o1.x, o1.y);
                 S.hh[2 * g.sizeh + idx] = f2h(o2.x, o2.y);
             } else {
                 dslash_site_h<0>(S.uh, S.phh, S.hh, g, idx, o0, o1, o2);
                 const float m2 = (float)S.m2f;
-                const float2 x0 = h2f(S.pph[idx]), x1 = h2f(S.pph[g.sizeh + idx]), x2 = h2f(S.pph[2 * g.sizeh + idx]);
+                // p' at this site, stored: the direction of this iteration
+                const float gm = (float)S.st->gammag;
+                const h2 q0 = pnext(S.pph, S.rh, idx, gm), q1 = pnext(S.pph, S.rh, g.sizeh + idx, gm),
+                         q2 = pnext(S.pph, S.rh, 2L * g.sizeh + idx, gm);
+                S.pph[idx] = q0;
+                S.pph[g.sizeh + idx] = q1;
+                S.pph[2 * g.sizeh + idx] = q2;
+                const float2 x0 = h2f(q0), x1 = h2f(q1), x2 = h2f(q2);
                 const h2 s0 = f2h(x0.x * m2 - o0.x, x0.y * m2 - o0.y);
                 const h2 s1 = f2h(x1.x * m2 - o1.x, x1.y * m2 - o1.y);
                 const h2 s2 = f2h(x2.x * m2 - o2.x, x2.y * m2 - o2.y);
@@ -186,7 +242,7 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
     } else if constexpr (ROLE == kRoleF64Doe) {
         u = S.u64; ph = S.ph64; in = S.x; out = S.h64;
     } else {
-        u = S.u64; ph = S.ph64; in = S.h64; out = S.s64; x = S.x; m2 = S.m2;
+        u = S.u64; ph = S.ph64; in = S.h64; out = S.s64; x = S.x; m2 = S.m2;  // F64Deo, F64DeoResid
     }
     double dot = 0.0;
     if (idx < g.sizeh) {
@@ -205,9 +261,20 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
             out[g.sizeh + idx] = s1;
             out[2 * g.sizeh + idx] = s2;
             if constexpr (ROLE == kRoleF32DeoDot) dot = re_dot(x0, s0) + re_dot(x1, s1) + re_dot(x2, s2);
+            if constexpr (ROLE == kRoleF64DeoResid) {
+                const C sv[3] = {s0, s1, s2};
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const long k = (long)c * g.sizeh + idx;
+                    const double2 bb = S.b[k];
+                    const double2 rr = make_double2(bb.x - (double)sv[c].x, bb.y - (double)sv[c].y);
+                    S.r64[k] = rr;
+                    dot += rr.x * rr.x + rr.y * rr.y;
+                }
+            }
         }
     }
-    if constexpr (ROLE == kRoleF32DeoDot) block_partial_at<BLK, 1>(dot, 0.0, S.partials, tile);
+    if constexpr (ROLE == kRoleF32DeoDot || ROLE == kRoleF64DeoResid) block_partial_at<BLK, 1>(dot, 0.0, S.partials, tile);
     }
 }
 
@@ -340,23 +407,6 @@ __global__ __launch_bounds__(kBlock) void kb_resid_norm(const BSys* __restrict__
 // Inner vectors hold scale x (true vector); the scale makes the residual's
 // entries O(1) after every reliable update (see kFinHScale).
 
-// r64 = b - s64 (FP64), partial |r64|^2
-__global__ __launch_bounds__(kBlock) void kb_h_resid64(const BSys* __restrict__ sys, int pred) {
-    BATCH_SITE_PROLOGUE(pred)
-    double v = 0.0;
-    if (i < sh) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const long k = c * sh + i;
-            const double2 bb = S.b[k], ss = S.s64[k];
-            const double2 rr = make_double2(bb.x - ss.x, bb.y - ss.y);
-            S.r64[k] = rr;
-            v += rr.x * rr.x + rr.y * rr.y;
-        }
-    }
-    block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
-}
-
 // r_h = (half) scale r64; start: p_h = r_h, out_f = 0; otherwise p_h *= rho.
 // Partial |r_h|^2 of the stored (rounded) values.
 __global__ __launch_bounds__(kBlock) void kb_h_rescale(const BSys* __restrict__ sys, int pred, int start) {
@@ -385,53 +435,39 @@ __global__ __launch_bounds__(kBlock) void kb_h_rescale(const BSys* __restrict__ 
     block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
 }
 
-// out_f += omega p_h; then r_h -= omega s_h with partial |r_h|^2, or (reliable
-// update due) x += out_f / scale, out_f = 0
-__global__ __launch_bounds__(kBlock) void kb_h_xr(const BSys* __restrict__ sys) {
-    BATCH_SITE_PROLOGUE(kIfRunning)
-    const float om = (float)S.st->omega;
-    const int ru = S.st->ru;
-    const double inv = 1.0 / S.st->scale;
+// alpha from the D_eo partials <p', s> (even slots, summed by every block in
+// the same order, so every block computes the same omega; block 0 commits the
+// scalars), then out_f += omega p_h, r_h -= omega s_h with partial |r_h|^2 in
+// the odd slots (the even ones are still being read by other blocks)
+__global__ __launch_bounds__(kBlock) void kb_h_xr(const BSys* __restrict__ sys, int dot_tile) {
+    BATCH_SITE_PROLOGUE(kIfGoing)
+    __shared__ double lds[kBlock / 64];
+    const int nparts = (int)((sh + dot_tile - 1) / dot_tile);
+    double a = 0.0;
+    for (int j = threadIdx.x; j < nparts; j += kBlock) a += S.partials[2 * j];
+    const double tot = block_sum_all<kBlock>(a, lds);
+    const float om = (float)(S.st->delta / tot);
+    if (blockIdx.x == 0 && threadIdx.x == 0) finalize<kFinHAlpha>(S.st, tot, 0.0);
     double v = 0.0;
     if (i < sh) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const long k = c * sh + i;
             const float2 pp = h2f(S.pph[k]), oo = S.outf[k];
-            const float2 on = make_float2(fmaf(pp.x, om, oo.x), fmaf(pp.y, om, oo.y));
-            if (!ru) {
-                S.outf[k] = on;
-                const float2 ss = h2f(S.sh[k]), rr = h2f(S.rh[k]);
-                const h2 rn = f2h(fmaf(ss.x, -om, rr.x), fmaf(ss.y, -om, rr.y));
-                S.rh[k] = rn;
-                const float2 rf = h2f(rn);
-                v += (double)rf.x * rf.x + (double)rf.y * rf.y;
-            } else {
-                const double2 xx = S.x[k];
-                S.x[k] = make_double2(xx.x + inv * (double)on.x, xx.y + inv * (double)on.y);
-                S.outf[k] = make_float2(0.f, 0.f);
-            }
+            S.outf[k] = make_float2(fmaf(pp.x, om, oo.x), fmaf(pp.y, om, oo.y));
+            const float2 ss = h2f(S.sh[k]), rr = h2f(S.rh[k]);
+            const h2 rn = f2h(fmaf(ss.x, -om, rr.x), fmaf(ss.y, -om, rr.y));
+            S.rh[k] = rn;
+            const float2 rf = h2f(rn);
+            v += (double)rf.x * rf.x + (double)rf.y * rf.y;
         }
     }
-    if (!ru) block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
+    block_partial_at<kBlock, 1>(v, 0.0, S.partials + 1, blockIdx.x);
 }
 
-// p_h = p_h gammag + r_h
-__global__ __launch_bounds__(kBlock) void kb_h_p(const BSys* __restrict__ sys) {
-    BATCH_SITE_PROLOGUE(kIfRunning)
-    if (i >= sh) return;
-    const float g = (float)S.st->gammag;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const long k = c * sh + i;
-        const float2 pp = h2f(S.pph[k]), rr = h2f(S.rh[k]);
-        S.pph[k] = f2h(fmaf(pp.x, g, rr.x), fmaf(pp.y, g, rr.y));
-    }
-}
-
-// x += out_f / scale
-__global__ __launch_bounds__(kBlock) void kb_h_final(const BSys* __restrict__ sys) {
-    BATCH_SITE_PROLOGUE(kAlways)
+// x += out_f / scale, out_f = 0
+__global__ __launch_bounds__(kBlock) void kb_h_flush(const BSys* __restrict__ sys, int pred) {
+    BATCH_SITE_PROLOGUE(pred)
     if (i >= sh) return;
     const double inv = 1.0 / S.st->scale;
 #pragma unroll
@@ -440,6 +476,7 @@ __global__ __launch_bounds__(kBlock) void kb_h_final(const BSys* __restrict__ sy
         const double2 xx = S.x[k];
         const float2 ff = S.outf[k];
         S.x[k] = make_double2(xx.x + inv * (double)ff.x, xx.y + inv * (double)ff.y);
+        S.outf[k] = make_float2(0.f, 0.f);
     }
 }
 
@@ -467,14 +504,17 @@ __global__ __launch_bounds__(kBlock) void kb_h_links(HLinks a) {
 
 // one block per system: sum its partials in tile order, apply finalizer F
 template <int F>
-__global__ __launch_bounds__(kBlock) void kb_finalize(const BSys* __restrict__ sys, int pred, int tile_size) {
+__global__ __launch_bounds__(kBlock) void kb_finalize(const BSys* __restrict__ sys, int pred, int tile_size, int odd) {
     const BSys& S = sys[blockIdx.x];
     const SolverCtl c = bctl(S, pred);
-    if (!pred_pass(c)) return;
     const int nparts = (S.g.sizeh + tile_size - 1) / tile_size;
-    __shared__ double lds[kBlock / 64];
+    // the partial loads are issued before the predicate's state load (both
+    // in bounds), so the two memory latencies overlap
     double a = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += kBlock) a += S.partials[2 * i];
+#pragma unroll 4
+    for (int i = threadIdx.x; i < nparts; i += kBlock) a += S.partials[2 * i + odd];
+    if (!pred_pass(c)) return;
+    __shared__ double lds[kBlock / 64];
     const double t = block_sum<kBlock>(a, lds);
     if (threadIdx.x == 0) finalize<F>(S.st, t, 0.0);
 }
@@ -513,9 +553,10 @@ static void launch_role(const Tile& t, int nsys, int max_sizeh, int pred) {
         hipLaunchKernelGGL((kb_dslash<ROLE, 256>), dim3(tiles, nsys), dim3(256), 0, S(), g_batch.dsys, pred, t.tmajor);
 }
 
+// odd = 1: partials stored in the odd slots (kb_h_xr)
 template <int F>
-static void fin(int nsys, int pred, int tile_size) {
-    hipLaunchKernelGGL((kb_finalize<F>), dim3(nsys), dim3(kBlock), 0, S(), g_batch.dsys, pred, tile_size);
+static void fin(int nsys, int pred, int tile_size, int odd = 0) {
+    hipLaunchKernelGGL((kb_finalize<F>), dim3(nsys), dim3(kBlock), 0, S(), g_batch.dsys, pred, tile_size, odd);
 }
 
 static bool pull_states(int n) {
@@ -592,7 +633,7 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
                          int iters_per_graph, int* cg_return, int* ru_return, double* true_res, bool half) {
     const char* nm = half ? "ostp_batched_half_solve" : "ostp_batched_mixed_solve";
     if (!require_init(nm)) return 0;
-    if (nsys <= 0 || !sys || iters_per_graph <= 0) {
+    if (nsys <= 0 || !sys || iters_per_graph < 0) {
         fail("%s: invalid arguments", nm);
         return 0;
     }
@@ -749,53 +790,66 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
 
     // r = b - M x (FP64); inner r = p = r, out = 0, delta = |r_inner|^2
     launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kAlways);
-    launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kAlways);
     if (half) {
-        hipLaunchKernelGGL(kb_h_resid64, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kAlways);
-        fin<kFinHScale>(nsys, kAlways, kBlock);
+        launch_role<kRoleF64DeoResid>(t, nsys, max_sizeh, kAlways);
+        fin<kFinHScale>(nsys, kAlways, t.blk);
         hipLaunchKernelGGL(kb_h_rescale, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kAlways, 1);
         fin<kFinHInit>(nsys, kAlways, kBlock);
     } else {
+        launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kAlways);
         hipLaunchKernelGGL(kb_mx_init, gsite, dim3(kBlock), 0, S(), B.dsys);
         fin<kFinCgRestart>(nsys, kAlways, kBlock);
     }
     if (!hip_ok(hipGetLastError(), nm)) return 0;
 
-    // ---- capture iters_per_graph iterations
+    // ---- one inner iteration of every system (predicated on each system's state)
+    auto enqueue_iteration = [&]() {
+        if (half) {
+            launch_role<kRoleH16Doe>(t, nsys, max_sizeh, kIfGoing);
+            launch_role<kRoleH16DeoDot>(t, nsys, max_sizeh, kIfGoing);
+            hipLaunchKernelGGL(kb_h_xr, gsite, dim3(kBlock), 0, S(), B.dsys, t.blk);
+            fin<kFinHLambda>(nsys, kIfGoing, kBlock, 1);
+            // p = p gammag + r happens inside the next iteration's two hops
+        } else {
+            launch_role<kRoleF32Doe>(t, nsys, max_sizeh, kIfRunning);
+            launch_role<kRoleF32DeoDot>(t, nsys, max_sizeh, kIfRunning);
+            fin<kFinMxAlpha>(nsys, kIfRunning, t.blk);
+            hipLaunchKernelGGL(kb_mx_xr, gsite, dim3(kBlock), 0, S(), B.dsys);
+            fin<kFinMxLambda>(nsys, kIfRunningNoRU, kBlock);
+            launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kIfRunningRU);
+            launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kIfRunningRU);
+            hipLaunchKernelGGL(kb_mx_resid, gsite, dim3(kBlock), 0, S(), B.dsys);
+            fin<kFinMxLambda>(nsys, kIfRunningRU, kBlock);
+            hipLaunchKernelGGL(kb_cg_p, gsite, dim3(kBlock), 0, S(), B.dsys);
+        }
+    };
+    // ---- iters_per_graph iterations captured once into a HIP graph (0: the same
+    // sequence launched directly, 16 iterations per round, for tools that do not
+    // follow stream capture)
+    // FP16 inner solver: the reliable update (due, or asked for by the inner
+    // exit test) runs once after each block of queued iterations; the
+    // iterations in between continue the inner recurrence
+    auto enqueue_reliable_update = [&]() {
+        if (!half) return;
+        hipLaunchKernelGGL(kb_h_flush, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kIfRunningRU);
+        launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kIfRunningRU);
+        launch_role<kRoleF64DeoResid>(t, nsys, max_sizeh, kIfRunningRU);
+        fin<kFinHScale>(nsys, kIfRunningRU, t.blk);
+        hipLaunchKernelGGL(kb_h_rescale, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kIfRunningRU, 0);
+        fin<kFinHRu>(nsys, kIfRunningRU, kBlock);
+    };
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    if (!hip_ok(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal), "begin capture")) return 0;
-    for (int j = 0; j < iters_per_graph && half; j++) {
-        launch_role<kRoleH16Doe>(t, nsys, max_sizeh, kIfRunning);
-        launch_role<kRoleH16DeoDot>(t, nsys, max_sizeh, kIfRunning);
-        fin<kFinHAlpha>(nsys, kIfRunning, t.blk);
-        hipLaunchKernelGGL(kb_h_xr, gsite, dim3(kBlock), 0, S(), B.dsys);
-        fin<kFinHLambda>(nsys, kIfRunningNoRU, kBlock);
-        launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kIfRunningRU);
-        launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kIfRunningRU);
-        hipLaunchKernelGGL(kb_h_resid64, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kIfRunningRU);
-        fin<kFinHScale>(nsys, kIfRunningRU, kBlock);
-        hipLaunchKernelGGL(kb_h_rescale, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kIfRunningRU, 0);
-        fin<kFinHLambda>(nsys, kIfRunningRU, kBlock);
-        hipLaunchKernelGGL(kb_h_p, gsite, dim3(kBlock), 0, S(), B.dsys);
-    }
-    for (int j = 0; j < iters_per_graph && !half; j++) {
-        launch_role<kRoleF32Doe>(t, nsys, max_sizeh, kIfRunning);
-        launch_role<kRoleF32DeoDot>(t, nsys, max_sizeh, kIfRunning);
-        fin<kFinMxAlpha>(nsys, kIfRunning, t.blk);
-        hipLaunchKernelGGL(kb_mx_xr, gsite, dim3(kBlock), 0, S(), B.dsys);
-        fin<kFinMxLambda>(nsys, kIfRunningNoRU, kBlock);
-        launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kIfRunningRU);
-        launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kIfRunningRU);
-        hipLaunchKernelGGL(kb_mx_resid, gsite, dim3(kBlock), 0, S(), B.dsys);
-        fin<kFinMxLambda>(nsys, kIfRunningRU, kBlock);
-        hipLaunchKernelGGL(kb_cg_p, gsite, dim3(kBlock), 0, S(), B.dsys);
-    }
-    const hipError_t ce = hipStreamEndCapture(S(), &graph);
-    if (!hip_ok(ce, "end capture")) return 0;
-    if (!hip_ok(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate")) {
-        (void)hipGraphDestroy(graph);
-        return 0;
+    if (iters_per_graph > 0) {
+        if (!hip_ok(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal), "begin capture")) return 0;
+        for (int j = 0; j < iters_per_graph; j++) enqueue_iteration();
+        enqueue_reliable_update();
+        const hipError_t ce = hipStreamEndCapture(S(), &graph);
+        if (!hip_ok(ce, "end capture")) return 0;
+        if (!hip_ok(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate")) {
+            (void)hipGraphDestroy(graph);
+            return 0;
+        }
     }
     bool ok = true;
     long last_total = -1;
@@ -813,15 +867,21 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
             break;
         }
         last_total = total;
-        if (!hip_ok(hipGraphLaunch(exec, S()), "graph launch")) { ok = false; break; }
+        if (exec) {
+            if (!hip_ok(hipGraphLaunch(exec, S()), "graph launch")) { ok = false; break; }
+        } else {
+            for (int j = 0; j < 16; j++) enqueue_iteration();
+            enqueue_reliable_update();
+            if (!hip_ok(hipGetLastError(), nm)) { ok = false; break; }
+        }
     }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (!ok) return 0;
 
     // ---- x += out_f; FP64 true residual of every system
     if (half)
-        hipLaunchKernelGGL(kb_h_final, gsite, dim3(kBlock), 0, S(), B.dsys);
+        hipLaunchKernelGGL(kb_h_flush, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kAlways);
     else
         hipLaunchKernelGGL(kb_add_f_to_d, gsite, dim3(kBlock), 0, S(), B.dsys);
     launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kAlways);

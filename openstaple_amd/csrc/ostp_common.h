@@ -98,4 +98,19 @@ __device__ __forceinline__ double block_sum(double v, double* lds /* >= NT/64 */
     return r;
 }
 
+// The same, with the total returned to every thread (each thread adds the
+// wave sums in the same order, so all threads hold identical values).
+template <int NT>
+__device__ __forceinline__ double block_sum_all(double v, double* lds /* >= NT/64 */) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) r += lds[w];
+    return r;
+}
+
 }  // namespace ostp

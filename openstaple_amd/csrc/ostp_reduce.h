@@ -35,8 +35,9 @@ enum Fin : int {
     // scale so that their entries stay O(1) (rescaled at every reliable update)
     kFinHScale,         // scale = sqrt(n / |r|^2) from the FP64 residual, rho = new / old
     kFinHInit,          // delta = |r_h|^2, running
-    kFinHAlpha,         // as kFinMxAlpha, rho = 1
-    kFinHLambda,        // as kFinMxLambda with delta rescaled by rho^2, exit test in true units
+    kFinHAlpha,         // as kFinMxAlpha, rho = 1; a due reliable update stays due until done
+    kFinHLambda,        // inner |r|^2, gammag, counters; inner exit test -> hold for a reliable update
+    kFinHRu,            // after a reliable update: gammag from the FP64 residual, exit test
 };
 
 // Predicates evaluated by every block at kernel entry (uniform).
@@ -46,6 +47,7 @@ enum Pred : int {
     kIfIter = 2,         // cg == iteration number this launch belongs to
     kIfRunningRU = 3,    // running and reliable update scheduled
     kIfRunningNoRU = 4,  // running and no reliable update scheduled
+    kIfGoing = 5,        // running and not holding for a reliable update
 };
 
 struct SolverCtl {
@@ -62,6 +64,7 @@ __device__ __forceinline__ bool pred_pass(const SolverCtl& c) {
     if (c.pred == kIfIter) return st->cg == c.iter;
     if (c.pred == kIfRunningRU) return st->status == kRunning && st->ru != 0;
     if (c.pred == kIfRunningNoRU) return st->status == kRunning && st->ru == 0;
+    if (c.pred == kIfGoing) return st->status == kRunning && st->hold == 0;
     return false;
 }
 
@@ -161,20 +164,37 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
         st->omega = st->delta / tot;
         st->rho = 1.0;
         if (st->last_max_res < st->delta) st->last_max_res = st->delta;
-        st->ru = (st->delta < st->mixed_delta * st->last_max_res) ? 1 : 0;
+        if (st->delta < st->mixed_delta * st->last_max_res) st->ru = 1;
     } else if (F == kFinHLambda) {
+        // the reliable update itself runs once per queued block of iterations
+        // (kFinHRu); until then the inner recurrence continues
         const double lambda = tot;
         st->lambda = lambda;
-        st->gammag = lambda / (st->delta * st->rho * st->rho);
+        st->delta_old = st->delta;
+        st->gammag = lambda / st->delta;
         st->delta = lambda;
         st->cg += 1;
-        if (st->ru) {
-            st->last_max_res = 0;
-            st->ru_count += 1;
-        }
         const double true2 = lambda / (st->scale * st->scale);
-        const bool cont = (sqrt(true2 / st->source_norm) > st->res * st->safety) && st->cg < st->max_cg;
-        if (!cont) st->status = kDone;
+        if (st->cg >= st->max_cg) {
+            st->status = kDone;
+        } else if (!(sqrt(true2 / st->source_norm) > st->res * st->safety)) {
+            st->ru = 1;    // inner residual converged: confirm with the FP64 residual
+            st->hold = 1;
+        }
+    } else if (F == kFinHRu) {
+        // r was replaced by the (rescaled) FP64 residual and p multiplied by
+        // rho: the last gammag uses the new |r|^2 over the previous |r|^2 in
+        // the new scale
+        const double lambda = tot;
+        st->lambda = lambda;
+        st->gammag = lambda / (st->delta_old * st->rho * st->rho);
+        st->delta = lambda;
+        st->ru = 0;
+        st->hold = 0;
+        st->last_max_res = 0;
+        st->ru_count += 1;
+        const double true2 = lambda / (st->scale * st->scale);
+        if (!(sqrt(true2 / st->source_norm) > st->res * st->safety)) st->status = kDone;
     } else if (F == kFinMxAlpha) {
         st->alpha = tot;
         st->omega = st->delta / tot;

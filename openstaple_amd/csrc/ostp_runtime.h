@@ -22,9 +22,11 @@ struct SolverState {
     double sum_im;
     double scale, rho;        // FP16 inner solver: inner vectors = scale x true; rescale at the last update
     double nentries;          // FP16 inner solver: complex entries of one vector (3 sizeh)
+    double delta_old;         // FP16 inner solver: |r|^2 before the last update
     int cg, cg_restarted, restart_every, max_cg;
     int status;               // SolverStatus
-    int ru;                   // mixed precision: reliable update in this iteration
+    int ru;                   // mixed precision: reliable update in this iteration (FP16: due)
+    int hold;                 // FP16 inner solver: inner iterations stop until the reliable update
     int ru_count;
     // multi-shift CG-M (inverter_multishift_full.c:48-53)
     int order, maxiter, maxiter_next, flag_cur;
