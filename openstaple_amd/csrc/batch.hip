@@ -44,6 +44,42 @@ __device__ __forceinline__ h2 f2h(float x, float y) {
     r.y = (_Float16)y;
     return r;
 }
+// FP16 storage records (one per half-site; each is one wide load):
+//   h6: three complex numbers, 12 B (a colour vector; one dwordx3 access)
+//   h8: four complex numbers, 16 B
+// A link of the FP16 copy is an h8 (row 0, then row 1 entry 0) plus an h6
+// (row 1 entries 1, 2, then the phase as (cos, sin)): 28 B per link.
+struct __attribute__((aligned(4))) h6 {
+    h2 v[3];
+};
+struct __attribute__((aligned(16))) h8 {
+    h2 v[4];
+};
+// Record loads and stores through global-address-space pointers (the record
+// pointers live in the system descriptor, where the compiler cannot infer the
+// address space; element-wise accesses merge into one dwordx3 / dwordx4)
+typedef __attribute__((address_space(1))) const h2 gch2;
+typedef __attribute__((address_space(1))) h2 gh2;
+__device__ __forceinline__ h6 ldg(const h6* p) {
+    gch2* g = (gch2*)p;
+    h6 r;
+    r.v[0] = g[0]; r.v[1] = g[1]; r.v[2] = g[2];
+    return r;
+}
+__device__ __forceinline__ h8 ldg(const h8* p) {
+    gch2* g = (gch2*)p;
+    h8 r;
+    r.v[0] = g[0]; r.v[1] = g[1]; r.v[2] = g[2]; r.v[3] = g[3];
+    return r;
+}
+__device__ __forceinline__ void stg(h6* p, const h6& v) {
+    gh2* g = (gh2*)p;
+    g[0] = v.v[0]; g[1] = v.v[1]; g[2] = v.v[2];
+}
+__device__ __forceinline__ void stg(h8* p, const h8& v) {
+    gh2* g = (gh2*)p;
+    g[0] = v.v[0]; g[1] = v.v[1]; g[2] = v.v[2]; g[3] = v.v[3];
+}
 
 struct BSys {
     const double2* u64[8];
@@ -60,9 +96,9 @@ struct BSys {
     double* partials;
     // FP16 inner solver: links rows 0,1 and phases (cos, sin) in half precision,
     // inner vectors in half precision (out_f stays FP32)
-    const h2* uh[8];
-    const h2* phh[8];
-    h2 *rh, *hh, *sh, *pph;
+    const h8* ula[8];
+    const h6* ulb[8];
+    h6 *rh, *hh, *sh, *pph;
 };
 
 enum BRole : int {
@@ -75,16 +111,18 @@ enum BRole : int {
     kRoleH16DeoDot,    // sh = m2f pph - D_eo hh, partial <pph,sh> (FP16 storage)
 };
 
-// One hop with half-precision storage (FP32 arithmetic); the phase is stored
-// as (cos, sin).  o += U e^{i th} in  or  o -= U^dag e^{-i th} in.
+// One hop with half-precision storage (FP32 arithmetic) on an input colour
+// vector already loaded: o += U e^{i th} v  or  o -= U^dag e^{-i th} v.
 template <bool DAG>
-__device__ __forceinline__ void hop_h(const h2* __restrict__ U, const h2* __restrict__ P, const h2* __restrict__ in,
-                                      int sizeh, int iu, int iv, float2& o0, float2& o1, float2& o2) {
-    const float2 a0 = h2f(U[iu]), a1 = h2f(U[sizeh + iu]), a2 = h2f(U[2 * sizeh + iu]);
-    const float2 b0 = h2f(U[3 * sizeh + iu]), b1 = h2f(U[4 * sizeh + iu]), b2 = h2f(U[5 * sizeh + iu]);
-    const float2 cs = h2f(P[iu]);
+__device__ __forceinline__ void hop_hv(const h8* __restrict__ LA, const h6* __restrict__ LB, int iu, const h6& vin,
+                                       float2& o0, float2& o1, float2& o2) {
+    const h8 la = ldg(LA + iu);
+    const h6 lb = ldg(LB + iu);
+    const float2 a0 = h2f(la.v[0]), a1 = h2f(la.v[1]), a2 = h2f(la.v[2]);
+    const float2 b0 = h2f(la.v[3]), b1 = h2f(lb.v[0]), b2 = h2f(lb.v[1]);
+    const float2 cs = h2f(lb.v[2]);
     const float2 ph = DAG ? make_float2(cs.x, -cs.y) : cs;
-    const float2 v0 = cmul(h2f(in[iv]), ph), v1 = cmul(h2f(in[sizeh + iv]), ph), v2 = cmul(h2f(in[2 * sizeh + iv]), ph);
+    const float2 v0 = cmul(h2f(vin.v[0]), ph), v1 = cmul(h2f(vin.v[1]), ph), v2 = cmul(h2f(vin.v[2]), ph);
     const float2 x0 = csub(cmul(a1, b2), cmul(a2, b1));
     const float2 x1 = csub(cmul(a2, b0), cmul(a0, b2));
     const float2 x2 = csub(cmul(a0, b1), cmul(a1, b0));
@@ -103,53 +141,27 @@ __device__ __forceinline__ void hop_h(const h2* __restrict__ U, const h2* __rest
 
 // p' = (half)(p gamma + r): the search-direction update of the previous
 // iteration, formed where it is read (identical rounding everywhere)
-__device__ __forceinline__ h2 pnext(const h2* __restrict__ p, const h2* __restrict__ r, long k, float g) {
-    const float2 pp = h2f(p[k]), rr = h2f(r[k]);
-    return f2h(fmaf(pp.x, g, rr.x), fmaf(pp.y, g, rr.y));
-}
-
-// hop_h whose input is p' at the neighbour
-template <bool DAG>
-__device__ __forceinline__ void hop_hp(const h2* __restrict__ U, const h2* __restrict__ P, const h2* __restrict__ pin,
-                                       const h2* __restrict__ rin, float g, int sizeh, int iu, int iv, float2& o0,
-                                       float2& o1, float2& o2) {
-    h2 v[3];
+__device__ __forceinline__ h6 pnext(const h6& p, const h6& r, float g) {
+    h6 q;
 #pragma unroll
-    for (int c = 0; c < 3; c++) v[c] = pnext(pin, rin, (long)c * sizeh + iv, g);
-    // reuse hop_h on a 3-entry local vector laid out with stride 1
-    const h2 loc[3] = {v[0], v[1], v[2]};
-    const float2 a0 = h2f(U[iu]), a1 = h2f(U[sizeh + iu]), a2 = h2f(U[2 * sizeh + iu]);
-    const float2 b0 = h2f(U[3 * sizeh + iu]), b1 = h2f(U[4 * sizeh + iu]), b2 = h2f(U[5 * sizeh + iu]);
-    const float2 cs = h2f(P[iu]);
-    const float2 ph = DAG ? make_float2(cs.x, -cs.y) : cs;
-    const float2 v0 = cmul(h2f(loc[0]), ph), v1 = cmul(h2f(loc[1]), ph), v2 = cmul(h2f(loc[2]), ph);
-    const float2 x0 = csub(cmul(a1, b2), cmul(a2, b1));
-    const float2 x1 = csub(cmul(a2, b0), cmul(a0, b2));
-    const float2 x2 = csub(cmul(a0, b1), cmul(a1, b0));
-    if (!DAG) {
-        float2 r0 = cmul(a0, v0); r0 = cfma(a1, v1, r0); r0 = cfma(a2, v2, r0);
-        float2 r1 = cmul(b0, v0); r1 = cfma(b1, v1, r1); r1 = cfma(b2, v2, r1);
-        float2 r2 = cmul(cconj(x0), v0); r2 = cfmac(x1, v1, r2); r2 = cfmac(x2, v2, r2);
-        o0 = cadd(o0, r0); o1 = cadd(o1, r1); o2 = cadd(o2, r2);
-    } else {
-        float2 r0 = cmul(cconj(a0), v0); r0 = cfmac(b0, v1, r0); r0 = cfma(x0, v2, r0);
-        float2 r1 = cmul(cconj(a1), v0); r1 = cfmac(b1, v1, r1); r1 = cfma(x1, v2, r1);
-        float2 r2 = cmul(cconj(a2), v0); r2 = cfmac(b2, v1, r2); r2 = cfma(x2, v2, r2);
-        o0 = csub(o0, r0); o1 = csub(o1, r1); o2 = csub(o2, r2);
+    for (int c = 0; c < 3; c++) {
+        const float2 pp = h2f(p.v[c]), rr = h2f(r.v[c]);
+        q.v[c] = f2h(fmaf(pp.x, g, rr.x), fmaf(pp.y, g, rr.y));
     }
+    return q;
 }
 
 template <int EO>
-__device__ __forceinline__ void dslash_site_h(const h2* const* u, const h2* const* ph, const h2* __restrict__ in,
+__device__ __forceinline__ void dslash_site_h(const h8* const* ula, const h6* const* ulb, const h6* __restrict__ in,
                                               const SiteGeom& g, int idx, float2& o0, float2& o1, float2& o2) {
     const Nbrs nb = site_neighbours<EO>(g, idx);
     constexpr int F = (EO == 0) ? 0 : 1;
     constexpr int B = 1 - F;
     o0 = o1 = o2 = make_float2(0.f, 0.f);
 #pragma unroll
-    for (int mu = 0; mu < 4; mu++) hop_h<true>(u[2 * mu + B], ph[2 * mu + B], in, g.sizeh, nb.im[mu], nb.im[mu], o0, o1, o2);
+    for (int mu = 0; mu < 4; mu++) hop_hv<true>(ula[2 * mu + B], ulb[2 * mu + B], nb.im[mu], ldg(in + nb.im[mu]), o0, o1, o2);
 #pragma unroll
-    for (int mu = 0; mu < 4; mu++) hop_h<false>(u[2 * mu + F], ph[2 * mu + F], in, g.sizeh, idx, nb.ip[mu], o0, o1, o2);
+    for (int mu = 0; mu < 4; mu++) hop_hv<false>(ula[2 * mu + F], ulb[2 * mu + F], idx, ldg(in + nb.ip[mu]), o0, o1, o2);
     o0.x *= 0.5f; o0.y *= 0.5f; o1.x *= 0.5f; o1.y *= 0.5f; o2.x *= 0.5f; o2.y *= 0.5f;
 }
 
@@ -196,33 +208,35 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
                 const Nbrs nb = site_neighbours<1>(g, idx);
                 o0 = o1 = o2 = make_float2(0.f, 0.f);
 #pragma unroll
-                for (int mu = 0; mu < 4; mu++)
-                    hop_hp<true>(S.uh[2 * mu], S.phh[2 * mu], S.pph, S.rh, gm, g.sizeh, nb.im[mu], nb.im[mu], o0, o1, o2);
+                for (int mu = 0; mu < 4; mu++) {
+                    const int j = nb.im[mu];
+                    hop_hv<true>(S.ula[2 * mu], S.ulb[2 * mu], j, pnext(ldg(S.pph + j), ldg(S.rh + j), gm), o0, o1, o2);
+                }
 #pragma unroll
-                for (int mu = 0; mu < 4; mu++)
-                    hop_hp<false>(S.uh[2 * mu + 1], S.phh[2 * mu + 1], S.pph, S.rh, gm, g.sizeh, idx, nb.ip[mu], o0, o1, o2);
+                for (int mu = 0; mu < 4; mu++) {
+                    const int j = nb.ip[mu];
+                    hop_hv<false>(S.ula[2 * mu + 1], S.ulb[2 * mu + 1], idx, pnext(ldg(S.pph + j), ldg(S.rh + j), gm), o0, o1, o2);
+                }
                 o0.x *= 0.5f; o0.y *= 0.5f; o1.x *= 0.5f; o1.y *= 0.5f; o2.x *= 0.5f; o2.y *= 0.5f;
-                S.hh[idx] = f2h(o0.x, o0.y);
-                S.hh[g.sizeh + idx] = f2h(o1.x, o1.y);
-                S.hh[2 * g.sizeh + idx] = f2h(o2.x, o2.y);
+                h6 hv;
+                hv.v[0] = f2h(o0.x, o0.y);
+                hv.v[1] = f2h(o1.x, o1.y);
+                hv.v[2] = f2h(o2.x, o2.y);
+                stg(S.hh + idx, hv);
             } else {
-                dslash_site_h<0>(S.uh, S.phh, S.hh, g, idx, o0, o1, o2);
+                dslash_site_h<0>(S.ula, S.ulb, S.hh, g, idx, o0, o1, o2);
                 const float m2 = (float)S.m2f;
                 // p' at this site, stored: the direction of this iteration
                 const float gm = (float)S.st->gammag;
-                const h2 q0 = pnext(S.pph, S.rh, idx, gm), q1 = pnext(S.pph, S.rh, g.sizeh + idx, gm),
-                         q2 = pnext(S.pph, S.rh, 2L * g.sizeh + idx, gm);
-                S.pph[idx] = q0;
-                S.pph[g.sizeh + idx] = q1;
-                S.pph[2 * g.sizeh + idx] = q2;
-                const float2 x0 = h2f(q0), x1 = h2f(q1), x2 = h2f(q2);
-                const h2 s0 = f2h(x0.x * m2 - o0.x, x0.y * m2 - o0.y);
-                const h2 s1 = f2h(x1.x * m2 - o1.x, x1.y * m2 - o1.y);
-                const h2 s2 = f2h(x2.x * m2 - o2.x, x2.y * m2 - o2.y);
-                S.sh[idx] = s0;
-                S.sh[g.sizeh + idx] = s1;
-                S.sh[2 * g.sizeh + idx] = s2;
-                dot = re_dot(x0, h2f(s0)) + re_dot(x1, h2f(s1)) + re_dot(x2, h2f(s2));
+                const h6 q = pnext(ldg(S.pph + idx), ldg(S.rh + idx), gm);
+                stg(S.pph + idx, q);
+                const float2 x0 = h2f(q.v[0]), x1 = h2f(q.v[1]), x2 = h2f(q.v[2]);
+                h6 sv;
+                sv.v[0] = f2h(x0.x * m2 - o0.x, x0.y * m2 - o0.y);
+                sv.v[1] = f2h(x1.x * m2 - o1.x, x1.y * m2 - o1.y);
+                sv.v[2] = f2h(x2.x * m2 - o2.x, x2.y * m2 - o2.y);
+                stg(S.sh + idx, sv);
+                dot = re_dot(x0, h2f(sv.v[0])) + re_dot(x1, h2f(sv.v[1])) + re_dot(x2, h2f(sv.v[2]));
             }
         }
         if constexpr (EOH == 0) block_partial_at<BLK, 1>(dot, 0.0, S.partials, tile);
@@ -415,22 +429,24 @@ __global__ __launch_bounds__(kBlock) void kb_h_rescale(const BSys* __restrict__ 
     const float rho = (float)S.st->rho;
     double v = 0.0;
     if (i < sh) {
+        h6 rv, pv = start ? h6{} : ldg(S.pph + i);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const long k = c * sh + i;
             const double2 rr = S.r64[k];
-            const h2 rh = f2h((float)(sc * rr.x), (float)(sc * rr.y));
-            S.rh[k] = rh;
-            const float2 rf = h2f(rh);
+            rv.v[c] = f2h((float)(sc * rr.x), (float)(sc * rr.y));
+            const float2 rf = h2f(rv.v[c]);
             v += (double)rf.x * rf.x + (double)rf.y * rf.y;
             if (start) {
-                S.pph[k] = rh;
+                pv.v[c] = rv.v[c];
                 S.outf[k] = make_float2(0.f, 0.f);
             } else {
-                const float2 pp = h2f(S.pph[k]);
-                S.pph[k] = f2h(pp.x * rho, pp.y * rho);
+                const float2 pp = h2f(pv.v[c]);
+                pv.v[c] = f2h(pp.x * rho, pp.y * rho);
             }
         }
+        stg(S.rh + i, rv);
+        stg(S.pph + i, pv);
     }
     block_partial_at<kBlock, 1>(v, 0.0, S.partials, blockIdx.x);
 }
@@ -450,17 +466,19 @@ __global__ __launch_bounds__(kBlock) void kb_h_xr(const BSys* __restrict__ sys, 
     if (blockIdx.x == 0 && threadIdx.x == 0) finalize<kFinHAlpha>(S.st, tot, 0.0);
     double v = 0.0;
     if (i < sh) {
+        const h6 pv = ldg(S.pph + i), sv = ldg(S.sh + i);
+        h6 rv = ldg(S.rh + i);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const long k = c * sh + i;
-            const float2 pp = h2f(S.pph[k]), oo = S.outf[k];
+            const float2 pp = h2f(pv.v[c]), oo = S.outf[k];
             S.outf[k] = make_float2(fmaf(pp.x, om, oo.x), fmaf(pp.y, om, oo.y));
-            const float2 ss = h2f(S.sh[k]), rr = h2f(S.rh[k]);
-            const h2 rn = f2h(fmaf(ss.x, -om, rr.x), fmaf(ss.y, -om, rr.y));
-            S.rh[k] = rn;
-            const float2 rf = h2f(rn);
+            const float2 ss = h2f(sv.v[c]), rr = h2f(rv.v[c]);
+            rv.v[c] = f2h(fmaf(ss.x, -om, rr.x), fmaf(ss.y, -om, rr.y));
+            const float2 rf = h2f(rv.v[c]);
             v += (double)rf.x * rf.x + (double)rf.y * rf.y;
         }
+        stg(S.rh + i, rv);
     }
     block_partial_at<kBlock, 1>(v, 0.0, S.partials + 1, blockIdx.x);
 }
@@ -484,22 +502,30 @@ __global__ __launch_bounds__(kBlock) void kb_h_flush(const BSys* __restrict__ sy
 struct HLinks {
     const double2* u[8];
     const double* ph[8];
-    h2* uh[8];
-    h2* phh[8];
+    h8* ula[8];
+    h6* ulb[8];
     int sizeh;
 };
+// FP16 copy of the links: rows 0 and 1 of the FP64 su3_soa planes and the
+// phase angle as (cos, sin), in the h8 + h6 records
 __global__ __launch_bounds__(kBlock) void kb_h_links(HLinks a) {
     const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
     if (gid >= 8L * a.sizeh) return;
     const int l = (int)(gid / a.sizeh), i = (int)(gid - (long)l * a.sizeh);
+    h2 e[6];
 #pragma unroll
     for (int p = 0; p < 6; p++) {
         const double2 v = a.u[l][(long)p * a.sizeh + i];
-        a.uh[l][(long)p * a.sizeh + i] = f2h((float)v.x, (float)v.y);
+        e[p] = f2h((float)v.x, (float)v.y);
     }
     double sn, cs;
     phase_sincos(a.ph[l][i], &sn, &cs);
-    a.phh[l][i] = f2h((float)cs, (float)sn);
+    h8 la;
+    h6 lb;
+    la.v[0] = e[0]; la.v[1] = e[1]; la.v[2] = e[2]; la.v[3] = e[3];
+    lb.v[0] = e[4]; lb.v[1] = e[5]; lb.v[2] = f2h((float)cs, (float)sn);
+    stg(a.ula[l] + i, la);
+    stg(a.ulb[l] + i, lb);
 }
 
 // one block per system: sum its partials in tile order, apply finalizer F
@@ -733,30 +759,28 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
         for (int k = 0; k < nsys; k++) {
             BSys& b = hs[k];
             const long sh = sizeh[k];
-            b.rh = (h2*)take(12 * sh);
-            b.hh = (h2*)take(12 * sh);
-            b.sh = (h2*)take(12 * sh);
-            b.pph = (h2*)take(12 * sh);
+            b.rh = (h6*)take(12 * sh);
+            b.hh = (h6*)take(12 * sh);
+            b.sh = (h6*)take(12 * sh);
+            b.pph = (h6*)take(12 * sh);
             const auto key = std::make_pair((const void*)b.u64[0], (const void*)b.ph64[0]);
             auto it = seen.find(key);
             if (it != seen.end()) {
                 for (int l = 0; l < 8; l++) {
-                    b.uh[l] = hs[it->second].uh[l];
-                    b.phh[l] = hs[it->second].phh[l];
+                    b.ula[l] = hs[it->second].ula[l];
+                    b.ulb[l] = hs[it->second].ulb[l];
                 }
                 continue;
             }
             seen[key] = k;
             HLinks a;
-            h2* ub = (h2*)take((size_t)8 * 6 * 4 * sh);
-            h2* pb = (h2*)take((size_t)8 * 4 * sh);
             for (int l = 0; l < 8; l++) {
                 a.u[l] = b.u64[l];
                 a.ph[l] = b.ph64[l];
-                a.uh[l] = ub + (size_t)l * 6 * sh;
-                a.phh[l] = pb + (size_t)l * sh;
-                b.uh[l] = a.uh[l];
-                b.phh[l] = a.phh[l];
+                a.ula[l] = (h8*)take((size_t)16 * sh);
+                a.ulb[l] = (h6*)take((size_t)12 * sh);
+                b.ula[l] = a.ula[l];
+                b.ulb[l] = a.ulb[l];
             }
             a.sizeh = (int)sh;
             hipLaunchKernelGGL(kb_h_links, dim3(blocks_for(8 * sh)), dim3(kBlock), 0, S(), a);
