@@ -101,6 +101,9 @@ struct BSys {
     h6 *rh, *hh, *sh, *pph;
 };
 
+// half-sites per thread of the FP16 x/r update (kb_h_xr)
+constexpr int kXrSites = 4;
+
 enum BRole : int {
     kRoleF32Doe = 0,   // hf = D_oe pf                          (FP32)
     kRoleF32DeoDot,    // sf = m2f pf - D_eo hf, partial <pf,sf> (FP32)
@@ -200,11 +203,62 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
     if constexpr (ROLE >= kRoleH16Doe) {
         constexpr int EOH = (ROLE == kRoleH16Doe) ? 1 : 0;
         double dot = 0.0;
+        float gm = 0.f;
+        if constexpr (EOH == 1) {
+            // End of the previous iteration, done here instead of in a
+            // one-block kernel: every block sums the x/r update's |r|^2
+            // partials (odd slots, kBlock tiles) in the same order, so every
+            // block takes the same gammag and the same exit decision.  Block 0
+            // publishes gammag/lambda; the D_eo kernel commits delta and the
+            // counter (nothing there reads them); when the iteration stops
+            // here, block 0 commits everything (the other blocks only return).
+            SolverState* st = S.st;
+            if (st->xr_fresh) {
+                __shared__ double lds[BLK / 64];
+                const int np = (g.sizeh + kBlock * kXrSites - 1) / (kBlock * kXrSites);
+                double a = 0.0;
+                for (int j = threadIdx.x; j < np; j += BLK) a += S.partials[2 * j + 1];
+                const double lambda = block_sum_all<BLK>(a, lds);
+                const double delta = st->delta;
+                const int cg1 = st->cg + 1;
+                const double true2 = lambda / (st->scale * st->scale);
+                const bool at_max = cg1 >= st->max_cg;
+                const bool conv = !(sqrt(true2 / st->source_norm) > st->res * st->safety);
+                gm = (float)(lambda / delta);
+                if (blockIdx.x == 0 && threadIdx.x == 0) {
+                    st->gammag = lambda / delta;
+                    st->lambda = lambda;
+                    if (at_max || conv) {
+                        st->delta_old = delta;
+                        st->delta = lambda;
+                        st->cg = cg1;
+                        st->xr_fresh = 2;  // consumed (still nonzero: the other blocks take this branch)
+                        if (at_max) {
+                            st->status = kDone;
+                        } else {
+                            st->ru = 1;  // inner residual converged: confirm with the FP64 residual
+                            st->hold = 1;
+                        }
+                    }
+                }
+                if (at_max || conv) return;
+            } else {
+                gm = (float)st->gammag;  // first iteration after the start or a reliable update
+            }
+        } else {
+            gm = (float)S.st->gammag;
+            if (blockIdx.x == 0 && threadIdx.x == 0 && S.st->xr_fresh) {
+                SolverState* st = S.st;
+                st->delta_old = st->delta;
+                st->delta = st->lambda;
+                st->cg += 1;
+                st->xr_fresh = 0;
+            }
+        }
         if (idx < g.sizeh) {
             float2 o0, o1, o2;
             if constexpr (EOH == 1) {
                 // D_oe of p' = p gammag + r (the previous iteration's direction update)
-                const float gm = (float)S.st->gammag;
                 const Nbrs nb = site_neighbours<1>(g, idx);
                 o0 = o1 = o2 = make_float2(0.f, 0.f);
 #pragma unroll
@@ -227,7 +281,6 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
                 dslash_site_h<0>(S.ula, S.ulb, S.hh, g, idx, o0, o1, o2);
                 const float m2 = (float)S.m2f;
                 // p' at this site, stored: the direction of this iteration
-                const float gm = (float)S.st->gammag;
                 const h6 q = pnext(ldg(S.pph + idx), ldg(S.rh + idx), gm);
                 stg(S.pph + idx, q);
                 const float2 x0 = h2f(q.v[0]), x1 = h2f(q.v[1]), x2 = h2f(q.v[2]);
@@ -454,31 +507,44 @@ __global__ __launch_bounds__(kBlock) void kb_h_rescale(const BSys* __restrict__ 
 // alpha from the D_eo partials <p', s> (even slots, summed by every block in
 // the same order, so every block computes the same omega; block 0 commits the
 // scalars), then out_f += omega p_h, r_h -= omega s_h with partial |r_h|^2 in
-// the odd slots (the even ones are still being read by other blocks)
+// the odd slots (the even ones are still being read by other blocks).  Each
+// block covers kXrSites consecutive tiles of kBlock half-sites, which keeps
+// the number of |r|^2 partials (read by every block of the next D_oe) small.
 __global__ __launch_bounds__(kBlock) void kb_h_xr(const BSys* __restrict__ sys, int dot_tile) {
-    BATCH_SITE_PROLOGUE(kIfGoing)
+    const BSys& S = sys[blockIdx.y];
+    if (!pred_pass(bctl(S, kIfGoing))) return;
+    const long sh = S.g.sizeh;
+    const int nxr = (int)((sh + kBlock * kXrSites - 1) / (kBlock * kXrSites));
+    if ((int)blockIdx.x >= nxr) return;
     __shared__ double lds[kBlock / 64];
     const int nparts = (int)((sh + dot_tile - 1) / dot_tile);
     double a = 0.0;
     for (int j = threadIdx.x; j < nparts; j += kBlock) a += S.partials[2 * j];
     const double tot = block_sum_all<kBlock>(a, lds);
     const float om = (float)(S.st->delta / tot);
-    if (blockIdx.x == 0 && threadIdx.x == 0) finalize<kFinHAlpha>(S.st, tot, 0.0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        finalize<kFinHAlpha>(S.st, tot, 0.0);
+        S.st->xr_fresh = 1;  // the next D_oe kernel ends this iteration
+    }
     double v = 0.0;
-    if (i < sh) {
-        const h6 pv = ldg(S.pph + i), sv = ldg(S.sh + i);
-        h6 rv = ldg(S.rh + i);
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const long k = c * sh + i;
-            const float2 pp = h2f(pv.v[c]), oo = S.outf[k];
-            S.outf[k] = make_float2(fmaf(pp.x, om, oo.x), fmaf(pp.y, om, oo.y));
-            const float2 ss = h2f(sv.v[c]), rr = h2f(rv.v[c]);
-            rv.v[c] = f2h(fmaf(ss.x, -om, rr.x), fmaf(ss.y, -om, rr.y));
-            const float2 rf = h2f(rv.v[c]);
-            v += (double)rf.x * rf.x + (double)rf.y * rf.y;
+    for (int u = 0; u < kXrSites; u++) {
+        const long i = ((long)blockIdx.x * kXrSites + u) * kBlock + threadIdx.x;
+        if (i < sh) {
+            const h6 pv = ldg(S.pph + i), sv = ldg(S.sh + i);
+            h6 rv = ldg(S.rh + i);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const long k = c * sh + i;
+                const float2 pp = h2f(pv.v[c]), oo = S.outf[k];
+                S.outf[k] = make_float2(fmaf(pp.x, om, oo.x), fmaf(pp.y, om, oo.y));
+                const float2 ss = h2f(sv.v[c]), rr = h2f(rv.v[c]);
+                rv.v[c] = f2h(fmaf(ss.x, -om, rr.x), fmaf(ss.y, -om, rr.y));
+                const float2 rf = h2f(rv.v[c]);
+                v += (double)rf.x * rf.x + (double)rf.y * rf.y;
+            }
+            stg(S.rh + i, rv);
         }
-        stg(S.rh + i, rv);
     }
     block_partial_at<kBlock, 1>(v, 0.0, S.partials + 1, blockIdx.x);
 }
@@ -791,6 +857,7 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
         return 0;
     const int nb_site = (max_sizeh + kBlock - 1) / kBlock;
     const dim3 gsite(nb_site, nsys);
+    const dim3 gxr((max_sizeh + kBlock * kXrSites - 1) / (kBlock * kXrSites), nsys);
 
     // ---- source norms, states
     hipLaunchKernelGGL(kb_resid_norm, gsite, dim3(kBlock), 0, S(), B.dsys, 0);
@@ -831,8 +898,7 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
         if (half) {
             launch_role<kRoleH16Doe>(t, nsys, max_sizeh, kIfGoing);
             launch_role<kRoleH16DeoDot>(t, nsys, max_sizeh, kIfGoing);
-            hipLaunchKernelGGL(kb_h_xr, gsite, dim3(kBlock), 0, S(), B.dsys, t.blk);
-            fin<kFinHLambda>(nsys, kIfGoing, kBlock, 1);
+            hipLaunchKernelGGL(kb_h_xr, gxr, dim3(kBlock), 0, S(), B.dsys, t.blk);
             // p = p gammag + r happens inside the next iteration's two hops
         } else {
             launch_role<kRoleF32Doe>(t, nsys, max_sizeh, kIfRunning);

@@ -36,7 +36,8 @@ enum Fin : int {
     kFinHScale,         // scale = sqrt(n / |r|^2) from the FP64 residual, rho = new / old
     kFinHInit,          // delta = |r_h|^2, running
     kFinHAlpha,         // as kFinMxAlpha, rho = 1; a due reliable update stays due until done
-    kFinHLambda,        // inner |r|^2, gammag, counters; inner exit test -> hold for a reliable update
+                        // (the end of an FP16 iteration -- |r|^2, gammag, exit test -- runs
+                        // inside the next D_oe kernel)
     kFinHRu,            // after a reliable update: gammag from the FP64 residual, exit test
 };
 
@@ -165,36 +166,28 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
         st->rho = 1.0;
         if (st->last_max_res < st->delta) st->last_max_res = st->delta;
         if (st->delta < st->mixed_delta * st->last_max_res) st->ru = 1;
-    } else if (F == kFinHLambda) {
-        // the reliable update itself runs once per queued block of iterations
-        // (kFinHRu); until then the inner recurrence continues
-        const double lambda = tot;
-        st->lambda = lambda;
-        st->delta_old = st->delta;
-        st->gammag = lambda / st->delta;
-        st->delta = lambda;
-        st->cg += 1;
-        const double true2 = lambda / (st->scale * st->scale);
-        if (st->cg >= st->max_cg) {
-            st->status = kDone;
-        } else if (!(sqrt(true2 / st->source_norm) > st->res * st->safety)) {
-            st->ru = 1;    // inner residual converged: confirm with the FP64 residual
-            st->hold = 1;
-        }
     } else if (F == kFinHRu) {
         // r was replaced by the (rescaled) FP64 residual and p multiplied by
         // rho: the last gammag uses the new |r|^2 over the previous |r|^2 in
-        // the new scale
+        // the new scale.  If the last x/r update's |r|^2 was not consumed yet
+        // (xr_fresh == 1; 2 means consumed), the previous |r|^2 is delta and
+        // this ends that iteration.
         const double lambda = tot;
         st->lambda = lambda;
-        st->gammag = lambda / (st->delta_old * st->rho * st->rho);
+        double prev = st->delta_old;
+        if (st->xr_fresh == 1) {
+            prev = st->delta;
+            st->cg += 1;
+        }
+        st->gammag = lambda / (prev * st->rho * st->rho);
         st->delta = lambda;
         st->ru = 0;
         st->hold = 0;
+        st->xr_fresh = 0;
         st->last_max_res = 0;
         st->ru_count += 1;
         const double true2 = lambda / (st->scale * st->scale);
-        if (!(sqrt(true2 / st->source_norm) > st->res * st->safety)) st->status = kDone;
+        if (!(sqrt(true2 / st->source_norm) > st->res * st->safety) || st->cg >= st->max_cg) st->status = kDone;
     } else if (F == kFinMxAlpha) {
         st->alpha = tot;
         st->omega = st->delta / tot;

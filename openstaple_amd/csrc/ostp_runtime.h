@@ -27,6 +27,8 @@ struct SolverState {
     int status;               // SolverStatus
     int ru;                   // mixed precision: reliable update in this iteration (FP16: due)
     int hold;                 // FP16 inner solver: inner iterations stop until the reliable update
+    int xr_fresh;             // FP16 inner solver: 1 = |r|^2 partials of an x/r update not yet
+                              // consumed, 2 = consumed by an iteration that stopped, 0 = none
     int ru_count;
     // multi-shift CG-M (inverter_multishift_full.c:48-53)
     int order, maxiter, maxiter_next, flag_cur;

@@ -120,6 +120,11 @@ def test_batched_max_cg_reports_unconverged():
     conv = lib.ostp_batched_mixed_solve(1, arr, 1e-12, 20, 0.1, 8, cg, None, tr)
     ost.check(0, "batched max_cg")
     assert conv == 0 and cg[0] == 20 and tr[0] > 1e-12
+    # FP16 inner: the iteration counter ends exactly at max_cg as well
+    dx.put(np.zeros_like(b))
+    conv = lib.ostp_batched_half_solve(1, arr, 1e-12, 20, 0.1, 8, cg, None, tr)
+    ost.check(0, "batched half max_cg")
+    assert conv == 0 and cg[0] == 20 and tr[0] > 1e-12
 
 
 def test_batched_half_precision_inner():
