@@ -47,13 +47,23 @@ __device__ __forceinline__ h2 f2h(float x, float y) {
 // FP16 storage records (one per half-site; each is one wide load):
 //   h6: three complex numbers, 12 B (a colour vector; one dwordx3 access)
 //   h8: four complex numbers, 16 B
+//   h4: two complex numbers, 8 B
 // A link of the FP16 copy is an h8 (row 0, then row 1 entry 0) plus an h6
-// (row 1 entries 1, 2, then the phase as (cos, sin)): 28 B per link.
+// (row 1 entries 1, 2, then the phase as (cos, sin)): 28 B per link.  When
+// every phase of the batch is +-1 (no background field, no imaginary
+// chemical potential) the phase is carried instead by bit 14 of row 0 entry
+// 0's real part (set: -1) and the second record is an h4 (row 1 entries 1, 2):
+// 24 B.  Bit 14 is the top exponent bit, zero for every FP16 value of
+// magnitude <= 1 (link entries), so the entry itself is kept exactly; the
+// check kernel falls back to the 28 B records if that entry exceeds 1.
 struct __attribute__((aligned(4))) h6 {
     h2 v[3];
 };
 struct __attribute__((aligned(16))) h8 {
     h2 v[4];
+};
+struct __attribute__((aligned(8))) h4 {
+    h2 v[2];
 };
 // Record loads and stores through global-address-space pointers (the record
 // pointers live in the system descriptor, where the compiler cannot infer the
@@ -71,6 +81,16 @@ __device__ __forceinline__ h8 ldg(const h8* p) {
     h8 r;
     r.v[0] = g[0]; r.v[1] = g[1]; r.v[2] = g[2]; r.v[3] = g[3];
     return r;
+}
+__device__ __forceinline__ h4 ldg(const h4* p) {
+    gch2* g = (gch2*)p;
+    h4 r;
+    r.v[0] = g[0]; r.v[1] = g[1];
+    return r;
+}
+__device__ __forceinline__ void stg(h4* p, const h4& v) {
+    gh2* g = (gh2*)p;
+    g[0] = v.v[0]; g[1] = v.v[1];
 }
 __device__ __forceinline__ void stg(h6* p, const h6& v) {
     gh2* g = (gh2*)p;
@@ -97,7 +117,8 @@ struct BSys {
     // FP16 inner solver: links rows 0,1 and phases (cos, sin) in half precision,
     // inner vectors in half precision (out_f stays FP32)
     const h8* ula[8];
-    const h6* ulb[8];
+    const h6* ulb[8];   // general phases
+    const h4* ulc[8];   // phases +-1 (sign in ula)
     h6 *rh, *hh, *sh, *pph;
 };
 
@@ -112,20 +133,15 @@ enum BRole : int {
     kRoleF64DeoResid,  // s64 = m2 x - D_eo h64, r64 = b - s64, partial |r64|^2 (FP64)
     kRoleH16Doe,       // hh = D_oe pph                         (FP16 storage)
     kRoleH16DeoDot,    // sh = m2f pph - D_eo hh, partial <pph,sh> (FP16 storage)
+    kRoleS16Doe,       // the same two with the sign-carrying 24 B links
+    kRoleS16DeoDot,
 };
 
-// One hop with half-precision storage (FP32 arithmetic) on an input colour
-// vector already loaded: o += U e^{i th} v  or  o -= U^dag e^{-i th} v.
+// One hop in FP32 arithmetic on rows 0, 1 of the link and an input colour
+// vector already multiplied by the phase: o += U v  or  o -= U^dag v.
 template <bool DAG>
-__device__ __forceinline__ void hop_hv(const h8* __restrict__ LA, const h6* __restrict__ LB, int iu, const h6& vin,
-                                       float2& o0, float2& o1, float2& o2) {
-    const h8 la = ldg(LA + iu);
-    const h6 lb = ldg(LB + iu);
-    const float2 a0 = h2f(la.v[0]), a1 = h2f(la.v[1]), a2 = h2f(la.v[2]);
-    const float2 b0 = h2f(la.v[3]), b1 = h2f(lb.v[0]), b2 = h2f(lb.v[1]);
-    const float2 cs = h2f(lb.v[2]);
-    const float2 ph = DAG ? make_float2(cs.x, -cs.y) : cs;
-    const float2 v0 = cmul(h2f(vin.v[0]), ph), v1 = cmul(h2f(vin.v[1]), ph), v2 = cmul(h2f(vin.v[2]), ph);
+__device__ __forceinline__ void hop_rows(float2 a0, float2 a1, float2 a2, float2 b0, float2 b1, float2 b2, float2 v0,
+                                         float2 v1, float2 v2, float2& o0, float2& o1, float2& o2) {
     const float2 x0 = csub(cmul(a1, b2), cmul(a2, b1));
     const float2 x1 = csub(cmul(a2, b0), cmul(a0, b2));
     const float2 x2 = csub(cmul(a0, b1), cmul(a1, b0));
@@ -142,6 +158,40 @@ __device__ __forceinline__ void hop_hv(const h8* __restrict__ LA, const h6* __re
     }
 }
 
+// One hop with FP16 storage, link at iu: o += U e^{i th} v  or  o -= U^dag e^{-i th} v.
+// SGN: 24 B link with the sign-carrying entry; otherwise 28 B with (cos, sin).
+template <bool DAG, bool SGN>
+__device__ __forceinline__ void hop_hv(const h8* __restrict__ LA, const h6* __restrict__ LB,
+                                       const h4* __restrict__ LC, int iu, const h6& vin, float2& o0, float2& o1,
+                                       float2& o2) {
+    const h8 la = ldg(LA + iu);
+    float2 a0 = h2f(la.v[0]);
+    const float2 a1 = h2f(la.v[1]), a2 = h2f(la.v[2]), b0 = h2f(la.v[3]);
+    float2 b1, b2, v0, v1, v2;
+    if constexpr (SGN) {
+        const h4 lc = ldg(LC + iu);
+        b1 = h2f(lc.v[0]);
+        b2 = h2f(lc.v[1]);
+        const unsigned short bits = __builtin_bit_cast(unsigned short, la.v[0].x);
+        const bool neg = (bits & 0x4000u) != 0;
+        a0.x = (float)__builtin_bit_cast(_Float16, (unsigned short)(bits & 0xbfffu));
+        const float2 i0 = h2f(vin.v[0]), i1 = h2f(vin.v[1]), i2 = h2f(vin.v[2]);
+        v0 = neg ? make_float2(-i0.x, -i0.y) : i0;
+        v1 = neg ? make_float2(-i1.x, -i1.y) : i1;
+        v2 = neg ? make_float2(-i2.x, -i2.y) : i2;
+    } else {
+        const h6 lb = ldg(LB + iu);
+        b1 = h2f(lb.v[0]);
+        b2 = h2f(lb.v[1]);
+        const float2 cs = h2f(lb.v[2]);
+        const float2 ph = DAG ? make_float2(cs.x, -cs.y) : cs;
+        v0 = cmul(h2f(vin.v[0]), ph);
+        v1 = cmul(h2f(vin.v[1]), ph);
+        v2 = cmul(h2f(vin.v[2]), ph);
+    }
+    hop_rows<DAG>(a0, a1, a2, b0, b1, b2, v0, v1, v2, o0, o1, o2);
+}
+
 // p' = (half)(p gamma + r): the search-direction update of the previous
 // iteration, formed where it is read (identical rounding everywhere)
 __device__ __forceinline__ h6 pnext(const h6& p, const h6& r, float g) {
@@ -154,17 +204,18 @@ __device__ __forceinline__ h6 pnext(const h6& p, const h6& r, float g) {
     return q;
 }
 
-template <int EO>
-__device__ __forceinline__ void dslash_site_h(const h8* const* ula, const h6* const* ulb, const h6* __restrict__ in,
-                                              const SiteGeom& g, int idx, float2& o0, float2& o1, float2& o2) {
+template <int EO, bool SGN>
+__device__ __forceinline__ void dslash_site_h(const h8* const* ula, const h6* const* ulb, const h4* const* ulc,
+                                              const h6* __restrict__ in, const SiteGeom& g, int idx, float2& o0,
+                                              float2& o1, float2& o2) {
     const Nbrs nb = site_neighbours<EO>(g, idx);
     constexpr int F = (EO == 0) ? 0 : 1;
     constexpr int B = 1 - F;
     o0 = o1 = o2 = make_float2(0.f, 0.f);
 #pragma unroll
-    for (int mu = 0; mu < 4; mu++) hop_hv<true>(ula[2 * mu + B], ulb[2 * mu + B], nb.im[mu], ldg(in + nb.im[mu]), o0, o1, o2);
+    for (int mu = 0; mu < 4; mu++) hop_hv<true, SGN>(ula[2 * mu + B], ulb[2 * mu + B], ulc[2 * mu + B], nb.im[mu], ldg(in + nb.im[mu]), o0, o1, o2);
 #pragma unroll
-    for (int mu = 0; mu < 4; mu++) hop_hv<false>(ula[2 * mu + F], ulb[2 * mu + F], idx, ldg(in + nb.ip[mu]), o0, o1, o2);
+    for (int mu = 0; mu < 4; mu++) hop_hv<false, SGN>(ula[2 * mu + F], ulb[2 * mu + F], ulc[2 * mu + F], idx, ldg(in + nb.ip[mu]), o0, o1, o2);
     o0.x *= 0.5f; o0.y *= 0.5f; o1.x *= 0.5f; o1.y *= 0.5f; o2.x *= 0.5f; o2.y *= 0.5f;
 }
 
@@ -201,7 +252,8 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
         idx = tile * BLK + (int)threadIdx.x;
     }
     if constexpr (ROLE >= kRoleH16Doe) {
-        constexpr int EOH = (ROLE == kRoleH16Doe) ? 1 : 0;
+        constexpr int EOH = (ROLE == kRoleH16Doe || ROLE == kRoleS16Doe) ? 1 : 0;
+        constexpr bool SGN = ROLE >= kRoleS16Doe;
         double dot = 0.0;
         float gm = 0.f;
         if constexpr (EOH == 1) {
@@ -264,12 +316,14 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
 #pragma unroll
                 for (int mu = 0; mu < 4; mu++) {
                     const int j = nb.im[mu];
-                    hop_hv<true>(S.ula[2 * mu], S.ulb[2 * mu], j, pnext(ldg(S.pph + j), ldg(S.rh + j), gm), o0, o1, o2);
+                    hop_hv<true, SGN>(S.ula[2 * mu], S.ulb[2 * mu], S.ulc[2 * mu], j,
+                                      pnext(ldg(S.pph + j), ldg(S.rh + j), gm), o0, o1, o2);
                 }
 #pragma unroll
                 for (int mu = 0; mu < 4; mu++) {
                     const int j = nb.ip[mu];
-                    hop_hv<false>(S.ula[2 * mu + 1], S.ulb[2 * mu + 1], idx, pnext(ldg(S.pph + j), ldg(S.rh + j), gm), o0, o1, o2);
+                    hop_hv<false, SGN>(S.ula[2 * mu + 1], S.ulb[2 * mu + 1], S.ulc[2 * mu + 1], idx,
+                                       pnext(ldg(S.pph + j), ldg(S.rh + j), gm), o0, o1, o2);
                 }
                 o0.x *= 0.5f; o0.y *= 0.5f; o1.x *= 0.5f; o1.y *= 0.5f; o2.x *= 0.5f; o2.y *= 0.5f;
                 h6 hv;
@@ -278,7 +332,7 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
                 hv.v[2] = f2h(o2.x, o2.y);
                 stg(S.hh + idx, hv);
             } else {
-                dslash_site_h<0>(S.ula, S.ulb, S.hh, g, idx, o0, o1, o2);
+                dslash_site_h<0, SGN>(S.ula, S.ulb, S.ulc, S.hh, g, idx, o0, o1, o2);
                 const float m2 = (float)S.m2f;
                 // p' at this site, stored: the direction of this iteration
                 const h6 q = pnext(ldg(S.pph + idx), ldg(S.rh + idx), gm);
@@ -572,9 +626,25 @@ struct HLinks {
     h6* ulb[8];
     int sizeh;
 };
+// *flag = 1 if some phase of the field is not +-1 after rounding to FP16, or
+// some row 0 entry 0 real part exceeds 1 in magnitude (every thread that finds
+// one stores the same value)
+__global__ __launch_bounds__(kBlock) void kb_h_phase_check(HLinks a, int* flag) {
+    const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (gid >= 8L * a.sizeh) return;
+    const int l = (int)(gid / a.sizeh), i = (int)(gid - (long)l * a.sizeh);
+    double sn, cs;
+    phase_sincos(a.ph[l][i], &sn, &cs);
+    const h2 p = f2h((float)cs, (float)sn);
+    const float c = (float)p.x, sv = (float)p.y;
+    const float e = (float)(_Float16)(float)a.u[l][i].x;
+    if (!(sv == 0.f && (c == 1.f || c == -1.f)) || !(fabsf(e) <= 1.f)) *flag = 1;
+}
+
 // FP16 copy of the links: rows 0 and 1 of the FP64 su3_soa planes and the
-// phase angle as (cos, sin), in the h8 + h6 records
-__global__ __launch_bounds__(kBlock) void kb_h_links(HLinks a) {
+// phase, as h8 + h6 records (phase as (cos, sin)) or, sgn, h8 + h4 records
+// (phase +-1 in bit 14 of row 0 entry 0's real part)
+__global__ __launch_bounds__(kBlock) void kb_h_links(HLinks a, int sgn) {
     const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
     if (gid >= 8L * a.sizeh) return;
     const int l = (int)(gid / a.sizeh), i = (int)(gid - (long)l * a.sizeh);
@@ -586,12 +656,23 @@ __global__ __launch_bounds__(kBlock) void kb_h_links(HLinks a) {
     }
     double sn, cs;
     phase_sincos(a.ph[l][i], &sn, &cs);
+    const h2 ph = f2h((float)cs, (float)sn);
     h8 la;
-    h6 lb;
     la.v[0] = e[0]; la.v[1] = e[1]; la.v[2] = e[2]; la.v[3] = e[3];
-    lb.v[0] = e[4]; lb.v[1] = e[5]; lb.v[2] = f2h((float)cs, (float)sn);
-    stg(a.ula[l] + i, la);
-    stg(a.ulb[l] + i, lb);
+    if (sgn) {
+        unsigned short bits = __builtin_bit_cast(unsigned short, la.v[0].x);
+        bits = (unsigned short)(bits | ((float)ph.x < 0.f ? 0x4000u : 0u));
+        la.v[0].x = __builtin_bit_cast(_Float16, bits);
+        h4 lc;
+        lc.v[0] = e[4]; lc.v[1] = e[5];
+        stg(a.ula[l] + i, la);
+        stg((h4*)a.ulb[l] + i, lc);
+    } else {
+        h6 lb;
+        lb.v[0] = e[4]; lb.v[1] = e[5]; lb.v[2] = ph;
+        stg(a.ula[l] + i, la);
+        stg(a.ulb[l] + i, lb);
+    }
 }
 
 // one block per system: sum its partials in tile order, apply finalizer F
@@ -633,6 +714,7 @@ struct Batch {
     size_t st_cap = 0;
 };
 static Batch g_batch;
+static bool g_sgn = false;  // FP16 links of this call carry their +-1 phase as a sign bit
 
 static hipStream_t S() { return ctx().stream; }
 
@@ -688,7 +770,8 @@ static Tile autotune(const std::string& sig, int nsys, int max_sizeh, bool half)
     for (const Tile& c0 : cands) {
         Tile c = c0;
         auto one = [&]() {
-            if (half) launch_role<kRoleH16Doe>(c, nsys, max_sizeh, kAlways);
+            if (half && g_sgn) launch_role<kRoleS16Doe>(c, nsys, max_sizeh, kAlways);
+            else if (half) launch_role<kRoleH16Doe>(c, nsys, max_sizeh, kAlways);
             else launch_role<kRoleF32Doe>(c, nsys, max_sizeh, kAlways);
         };
         one();
@@ -749,7 +832,7 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
         max_sizeh = std::max(max_sizeh, (int)sizeh[k]);
         max_t64 = std::max(max_t64, (int)((sizeh[k] + 63) / 64));
         arena += (size_t)(3 * 48 + 5 * 24) * sizeh[k] + 256 * 8;  // vectors (+ alignment slack)
-        if (half) arena += (size_t)(4 * 12 + 8 * 28) * sizeh[k] + 256 * 20;  // half vectors + a half link copy
+        if (half) arena += (size_t)(4 * 12 + 8 * 28) * sizeh[k] + 256 * 21;  // half vectors + a half link copy + flag
     }
     const size_t part_per = (size_t)2 * max_t64;
     arena += (sizeof(double) * part_per + 256) * nsys;
@@ -822,6 +905,7 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
     if (half) {
         // half vectors, and one half copy of each distinct gauge field (systems sharing links share it)
         std::map<std::pair<const void*, const void*>, int> seen;
+        std::vector<HLinks> fields;
         for (int k = 0; k < nsys; k++) {
             BSys& b = hs[k];
             const long sh = sizeh[k];
@@ -835,6 +919,7 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
                 for (int l = 0; l < 8; l++) {
                     b.ula[l] = hs[it->second].ula[l];
                     b.ulb[l] = hs[it->second].ulb[l];
+                    b.ulc[l] = hs[it->second].ulc[l];
                 }
                 continue;
             }
@@ -844,13 +929,26 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
                 a.u[l] = b.u64[l];
                 a.ph[l] = b.ph64[l];
                 a.ula[l] = (h8*)take((size_t)16 * sh);
-                a.ulb[l] = (h6*)take((size_t)12 * sh);
+                a.ulb[l] = (h6*)take((size_t)12 * sh);  // room for either second record
                 b.ula[l] = a.ula[l];
                 b.ulb[l] = a.ulb[l];
+                b.ulc[l] = (const h4*)a.ulb[l];
             }
             a.sizeh = (int)sh;
-            hipLaunchKernelGGL(kb_h_links, dim3(blocks_for(8 * sh)), dim3(kBlock), 0, S(), a);
+            fields.push_back(a);
         }
+        // the 24 B sign-carrying links if every phase of every field is +-1
+        int* flag = (int*)take(256);
+        int h_flag = 0;
+        if (!hip_ok(hipMemsetAsync(flag, 0, sizeof(int), S()), "phase flag")) return 0;
+        for (const HLinks& a : fields)
+            hipLaunchKernelGGL(kb_h_phase_check, dim3(blocks_for(8L * a.sizeh)), dim3(kBlock), 0, S(), a, flag);
+        if (!hip_ok(hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, S()), "phase flag") ||
+            !hip_ok(hipStreamSynchronize(S()), "phase flag"))
+            return 0;
+        g_sgn = (h_flag == 0);
+        for (const HLinks& a : fields)
+            hipLaunchKernelGGL(kb_h_links, dim3(blocks_for(8L * a.sizeh)), dim3(kBlock), 0, S(), a, (int)g_sgn);
         if (!hip_ok(hipGetLastError(), "half links")) return 0;
     }
     if (!hip_ok(hipMemcpyAsync(B.dsys, hs.data(), sizeof(BSys) * nsys, hipMemcpyHostToDevice, S()), "descriptors"))
@@ -876,7 +974,7 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
         s.status = kRunning;
     }
     if (!push_states(nsys)) return 0;
-    const std::string sig = signature(hs) + (half ? " fp16" : " fp32");
+    const std::string sig = signature(hs) + (half ? (g_sgn ? " fp16 sign-phase" : " fp16") : " fp32");
     const Tile t = autotune(sig, nsys, max_sizeh, half);
 
     // r = b - M x (FP64); inner r = p = r, out = 0, delta = |r_inner|^2
@@ -896,8 +994,13 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
     // ---- one inner iteration of every system (predicated on each system's state)
     auto enqueue_iteration = [&]() {
         if (half) {
-            launch_role<kRoleH16Doe>(t, nsys, max_sizeh, kIfGoing);
-            launch_role<kRoleH16DeoDot>(t, nsys, max_sizeh, kIfGoing);
+            if (g_sgn) {
+                launch_role<kRoleS16Doe>(t, nsys, max_sizeh, kIfGoing);
+                launch_role<kRoleS16DeoDot>(t, nsys, max_sizeh, kIfGoing);
+            } else {
+                launch_role<kRoleH16Doe>(t, nsys, max_sizeh, kIfGoing);
+                launch_role<kRoleH16DeoDot>(t, nsys, max_sizeh, kIfGoing);
+            }
             hipLaunchKernelGGL(kb_h_xr, gxr, dim3(kBlock), 0, S(), B.dsys, t.blk);
             // p = p gammag + r happens inside the next iteration's two hops
         } else {

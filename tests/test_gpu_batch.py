@@ -25,7 +25,7 @@ RES = 1e-9
 class Field:
     """A gauge field (FP64 + FP32 copies and phases) on the device."""
 
-    def __init__(self, shape, par_temp, kind, seed):
+    def __init__(self, shape, par_temp, kind, seed, mu=0.0):
         self.o = Oracle(*shape, par_temp=par_temp)
         self.shape = shape
         self.par_temp = par_temp
@@ -33,7 +33,7 @@ class Field:
         self.u = o.gen_expH(0.3, seed) if kind == "expH" else o.gen_cold(0.5, seed)
         if par_temp:
             o.init_k(self.u, 0.25 * seed, 3, shape[:3])
-        self.ph = o.phases()
+        self.ph = o.phases(mu=mu)
         self.uf = o.to_float_gauge(self.u)
         self.du, self.duf = Dev(self.u), Dev(self.uf)
         self.dph, self.dphf = Dev(self.ph), Dev(self.ph.astype(np.float32))
@@ -127,12 +127,15 @@ def test_batched_max_cg_reports_unconverged():
     assert conv == 0 and cg[0] == 20 and tr[0] > 1e-12
 
 
-def test_batched_half_precision_inner():
+@pytest.mark.parametrize("mu", [0.0, 0.2])
+def test_batched_half_precision_inner(mu):
     """FP16 storage in the inner solver: every system still reaches its FP64
-    true residual (reliable updates), within 1.5x the FP32-inner iterations."""
+    true residual (reliable updates), within 1.5x the FP32-inner iterations.
+    mu = 0: every phase is +-1 (24 B sign-carrying FP16 links); mu = 0.2
+    (imaginary chemical potential): general phases (28 B links with cos, sin)."""
     init_lattice((8, 8, 8, 8))
-    main = Field((8, 8, 8, 8), False, "expH", 1)
-    reps = [Field((4, 6, 8, 10), True, "cold", s) for s in (2, 3)]
+    main = Field((8, 8, 8, 8), False, "expH", 1, mu)
+    reps = [Field((4, 6, 8, 10), True, "cold", s, mu) for s in (2, 3)]
     cases = [(main, main.o.gen_z2(11, k), 0.05) for k in range(3)]
     cases += [(f, f.o.gen_z2(12, k), 0.1 + 0.05 * k) for k, f in enumerate(reps)]
     n = len(cases)
@@ -148,7 +151,11 @@ def test_batched_half_precision_inner():
     cg, ru, tr = (C.c_int * n)(), (C.c_int * n)(), (C.c_double * n)()
     conv = lib.ostp_batched_half_solve(n, arr, RES, 10000, 0.1, 8, cg, ru, tr)
     ost.check(0, "batched half solve")
-    print("\nFP16-inner iterations", list(cg), "FP32-inner", list(cg32), "reliable updates", list(ru))
+    print(f"\nmu={mu}: FP16-inner iterations", list(cg), "FP32-inner", list(cg32), "reliable updates", list(ru))
+    rep = C.create_string_buffer(lib.ostp_autotune_report(None, 0) + 1)
+    lib.ostp_autotune_report(rep, len(rep))
+    last = rep.value.strip().splitlines()[-1]  # the line of this batch's signature (tuned just now)
+    assert (b"sign-phase" in last) == (mu == 0.0), last
     assert conv == n, list(tr)
     for k, (f, b, m) in enumerate(cases):
         x = keep[k][1].get(b)
