@@ -53,6 +53,8 @@ def parse():
                     help="(b, x) pairs from plain FP64 CG to train the preconditioner on (0: skip)")
     ap.add_argument("--learned-steps", type=int, default=300)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve workload")
+    ap.add_argument("--iters-per-graph", type=int, default=16,
+                    help="iterations captured per HIP graph in the batched driver (0: direct launches, for profilers)")
     ap.add_argument("--dslash-only", action="store_true",
                     help="only the timed Dslash loop and the copy kernel (for counter runs)")
     return ap.parse_args()
@@ -185,7 +187,7 @@ def main():
     if not args.no_solves:
         result["solves"] = run_solves(ost, lib, L, u, ph, b, uf, phf, args)
     if not args.no_batch:
-        result["batched"] = run_batched(ost, lib)
+        result["batched"] = run_batched(ost, lib, args)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(shape, args.cpu_seconds)
@@ -310,11 +312,11 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     arr = (ost.BatchSystem * 1)(sysd)
     cgs, rus, trs = (C.c_int * 1)(), (C.c_int * 1)(), (C.c_double * 1)()
     lib.ostp_memset(x, 0, vb)
-    lib.ostp_batched_mixed_solve(1, arr, 1e-10, 40000, 0.1, 16, cgs, rus, trs)  # warm-up: autotune, graph
+    lib.ostp_batched_mixed_solve(1, arr, 1e-10, 40000, 0.1, args.iters_per_graph, cgs, rus, trs)  # warm-up: autotune, graph
     lib.ostp_memset(x, 0, vb)
     lib.ostp_synchronize()
     t0 = time.perf_counter()
-    conv = lib.ostp_batched_mixed_solve(1, arr, 1e-10, 40000, 0.1, 16, cgs, rus, trs)
+    conv = lib.ostp_batched_mixed_solve(1, arr, 1e-10, 40000, 0.1, args.iters_per_graph, cgs, rus, trs)
     lib.ostp_synchronize()
     t1 = time.perf_counter()
     res["cg_mixed_graph"] = {"seconds": round(t1 - t0, 4), "iterations": cgs[0], "reliable_updates": rus[0],
@@ -322,11 +324,11 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     res["cg_speedup_mixed_graph_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
     # FP16 storage in the inner solver (same graph-captured driver)
     lib.ostp_memset(x, 0, vb)
-    lib.ostp_batched_half_solve(1, arr, 1e-10, 40000, 0.1, 16, cgs, rus, trs)  # warm-up
+    lib.ostp_batched_half_solve(1, arr, 1e-10, 40000, 0.1, args.iters_per_graph, cgs, rus, trs)  # warm-up
     lib.ostp_memset(x, 0, vb)
     lib.ostp_synchronize()
     t0 = time.perf_counter()
-    conv = lib.ostp_batched_half_solve(1, arr, 1e-10, 40000, 0.1, 16, cgs, rus, trs)
+    conv = lib.ostp_batched_half_solve(1, arr, 1e-10, 40000, 0.1, args.iters_per_graph, cgs, rus, trs)
     lib.ostp_synchronize()
     t1 = time.perf_counter()
     res["cg_half_graph"] = {"seconds": round(t1 - t0, 4), "iterations": cgs[0], "reliable_updates": rus[0],
@@ -367,7 +369,7 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     return res
 
 
-def run_batched(ost, lib):
+def run_batched(ost, lib, args):
     """24 Z2 sources on a 32^3x8 field plus 6 parallel-tempering replicas on
     24^3x8 fields (K = c_r on the t-boundary defect, c_r = 0, 0.2, ..., 1),
     m = 0.02, to 1e-9: one batched solve (HIP graph) against the same 30
@@ -428,12 +430,12 @@ def run_batched(ost, lib):
     n = len(systems)
     arr = (ost.BatchSystem * n)(*systems)
     cg, ru, tr = (C.c_int * n)(), (C.c_int * n)(), (C.c_double * n)()
-    lib.ostp_batched_mixed_solve(n, arr, res, 40000, 0.1, 16, cg, ru, tr)  # warm-up: autotune
+    lib.ostp_batched_mixed_solve(n, arr, res, 40000, 0.1, args.iters_per_graph, cg, ru, tr)  # warm-up: autotune
     for sy in systems:
         lib.ostp_memset(sy.out, 0, 48 * sy.nd[0] * sy.nd[1] * sy.nd[2] * sy.nd[3] // 2)
     lib.ostp_synchronize()
     t0 = time.perf_counter()
-    conv = lib.ostp_batched_mixed_solve(n, arr, res, 40000, 0.1, 16, cg, ru, tr)
+    conv = lib.ostp_batched_mixed_solve(n, arr, res, 40000, 0.1, args.iters_per_graph, cg, ru, tr)
     lib.ostp_synchronize()
     t1 = time.perf_counter()
     # algorithmic Dslash bytes: FP32 pairs per iteration, FP64 pairs per reliable update and at start/end
@@ -445,12 +447,12 @@ def run_batched(ost, lib):
     for sy in systems:
         lib.ostp_memset(sy.out, 0, 48 * sy.nd[0] * sy.nd[1] * sy.nd[2] * sy.nd[3] // 2)
     ch, rh, trh = (C.c_int * n)(), (C.c_int * n)(), (C.c_double * n)()
-    lib.ostp_batched_half_solve(n, arr, res, 40000, 0.1, 16, ch, rh, trh)  # warm-up: autotune
+    lib.ostp_batched_half_solve(n, arr, res, 40000, 0.1, args.iters_per_graph, ch, rh, trh)  # warm-up: autotune
     for sy in systems:
         lib.ostp_memset(sy.out, 0, 48 * sy.nd[0] * sy.nd[1] * sy.nd[2] * sy.nd[3] // 2)
     lib.ostp_synchronize()
     th0 = time.perf_counter()
-    convh = lib.ostp_batched_half_solve(n, arr, res, 40000, 0.1, 16, ch, rh, trh)
+    convh = lib.ostp_batched_half_solve(n, arr, res, 40000, 0.1, args.iters_per_graph, ch, rh, trh)
     lib.ostp_synchronize()
     th1 = time.perf_counter()
     out = {"systems": n, "converged": conv, "seconds": round(t1 - t0, 4), "iterations_total": sum(cg),

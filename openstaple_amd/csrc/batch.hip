@@ -570,6 +570,21 @@ __global__ __launch_bounds__(kBlock) void kb_h_xr(const BSys* __restrict__ sys, 
     const long sh = S.g.sizeh;
     const int nxr = (int)((sh + kBlock * kXrSites - 1) / (kBlock * kXrSites));
     if ((int)blockIdx.x >= nxr) return;
+    // the site data does not depend on alpha: load it before the sum so the
+    // memory latency overlaps the partial reads
+    h6 pv[kXrSites], sv[kXrSites], rv[kXrSites];
+    float2 oo[kXrSites][3];
+#pragma unroll
+    for (int u = 0; u < kXrSites; u++) {
+        const long i = ((long)blockIdx.x * kXrSites + u) * kBlock + threadIdx.x;
+        if (i < sh) {
+            pv[u] = ldg(S.pph + i);
+            sv[u] = ldg(S.sh + i);
+            rv[u] = ldg(S.rh + i);
+#pragma unroll
+            for (int c = 0; c < 3; c++) oo[u][c] = S.outf[c * sh + i];
+        }
+    }
     __shared__ double lds[kBlock / 64];
     const int nparts = (int)((sh + dot_tile - 1) / dot_tile);
     double a = 0.0;
@@ -585,19 +600,16 @@ __global__ __launch_bounds__(kBlock) void kb_h_xr(const BSys* __restrict__ sys, 
     for (int u = 0; u < kXrSites; u++) {
         const long i = ((long)blockIdx.x * kXrSites + u) * kBlock + threadIdx.x;
         if (i < sh) {
-            const h6 pv = ldg(S.pph + i), sv = ldg(S.sh + i);
-            h6 rv = ldg(S.rh + i);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                const long k = c * sh + i;
-                const float2 pp = h2f(pv.v[c]), oo = S.outf[k];
-                S.outf[k] = make_float2(fmaf(pp.x, om, oo.x), fmaf(pp.y, om, oo.y));
-                const float2 ss = h2f(sv.v[c]), rr = h2f(rv.v[c]);
-                rv.v[c] = f2h(fmaf(ss.x, -om, rr.x), fmaf(ss.y, -om, rr.y));
-                const float2 rf = h2f(rv.v[c]);
+                const float2 pp = h2f(pv[u].v[c]);
+                S.outf[c * sh + i] = make_float2(fmaf(pp.x, om, oo[u][c].x), fmaf(pp.y, om, oo[u][c].y));
+                const float2 ss = h2f(sv[u].v[c]), rr = h2f(rv[u].v[c]);
+                rv[u].v[c] = f2h(fmaf(ss.x, -om, rr.x), fmaf(ss.y, -om, rr.y));
+                const float2 rf = h2f(rv[u].v[c]);
                 v += (double)rf.x * rf.x + (double)rf.y * rf.y;
             }
-            stg(S.rh + i, rv);
+            stg(S.rh + i, rv[u]);
         }
     }
     block_partial_at<kBlock, 1>(v, 0.0, S.partials + 1, blockIdx.x);
@@ -709,6 +721,8 @@ struct Batch {
     BSys* dsys = nullptr;
     SolverState* st = nullptr;
     SolverState* st_host = nullptr;
+    SolverState* st_snap[2] = {nullptr, nullptr};  // pinned snapshots taken behind queued replays
+    hipEvent_t snap_ev[2] = {nullptr, nullptr};
     void* arena = nullptr;
     size_t arena_bytes = 0;
     size_t st_cap = 0;
@@ -847,13 +861,18 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
     if ((size_t)nsys > B.st_cap) {
         if (B.st) (void)hipFree(B.st);
         if (B.st_host) (void)hipHostFree(B.st_host);
+        for (int j = 0; j < 2; j++)
+            if (B.st_snap[j]) (void)hipHostFree(B.st_snap[j]);
         if (B.dsys) (void)hipFree(B.dsys);
         B.st = nullptr;
         B.st_host = nullptr;
+        B.st_snap[0] = B.st_snap[1] = nullptr;
         B.dsys = nullptr;
         B.st_cap = 0;
         if (!hip_ok(hipMalloc(&B.st, sizeof(SolverState) * nsys), "batch states") ||
             !hip_ok(hipHostMalloc(&B.st_host, sizeof(SolverState) * nsys, hipHostMallocDefault), "batch states") ||
+            !hip_ok(hipHostMalloc(&B.st_snap[0], sizeof(SolverState) * nsys, hipHostMallocDefault), "batch states") ||
+            !hip_ok(hipHostMalloc(&B.st_snap[1], sizeof(SolverState) * nsys, hipHostMallocDefault), "batch states") ||
             !hip_ok(hipMalloc(&B.dsys, sizeof(BSys) * nsys), "batch descriptors"))
             return 0;
         B.st_cap = nsys;
@@ -1044,28 +1063,50 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
             return 0;
         }
     }
-    bool ok = true;
-    long last_total = -1;
-    for (;;) {
-        if (!pull_states(nsys)) { ok = false; break; }
-        long total = 0;
-        bool running = false;
-        for (int k = 0; k < nsys; k++) {
-            total += B.st_host[k].cg;
-            running = running || B.st_host[k].status == kRunning;
-        }
-        if (!running) break;
-        if (total == last_total) {
-            ok = fail("%s: solver states did not advance", nm);
-            break;
-        }
-        last_total = total;
+    // Replays are pipelined: replay k+1 is queued (with a state snapshot
+    // behind it) before the host waits for the snapshot taken after replay
+    // k, so the GPU never idles on the host's convergence check.  The replay
+    // queued past the end runs as predicated no-ops.
+    for (int j = 0; j < 2; j++)
+        if (!B.snap_ev[j] && !hip_ok(hipEventCreateWithFlags(&B.snap_ev[j], hipEventDisableTiming), "event"))
+            return 0;
+    auto replay = [&](int slot) -> bool {
         if (exec) {
-            if (!hip_ok(hipGraphLaunch(exec, S()), "graph launch")) { ok = false; break; }
+            if (!hip_ok(hipGraphLaunch(exec, S()), "graph launch")) return false;
         } else {
             for (int j = 0; j < 16; j++) enqueue_iteration();
             enqueue_reliable_update();
-            if (!hip_ok(hipGetLastError(), nm)) { ok = false; break; }
+            if (!hip_ok(hipGetLastError(), nm)) return false;
+        }
+        return hip_ok(hipMemcpyAsync(B.st_snap[slot], B.st, sizeof(SolverState) * nsys, hipMemcpyDeviceToHost, S()),
+                      "state snapshot") &&
+               hip_ok(hipEventRecord(B.snap_ev[slot], S()), "event");
+    };
+    bool ok = true;
+    long last_total = -1;
+    bool any = false;
+    for (int k = 0; k < nsys; k++) any = any || B.st_host[k].status == kRunning;
+    if (any) {
+        int cur = 0;
+        ok = replay(cur);
+        while (ok) {
+            if (!replay(cur ^ 1) || !hip_ok(hipEventSynchronize(B.snap_ev[cur]), "snapshot wait")) {
+                ok = false;
+                break;
+            }
+            long total = 0;
+            bool running = false;
+            for (int k = 0; k < nsys; k++) {
+                total += B.st_snap[cur][k].cg;
+                running = running || B.st_snap[cur][k].status == kRunning;
+            }
+            if (!running) break;
+            if (total == last_total) {
+                ok = fail("%s: solver states did not advance", nm);
+                break;
+            }
+            last_total = total;
+            cur ^= 1;
         }
     }
     if (exec) (void)hipGraphExecDestroy(exec);
