@@ -233,7 +233,15 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     sf = dm(order * vb)
     sf32 = dm(order * vb // 2)
     x = dm(vb)
+    x64 = dm(vb)      # FP64 CG solution: the reference point of the reduced-precision solutions
+    out64 = dm(order * vb)
+    diff = dm(vb)
     res = {}
+
+    def rel_err(a, ref):
+        """||a - ref|| / ||ref|| (FP64, library BLAS)."""
+        lib.combine_in1_minus_in2(a, ref, diff)
+        return (lib.l2norm2_global(diff) / lib.l2norm2_global(ref)) ** 0.5
 
     # (a) FP64 CG-M
     cg = C.c_int(0)
@@ -245,6 +253,8 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     t1 = time.perf_counter()
     res["multishift_fp64"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "status": st,
                               "max_shift_true_residual": max(sres[k] for k in range(order)), "shifts": order}
+    for k in range(order):
+        lib.assign_in_to_out(out + k * vb, out64 + k * vb)
 
     # (b) FP32-inner: FP32 CG-M then per-shift FP32 CG with FP64 reliable updates
     ip = ost.InverterPackage()
@@ -269,7 +279,9 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
         lib.combine_in1_minus_in2(b, s_tmp, h_tmp)
         worst = max(worst, (lib.l2norm2_global(h_tmp) / bn) ** 0.5)
     res["multishift_mixed"] = {"seconds": round(t1 - t0, 4), "wrapper_iterations": it,
-                               "max_shift_true_residual": worst}
+                               "max_shift_true_residual": worst,
+                               "max_shift_err_vs_fp64": max(rel_err(out + k * vb, out64 + k * vb)
+                                                            for k in range(order))}
     res["multishift_speedup_mixed_vs_fp64"] = round(res["multishift_fp64"]["seconds"] / (t1 - t0), 3)
 
     # (c) plain FP64 CG and FP32-inner CG at m = 0.005 on a 2-step stout-smeared
@@ -290,6 +302,7 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     t1 = time.perf_counter()
     res["cg_fp64"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "true_residual": tr.value,
                       "status": st, "mass": m4, "field": "eps=0.7 seed 2, 2 stout steps rho=0.15"}
+    lib.assign_in_to_out(x, x64)
     ru = C.c_int(0)
     lib.ostp_memset(x, 0, vb)
     lib.ostp_synchronize()
@@ -299,7 +312,7 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     lib.ostp_synchronize()
     t1 = time.perf_counter()
     res["cg_mixed"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "reliable_updates": ru.value,
-                       "true_residual": tr.value, "status": st}
+                       "true_residual": tr.value, "status": st, "err_vs_fp64": rel_err(x, x64)}
     res["cg_speedup_mixed_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
 
     # the same mixed-precision algorithm as one system of the batched solver: its
@@ -320,7 +333,7 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     lib.ostp_synchronize()
     t1 = time.perf_counter()
     res["cg_mixed_graph"] = {"seconds": round(t1 - t0, 4), "iterations": cgs[0], "reliable_updates": rus[0],
-                             "true_residual": trs[0], "status": int(conv == 1)}
+                             "true_residual": trs[0], "status": int(conv == 1), "err_vs_fp64": rel_err(x, x64)}
     res["cg_speedup_mixed_graph_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
     # FP16 storage in the inner solver (same graph-captured driver)
     lib.ostp_memset(x, 0, vb)
@@ -332,7 +345,7 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     lib.ostp_synchronize()
     t1 = time.perf_counter()
     res["cg_half_graph"] = {"seconds": round(t1 - t0, 4), "iterations": cgs[0], "reliable_updates": rus[0],
-                            "true_residual": trs[0], "status": int(conv == 1)}
+                            "true_residual": trs[0], "status": int(conv == 1), "err_vs_fp64": rel_err(x, x64)}
     res["cg_speedup_half_graph_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
 
     # learned preconditioner: trained on (b, x) pairs from the plain FP64 CG with
@@ -355,7 +368,7 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
         lib.ostp_synchronize()
         t4 = time.perf_counter()
         res["cg_learned_fcg"] = {"seconds": round(t4 - t3, 4), "iterations": its, "fallback_iterations": fb,
-                                 "true_residual": trr, "status": st,
+                                 "true_residual": trr, "status": st, "err_vs_fp64": rel_err(x, x64),
                                  "training": {"pairs": args.learned_pairs, "pair_seconds": round(t1 - t0, 2),
                                               "steps": args.learned_steps, "train_seconds": round(t2 - t1, 2),
                                               "loss_first": round(float(hist[:5].mean()), 4),
@@ -364,7 +377,7 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
         res["cg_speedup_learned_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t4 - t3), 3)
         res["cg_iteration_ratio_learned_vs_fp64"] = round(res["cg_fp64"]["iterations"] / max(1, its + fb), 3)
     ost.check(0, "solves")
-    for p in [*d64, *d32, out, sf, sf32, x, u4f]:
+    for p in [*d64, *d32, out, sf, sf32, x, x64, out64, diff, u4f]:
         lib.ostp_free(p)
     return res
 

@@ -58,9 +58,15 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> a) {
         C o0, o1, o2;
         dslash_site<R, EO>(a.u, a.ph, a.in, a.g, idx, o0, o1, o2);
         if (MODE == kPlain) {
-            a.out[idx] = o0;
-            a.out[a.g.sizeh + idx] = o1;
-            a.out[2 * a.g.sizeh + idx] = o2;
+            if constexpr (sizeof(R) == 8) {
+                st_nt(a.out + idx, o0);
+                st_nt(a.out + a.g.sizeh + idx, o1);
+                st_nt(a.out + 2 * a.g.sizeh + idx, o2);
+            } else {
+                a.out[idx] = o0;
+                a.out[a.g.sizeh + idx] = o1;
+                a.out[2 * a.g.sizeh + idx] = o2;
+            }
         } else {
             // combine_in1xferm_mass_minus_in2: out = x*m2 - D in  (promoted to double)
             const C x0 = a.x[idx], x1 = a.x[a.g.sizeh + idx], x2 = a.x[2 * a.g.sizeh + idx];
@@ -68,9 +74,15 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> a) {
             const C s0 = mk<C>((double)x0.x * m2 - (double)o0.x, (double)x0.y * m2 - (double)o0.y);
             const C s1v = mk<C>((double)x1.x * m2 - (double)o1.x, (double)x1.y * m2 - (double)o1.y);
             const C s2v = mk<C>((double)x2.x * m2 - (double)o2.x, (double)x2.y * m2 - (double)o2.y);
-            a.out[idx] = s0;
-            a.out[a.g.sizeh + idx] = s1v;
-            a.out[2 * a.g.sizeh + idx] = s2v;
+            if constexpr (sizeof(R) == 8) {
+                st_nt(a.out + idx, s0);
+                st_nt(a.out + a.g.sizeh + idx, s1v);
+                st_nt(a.out + 2 * a.g.sizeh + idx, s2v);
+            } else {
+                a.out[idx] = s0;
+                a.out[a.g.sizeh + idx] = s1v;
+                a.out[2 * a.g.sizeh + idx] = s2v;
+            }
             if (MODE == kMdagMDot) dot = re_dot(x0, s0) + re_dot(x1, s1v) + re_dot(x2, s2v);
         }
     }
