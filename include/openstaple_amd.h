@@ -326,6 +326,23 @@ int ostp_batched_mixed_solve(int nsys, const ostp_batch_system *systems, double 
 int ostp_batched_half_solve(int nsys, const ostp_batch_system *systems, double res, int max_cg,
                             double mixed_delta, int iters_per_graph, int *cg_return, int *ru_return,
                             double *true_res);
+/* Rational solve with reduced-precision inner work: FP32 CG-M (FP32 Dslash and
+ * vectors, the FP64 shift recurrences of multishift_invert) with FP64
+ * reliable updates of the base residual (r = b - M x0 whenever |r|^2 has
+ * fallen by 0.1 since the last update; FP32 solution accumulators added to the
+ * FP64 solutions), then every shift refined, all together, by the batched
+ * solver (half_inner: FP16 storage, else FP32; FP64 reliable updates) from
+ * its CG-M solution.
+ * Buffers: ip.u, ip.u_f, ip.ferm_shift_temp_f (approx_order FP32 vectors),
+ * ip.loc_{r,h,s,p}_f, ip.out_f, ip.loc_{r,h,s,p} (FP64 work); pars->phases,
+ * pars->phases_f; out holds approx_order vec3_soa.  cg_multishift: CG-M
+ * iterations; cg_refine[k], true_res[k] per shift (may be NULL).  Returns the number of shifts whose FP64 true residual
+ * ||b - (M + RA_b[k]) x_k|| / ||b|| is <= res.  No reference counterpart (the
+ * reference's single-precision accelerated path is inverter_multishift_wrapper
+ * with singlePInvAccelMultiInv). */
+int ostp_multishift_mixed_solve(inverter_package ip, ferm_param *pars, RationalApprox *approx, vec3_soa *out,
+                                const vec3_soa *in, double res, int max_cg, int half_inner, int *cg_multishift,
+                                int *cg_refine, double *true_res);
 /* Text of the autotuner's measurements so far (candidates and choice per
  * batch signature); returns its length, copies up to len-1 bytes into buf. */
 int ostp_autotune_report(char *buf, int len);

@@ -206,6 +206,7 @@ _SIGS = {
     "convergence_messages": (None, [_I, _I]),
     "inverter_wrapper": (_I, [InverterPackage, _P, _P, _P, _D, _I, _D, _I]),
     "inverter_multishift_wrapper": (_I, [InverterPackage, _P, _P, _P, _P, _D, _I, _I]),
+    "ostp_multishift_mixed_solve": (_I, [InverterPackage, _P, _P, _P, _P, _D, _I, _I, _P, _P, _P]),
     "ostp_cg_solve": (_I, [_P, _P, _P, _P, _D, _P, _P, _P, _P, _I, _D, _I, C.POINTER(_I), C.POINTER(_D)]),
     "ostp_mixed_solve": (_I, [InverterPackage, _P, _P, _P, _D, _I, _D, _D, C.POINTER(_I), C.POINTER(_I),
                               C.POINTER(_D)]),

@@ -1137,6 +1137,11 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
     return converged;
 }
 
+int batched_solve_internal(int nsys, const ostp_batch_system* sys, double res, int max_cg, double mixed_delta,
+                           int iters_per_graph, int* cg_return, int* ru_return, double* true_res, bool half) {
+    return batched_solve(nsys, sys, res, max_cg, mixed_delta, iters_per_graph, cg_return, ru_return, true_res, half);
+}
+
 }  // namespace ostp
 
 extern "C" {

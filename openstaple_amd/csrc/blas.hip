@@ -150,6 +150,8 @@ bool launch_finalize(int fin, const SolverCtl& c, int nparts) {
         case kFinCgRestart: OSTP_F(kFinCgRestart, 1); break;
         case kFinMsAlpha: OSTP_F(kFinMsAlpha, 1); break;
         case kFinMsLambda: OSTP_F(kFinMsLambda, 1); break;
+        case kFinMxmAlpha: OSTP_F(kFinMxmAlpha, 1); break;
+        case kFinMxmLambda: OSTP_F(kFinMxmLambda, 1); break;
         case kFinMxAlpha: OSTP_F(kFinMxAlpha, 1); break;
         case kFinMxLambda: OSTP_F(kFinMxLambda, 1); break;
         case kFinFcgAlpha: OSTP_F(kFinFcgAlpha, 1); break;

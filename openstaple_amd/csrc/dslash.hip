@@ -136,7 +136,7 @@ bool launch_dslash(int eo, int mode, int fin, const void* u, const void* ph, con
     } else {
         if (eo != 0) return fail("fused dot epilogue only on Deo");
         if (fin != kFinStore && fin != kFinCgAlpha && fin != kFinMsAlpha && fin != kFinMxAlpha &&
-            fin != kFinFcgAlpha)
+            fin != kFinFcgAlpha && fin != kFinMxmAlpha)
             return fail("unsupported dslash finalizer %d", fin);
         OSTP_L(0, kMdagMDot);
     }

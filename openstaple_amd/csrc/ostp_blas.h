@@ -33,4 +33,9 @@ bool launch_dot(const void* a, const void* b, int complex_dot, const SolverCtl& 
 bool reduce_to_host(int kind, const void* a, const void* b, int single, double* re, double* im);
 SolverCtl plain_ctl();
 
+// Batched even-odd solves (batch.hip); the C entry points ostp_batched_mixed_solve
+// (half = false) and ostp_batched_half_solve (half = true) call it as well.
+int batched_solve_internal(int nsys, const ostp_batch_system* sys, double res, int max_cg, double mixed_delta,
+                           int iters_per_graph, int* cg_return, int* ru_return, double* true_res, bool half);
+
 }  // namespace ostp

@@ -24,6 +24,8 @@ enum Fin : int {
     kFinCgRestart,      // delta = |r|^2 at a (re)start          (inverter_full.c:60-66)
     kFinMsAlpha,        // CG-M alpha + shift recurrences     (inverter_multishift_full.c:116-136)
     kFinMsLambda,       // CG-M lambda, gammas, flags          (inverter_multishift_full.c:141-171)
+    kFinMxmAlpha,       // the same two for the FP32 CG-M with FP64 reliable updates: also decide
+    kFinMxmLambda,      //   whether the base residual is replaced (as kFinMxAlpha / kFinMxLambda)
     kFinMxAlpha,        // mixed: omega + reliable-update test (inverter_mixedp.c:103-114)
     kFinMxLambda,       // mixed: lambda, gammag, exit test     (inverter_mixedp.c:137-155)
     kFinStore2,         // st->sum, st->sum_im (complex dot)
@@ -93,7 +95,7 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
         const bool cont = (sqrt(lambda / st->source_norm) > st->res * st->safety) &&
                           st->cg_restarted < st->restart_every;
         if (!cont) st->status = kInnerExit;
-    } else if (F == kFinMsAlpha) {
+    } else if (F == kFinMsAlpha || F == kFinMxmAlpha) {
         // flags/maxiter computed at the end of the previous iteration become current
         st->flag_cur ^= 1;
         const int cur = st->flag_cur;
@@ -111,7 +113,11 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
                 st->omegas[k] = omega * ziii / zii;
             }
         }
-    } else if (F == kFinMsLambda) {
+        if (F == kFinMxmAlpha) {
+            if (st->last_max_res < st->delta) st->last_max_res = st->delta;
+            st->ru = (st->delta < st->mixed_delta * st->last_max_res) ? 1 : 0;
+        }
+    } else if (F == kFinMsLambda || F == kFinMxmLambda) {
         const double lambda = tot;
         const int cur = st->flag_cur, nxt = cur ^ 1;
         st->lambda = lambda;
@@ -137,6 +143,10 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
         st->delta = lambda;
         st->cg += 1;
         if (!(maxiter > 0 && st->cg < st->max_cg)) st->status = kDone;
+        if (F == kFinMxmLambda && st->ru) {
+            st->last_max_res = 0;
+            st->ru_count += 1;
+        }
     } else if (F == kFinFcgAlpha) {
         st->alpha = st->delta / tot;
         st->omega = st->alpha;  // the x/r update kernel reads omega

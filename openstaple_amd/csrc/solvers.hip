@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
+#include <vector>
 
 namespace ostp {
 
@@ -123,6 +124,68 @@ __global__ __launch_bounds__(kBlock) void k_ms_update(const C* __restrict__ r, C
         xk[i] = mkc<C>((double)xv.x - om * (double)pv.x, (double)xv.y - om * (double)pv.y);
         pk[i] = mkc<C>(gk * (double)pv.x + zk * (double)rr.x, gk * (double)pv.y + zk * (double)rr.y);
     }
+}
+
+// FP32 CG-M with FP64 reliable updates: r += omega s and, for the base
+// system's solution (needed for the FP64 residual), x0_f -= omega p; |r|^2
+__global__ __launch_bounds__(kBlock) void k_msx_r(float2* __restrict__ r, const float2* __restrict__ s,
+                                                  const float2* __restrict__ p, float2* __restrict__ x0f, int sizeh,
+                                                  SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const double om = ctl.st->omega;
+    double v = 0.0;
+    if (i < sizeh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = (long)c * sizeh + i;
+            const float2 rr = r[k], ss = s[k], pp = p[k], xx = x0f[k];
+            const float2 rn = make_float2((float)((double)rr.x + om * (double)ss.x), (float)((double)rr.y + om * (double)ss.y));
+            r[k] = rn;
+            x0f[k] = make_float2((float)((double)xx.x - om * (double)pp.x), (float)((double)xx.y - om * (double)pp.y));
+            v += (double)rn.x * rn.x + (double)rn.y * rn.y;
+        }
+    }
+    block_partial<1>(v, 0.0, ctl);
+}
+
+// x_k += x_k_f, x_k_f = 0 for the base system (slot 0) and the `order` shifts
+// (slots 1..order): the FP32 accumulators are moved into the FP64 solutions
+struct MsxFlush {
+    double2* x64[kMaxShifts + 1];
+    float2* x32[kMaxShifts + 1];
+    int n;
+};
+__global__ __launch_bounds__(kBlock) void k_msx_flush(MsxFlush f, long n3, SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n3) return;
+    for (int k = 0; k < f.n; k++) {
+        const float2 a = f.x32[k][i];
+        const double2 x = f.x64[k][i];
+        f.x64[k][i] = make_double2(x.x + (double)a.x, x.y + (double)a.y);
+        f.x32[k][i] = make_float2(0.f, 0.f);
+    }
+}
+
+// reliable update of the base residual: r64 = b - s64 (s64 = M x0 in FP64),
+// r_f = (float) r64, |r_f|^2 of the stored values
+__global__ __launch_bounds__(kBlock) void k_msx_resid(const double2* __restrict__ b, const double2* __restrict__ s,
+                                                      float2* __restrict__ rf, int sizeh, SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double v = 0.0;
+    if (i < sizeh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = (long)c * sizeh + i;
+            const double2 bb = b[k], ss = s[k];
+            const float2 r = make_float2((float)(bb.x - ss.x), (float)(bb.y - ss.y));
+            rf[k] = r;
+            v += (double)r.x * r.x + (double)r.y * r.y;
+        }
+    }
+    block_partial<1>(v, 0.0, ctl);
 }
 
 // out = a0*in + sum_k a_k x_k   (inverter_multishift_full.c:254-282)
@@ -763,6 +826,136 @@ int inverter_multishift_wrapper(inverter_package ip, ferm_param* pars, RationalA
         total_iterations += cg_return;
     }
     return total_iterations;
+}
+
+// Reduced-precision rational solve.  FP32 CG-M: FP32 Dslash and vectors, the
+// reference's FP64 shift recurrences (inverter_multishift_full.c), FP32
+// accumulators for the shifted solutions and for the base system's solution
+// x0.  Reliable updates (as in inverter_mixedp.c: the base |r|^2 has fallen
+// by mixed_delta since the last one): the accumulators are added to the FP64
+// solutions and the base residual is replaced by the FP64 b - M x0, so the
+// Krylov space is kept and the recurrences continue (the iteration count is
+// the FP64 CG-M's).  Shifted residuals are zeta_k r by construction and are
+// never re-synchronised, so the FP32 search directions leave errors in the
+// shifted solutions (relative residuals ~1e-8 at 32^4): every shift is then
+// refined by the batched solver, all together, from its CG-M solution.
+int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalApprox* approx, vec3_soa* out,
+                                const vec3_soa* in, double res, int max_cg, int half_inner, int* cg_multishift,
+                                int* cg_refine, double* true_res) {
+    const char* nm = "ostp_multishift_mixed_solve";
+    if (!require_init(nm)) return 0;
+    const int order = approx->approx_order;
+    if (order < 1 || order > kMaxShifts) {
+        fail("%s: approx_order %d", nm, order);
+        return 0;
+    }
+    const Lattice& L = ctx().lat;
+    const long sizeh = L.sizeh, n3 = 3 * sizeh;
+    const int nbh = blocks_for(sizeh), nb3 = blocks_for(n3);
+    const size_t vb = L.vec3_bytes(), vbf = L.vec3f_bytes();
+    void* bf = scratch(g_aux1_f, g_aux1_bytes, vbf);
+    void* xsf = scratch(g_shiftmulti_f, g_shiftmulti_bytes, (size_t)order * vbf);
+    const void* db = resolve(in, vb, nm);
+    void* dout = resolve(out, (size_t)order * vb, nm);
+    const void* u = resolve(ip.u, 8 * L.su3_bytes(), nm);
+    const void* uf = resolve(ip.u_f, 8 * L.su3f_bytes(), nm);
+    const void* ph = resolve(pars->phases, 8 * L.dsoa_bytes(), nm);
+    const void* phf = resolve(pars->phases_f, 8 * L.fsoa_bytes(), nm);
+    void* pk = resolve(ip.ferm_shift_temp_f, (size_t)order * vbf, nm);
+    void* r = resolve(ip.loc_r_f, vbf, nm);
+    void* h = resolve(ip.loc_h_f, vbf, nm);
+    void* sv = resolve(ip.loc_s_f, vbf, nm);
+    void* p = resolve(ip.loc_p_f, vbf, nm);
+    void* x0f = resolve(ip.out_f, vbf, nm);
+    void* x0 = resolve(ip.loc_p, vb, nm);
+    void* h64 = resolve(ip.loc_h, vb, nm);
+    void* s64 = resolve(ip.loc_s, vb, nm);
+    void* r64 = resolve(ip.loc_r, vb, nm);
+    if (!bf || !xsf || !db || !dout || !u || !uf || !ph || !phf || !pk || !r || !h || !sv || !p || !x0f || !x0 ||
+        !h64 || !s64 || !r64)
+        return 0;
+    const double m2 = pars->ferm_mass * pars->ferm_mass;
+
+    // x_k = x_k_f = x0 = x0_f = 0 ; r = p = p_k = (float) b
+    if (!launch_d2f(db, bf, n3)) return 0;
+    bool ok = hip_ok(hipMemsetAsync(dout, 0, (size_t)order * vb, S()), "zero x_k") &&
+              hip_ok(hipMemsetAsync(xsf, 0, (size_t)order * vbf, S()), "zero x_k_f") &&
+              hip_ok(hipMemsetAsync(x0, 0, vb, S()), "zero x0") && hip_ok(hipMemsetAsync(x0f, 0, vbf, S()), "zero x0_f") &&
+              hip_ok(hipMemcpyAsync(r, bf, vbf, hipMemcpyDeviceToDevice, S()), "r = b") &&
+              hip_ok(hipMemcpyAsync(p, bf, vbf, hipMemcpyDeviceToDevice, S()), "p = b");
+    for (int k = 0; ok && k < order; k++)
+        ok = hip_ok(hipMemcpyAsync((char*)pk + k * vbf, bf, vbf, hipMemcpyDeviceToDevice, S()), "p_k = b");
+    if (!ok) return 0;
+    double sn = 0.0;
+    if (!reduce_to_host(0, bf, nullptr, 1, &sn, nullptr)) return 0;
+    init_state(res, 0.95, max_cg, 0);
+    SolverState& s0 = H();
+    s0.delta = sn;
+    s0.source_norm = sn;
+    s0.omega = 1.0;
+    s0.gammag = 0.0;
+    s0.order = order;
+    s0.flag_cur = 0;
+    s0.mixed_delta = 0.1;
+    for (int k = 0; k < order; k++) {
+        s0.flags[1][k] = 1;
+        s0.zeta_i[k] = 1.0;
+        s0.zeta_ii[k] = 1.0;
+        s0.gammas[k] = 0.0;
+        s0.shifts[k] = approx->RA_b[k];
+    }
+    s0.maxiter_next = order;
+    if (!push_state()) return 0;
+
+    MsxFlush fl;
+    fl.n = order + 1;
+    fl.x64[0] = (double2*)x0;
+    fl.x32[0] = (float2*)x0f;
+    for (int k = 0; k < order; k++) {
+        fl.x64[k + 1] = (double2*)((char*)dout + k * vb);
+        fl.x32[k + 1] = (float2*)((char*)xsf + k * vbf);
+    }
+    auto iteration = [&](int j) -> bool {
+        if (!launch_mdagm<float>(uf, phf, p, h, sv, m2, 2, kFinMxmAlpha, ctl_of(kIfRunning))) return false;
+        hipLaunchKernelGGL(k_msx_r, dim3(nbh), dim3(kBlock), 0, S(), (float2*)r, (const float2*)sv,
+                           (const float2*)p, (float2*)x0f, (int)sizeh, ctl_of(kIfRunning));
+        // reliable update: accumulators into the FP64 solutions, r = b - M x0 in FP64
+        hipLaunchKernelGGL(k_msx_flush, dim3(nb3), dim3(kBlock), 0, S(), fl, n3, ctl_of(kIfRunningRU));
+        if (!launch_mdagm<double>(u, ph, x0, h64, s64, m2, 1, kFinStore, ctl_of(kIfRunningRU))) return false;
+        hipLaunchKernelGGL(k_msx_resid, dim3(nbh), dim3(kBlock), 0, S(), (const double2*)db, (const double2*)s64,
+                           (float2*)r, (int)sizeh, ctl_of(kIfRunningRU));
+        if (!launch_finalize(kFinMxmLambda, ctl_of(kIfRunning), nbh)) return false;
+        hipLaunchKernelGGL((k_ms_update<float2>), dim3(nb3), dim3(kBlock), 0, S(), (const float2*)r, (float2*)p,
+                           (float2*)xsf, (float2*)pk, n3, n3, ctl_of(kIfIter, j));
+        return check_launch("mixed multishift iteration");
+    };
+    if (!run_batches(iteration)) return 0;
+    const int cg = H().cg;
+    if (cg_multishift) *cg_multishift = cg;
+    hipLaunchKernelGGL(k_msx_flush, dim3(nb3), dim3(kBlock), 0, S(), fl, n3, ctl_of(kAlways));
+    if (!check_launch("flush")) return 0;
+
+    // every shift goes to the batched solver, which starts from the CG-M
+    // solution with the FP64 residual b - (M + shift_k) x_k (one that is already
+    // below res stops after its first inner iteration and reliable update)
+    std::vector<ostp_batch_system> sys(order);
+    for (int k = 0; k < order; k++) {
+        ostp_batch_system& q = sys[k];
+        for (int d = 0; d < 4; d++) q.nd[d] = L.nd[d];
+        q.par_temp = L.par_temp;
+        q.u = ip.u;
+        q.u_f = ip.u_f;
+        q.phases = pars->phases;
+        q.phases_f = pars->phases_f;
+        q.in = in;
+        q.out = (vec3_soa*)((char*)out + k * vb);
+        q.mass = std::sqrt(m2 + approx->RA_b[k]);
+    }
+    const int converged = batched_solve_internal(order, sys.data(), res, max_cg, 0.1, 16, cg_refine, nullptr,
+                                                 true_res, half_inner != 0);
+    if (verbosity_lv > 0)
+        printf("Terminated mixed multishift: %d CG-M iterations, %d of %d shifts converged\n", cg, converged, order);
+    return converged;
 }
 
 // Extended entry points returning the diagnostics the reference only prints.
