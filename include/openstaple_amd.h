@@ -347,6 +347,40 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param *pars, RationalA
  * batch signature); returns its length, copies up to len-1 bytes into buf. */
 int ostp_autotune_report(char *buf, int len);
 
+/* ============================== eigenvalue bounds (OpenAcc/find_min_max.h) */
+/* find_min_max.h:9-14: power iterations of M = m^2 - Deo Doe from loc_p (a
+ * Gaussian start vector, overwritten), until the norm changes by <= 1e-5
+ * relatively; returns the last norm.  Device-resident loop. */
+double ker_find_max_eigenvalue_openacc(const su3_soa *u, ferm_param *pars, vec3_soa *loc_r, vec3_soa *loc_h,
+                                       vec3_soa *loc_p);
+/* find_min_max.h:17-23: the same on (m^2 + (max - m^2) - m^2) - Deo Doe,
+ * returns max - norm (the reference's formula, left unused by its caller) */
+double ker_find_min_eigenvalue_openacc(const su3_soa *u, ferm_param *pars, vec3_soa *loc_r, vec3_soa *loc_h,
+                                       vec3_soa *loc_p, double max);
+/* find_min_max.h:26-32: minmax[0] = m^2, minmax[1] = max eigenvalue (loc_p1) */
+void find_min_max_eigenvalue_soloopenacc(const su3_soa *u, ferm_param *pars, vec3_soa *loc_r, vec3_soa *loc_h,
+                                         vec3_soa *loc_p1, vec3_soa *loc_p2, double *minmax);
+/* The first two with the iteration count (no reference counterpart). */
+double ostp_find_max_eigenvalue(const su3_soa *u, ferm_param *pars, vec3_soa *loc_r, vec3_soa *loc_h,
+                                vec3_soa *loc_p, int *iterations);
+double ostp_find_min_eigenvalue(const su3_soa *u, ferm_param *pars, vec3_soa *loc_r, vec3_soa *loc_h,
+                                vec3_soa *loc_p, double max, int *iterations);
+
+/* ====================== rational approximations (RationalApprox/rationalapprox.h) */
+/* Host only.  rationalapprox.h:18-20: file names (malloc'ed, caller frees) */
+char *rational_approx_filename(double error, int exponent_num, int exponent_den, double lambda_min);
+char *rational_approx_filename_old(int approx_order, int exponent_num, int exponent_den, double lambda_min);
+/* rationalapprox.h:23-24: read a .REMEZ file (0 ok, 1 missing, 2 malformed) */
+int rationalapprox_read(RationalApprox *rational_approx);
+int rationalapprox_read_custom_nomefile(RationalApprox *rational_approx, char *nomefile);
+/* rationalapprox.h:27 */
+void rationalapprox_save(const char *nomefile, RationalApprox *rational_approx);
+/* rationalapprox.h:29-33: rescale to [0.95 min, 1.05 max]; renormalise to
+ * lambda_max = 1; evaluate a0 + sum_k a_k / (x + b_k) */
+void rescale_rational_approximation(RationalApprox *in, RationalApprox *out, double *minmax);
+void renormalize_rational_approximation(RationalApprox *in, RationalApprox *out);
+double rational_approx_evaluate(RationalApprox *in, double x);
+
 /* ================================== learned preconditioner (new) ======= */
 /* No reference counterpart.  A gauge-equivariant network of two
  * parallel-transport layers with OSTP_PTC_CHANNELS channels approximating

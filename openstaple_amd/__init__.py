@@ -207,6 +207,19 @@ _SIGS = {
     "inverter_wrapper": (_I, [InverterPackage, _P, _P, _P, _D, _I, _D, _I]),
     "inverter_multishift_wrapper": (_I, [InverterPackage, _P, _P, _P, _P, _D, _I, _I]),
     "ostp_multishift_mixed_solve": (_I, [InverterPackage, _P, _P, _P, _P, _D, _I, _I, _P, _P, _P]),
+    "ker_find_max_eigenvalue_openacc": (_D, [_P, _P, _P, _P, _P]),
+    "ker_find_min_eigenvalue_openacc": (_D, [_P, _P, _P, _P, _P, _D]),
+    "find_min_max_eigenvalue_soloopenacc": (None, [_P, _P, _P, _P, _P, _P, _P]),
+    "ostp_find_max_eigenvalue": (_D, [_P, _P, _P, _P, _P, _P]),
+    "ostp_find_min_eigenvalue": (_D, [_P, _P, _P, _P, _P, _D, _P]),
+    "rational_approx_filename": (_P, [_D, _I, _I, _D]),
+    "rational_approx_filename_old": (_P, [_I, _I, _I, _D]),
+    "rationalapprox_read": (_I, [_P]),
+    "rationalapprox_read_custom_nomefile": (_I, [_P, C.c_char_p]),
+    "rationalapprox_save": (None, [C.c_char_p, _P]),
+    "rescale_rational_approximation": (None, [_P, _P, _P]),
+    "renormalize_rational_approximation": (None, [_P, _P]),
+    "rational_approx_evaluate": (_D, [_P, _D]),
     "ostp_cg_solve": (_I, [_P, _P, _P, _P, _D, _P, _P, _P, _P, _I, _D, _I, C.POINTER(_I), C.POINTER(_D)]),
     "ostp_mixed_solve": (_I, [InverterPackage, _P, _P, _P, _D, _I, _D, _D, C.POINTER(_I), C.POINTER(_I),
                               C.POINTER(_D)]),

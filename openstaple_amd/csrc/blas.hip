@@ -126,6 +126,7 @@ bool launch_norm2(const void* a, const SolverCtl& ctl) {
     return hip_ok(hipGetLastError(), "k_reduce") && launch_finalize(F, ctl, nb(sizeh));
 }
 template bool launch_norm2<double2, kFinStore>(const void*, const SolverCtl&);
+template bool launch_norm2<double2, kFinPower>(const void*, const SolverCtl&);
 template bool launch_norm2<float2, kFinStore>(const void*, const SolverCtl&);
 
 bool launch_dot(const void* a, const void* b, int complex_dot, const SolverCtl& ctl) {
@@ -151,6 +152,7 @@ bool launch_finalize(int fin, const SolverCtl& c, int nparts) {
         case kFinMsAlpha: OSTP_F(kFinMsAlpha, 1); break;
         case kFinMsLambda: OSTP_F(kFinMsLambda, 1); break;
         case kFinMxmAlpha: OSTP_F(kFinMxmAlpha, 1); break;
+        case kFinPower: OSTP_F(kFinPower, 1); break;
         case kFinMxmLambda: OSTP_F(kFinMxmLambda, 1); break;
         case kFinMxAlpha: OSTP_F(kFinMxAlpha, 1); break;
         case kFinMxLambda: OSTP_F(kFinMxLambda, 1); break;

@@ -24,6 +24,8 @@ enum Fin : int {
     kFinCgRestart,      // delta = |r|^2 at a (re)start          (inverter_full.c:60-66)
     kFinMsAlpha,        // CG-M alpha + shift recurrences     (inverter_multishift_full.c:116-136)
     kFinMsLambda,       // CG-M lambda, gammas, flags          (inverter_multishift_full.c:141-171)
+    kFinPower,          // power method: norm = sqrt(|p|^2), stop at relative change <= 1e-5
+                        //   (find_min_max.c:21-58)
     kFinMxmAlpha,       // the same two for the FP32 CG-M with FP64 reliable updates: also decide
     kFinMxmLambda,      //   whether the base residual is replaced (as kFinMxAlpha / kFinMxLambda)
     kFinMxAlpha,        // mixed: omega + reliable-update test (inverter_mixedp.c:103-114)
@@ -147,6 +149,14 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
             st->last_max_res = 0;
             st->ru_count += 1;
         }
+    } else if (F == kFinPower) {
+        // lambda holds the previous norm
+        const double norm = sqrt(tot);
+        double change = fabs(st->lambda - norm);
+        change /= norm;
+        st->lambda = norm;
+        st->cg += 1;
+        if (!(change > 1.0e-5) || st->cg >= st->max_cg) st->status = kDone;
     } else if (F == kFinFcgAlpha) {
         st->alpha = st->delta / tot;
         st->omega = st->alpha;  // the x/r update kernel reads omega

@@ -188,6 +188,18 @@ __global__ __launch_bounds__(kBlock) void k_msx_resid(const double2* __restrict_
     block_partial<1>(v, 0.0, ctl);
 }
 
+// power method step (find_min_max.c:35-40): r = p / norm (the reference
+// scales p in place and copies it to r; p is overwritten next by M r)
+__global__ __launch_bounds__(kBlock) void k_pm_scale(const double2* __restrict__ p, double2* __restrict__ r, long n3,
+                                                     SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n3) return;
+    const double inorm = 1.0 / ctl.st->lambda;
+    const double2 v = p[i];
+    r[i] = make_double2(inorm * v.x, inorm * v.y);
+}
+
 // out = a0*in + sum_k a_k x_k   (inverter_multishift_full.c:254-282)
 struct RecombineCoeffs {
     double a0;
@@ -956,6 +968,84 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
     if (verbosity_lv > 0)
         printf("Terminated mixed multishift: %d CG-M iterations, %d of %d shifts converged\n", cg, converged, order);
     return converged;
+}
+
+// Power iterations with the operator coef - Deo Doe (find_min_max.c:21-97):
+// scalars on the device, iterations queued in batches like the CG.  Returns
+// the last norm (NaN on failure), the iteration count in *iters.
+static double power_method(const void* u, const void* ph, double coef, void* r, void* h, void* p, int* iters) {
+    const long n3 = 3 * ctx().lat.sizeh;
+    double sn = 0.0;
+    if (!reduce_to_host(0, p, nullptr, 0, &sn, nullptr)) return NAN;
+    init_state(0.0, 0.0, 1000000, 0);  // the reference has no iteration limit
+    H().lambda = std::sqrt(sn);
+    if (!push_state()) return NAN;
+    auto iteration = [&](int) -> bool {
+        hipLaunchKernelGGL(k_pm_scale, dim3(blocks_for(n3)), dim3(kBlock), 0, S(), (const double2*)p, (double2*)r, n3,
+                           ctl_of(kIfRunning));
+        if (!launch_mdagm<double>(u, ph, r, h, p, coef, 1, kFinStore, ctl_of(kIfRunning))) return false;  // p = coef r - Deo Doe r
+        return launch_norm2<double2, kFinPower>(p, ctl_of(kIfRunning)) && check_launch("power method");
+    };
+    if (!run_batches(iteration)) return NAN;
+    if (iters) *iters = H().cg;
+    return H().lambda;
+}
+
+static bool power_buffers(const char* nm, const su3_soa* u, ferm_param* pars, vec3_soa* loc_r, vec3_soa* loc_h,
+                          vec3_soa* loc_p, const void** du, const void** dph, void** dr, void** dh, void** dp) {
+    const Lattice& L = ctx().lat;
+    *du = resolve(u, 8 * L.su3_bytes(), nm);
+    *dph = resolve(pars->phases, 8 * L.dsoa_bytes(), nm);
+    *dr = resolve(loc_r, L.vec3_bytes(), nm);
+    *dh = resolve(loc_h, L.vec3_bytes(), nm);
+    *dp = resolve(loc_p, L.vec3_bytes(), nm);
+    return *du && *dph && *dr && *dh && *dp;
+}
+
+double ostp_find_max_eigenvalue(const su3_soa* u, ferm_param* pars, vec3_soa* loc_r, vec3_soa* loc_h,
+                                vec3_soa* loc_p, int* iterations) {
+    const char* nm = "ker_find_max_eigenvalue_openacc";
+    if (!require_init(nm)) return NAN;
+    const void *du, *dph;
+    void *dr, *dh, *dp;
+    if (!power_buffers(nm, u, pars, loc_r, loc_h, loc_p, &du, &dph, &dr, &dh, &dp)) return NAN;
+    // fermion_matrix_multiplication: m^2 in - Deo Doe in
+    return power_method(du, dph, pars->ferm_mass * pars->ferm_mass, dr, dh, dp, iterations);
+}
+
+double ostp_find_min_eigenvalue(const su3_soa* u, ferm_param* pars, vec3_soa* loc_r, vec3_soa* loc_h,
+                                vec3_soa* loc_p, double max, int* iterations) {
+    const char* nm = "ker_find_min_eigenvalue_openacc";
+    if (!require_init(nm)) return NAN;
+    const void *du, *dph;
+    void *dr, *dh, *dp;
+    if (!power_buffers(nm, u, pars, loc_r, loc_h, loc_p, &du, &dph, &dr, &dh, &dp)) return NAN;
+    // fermion_matrix_multiplication_shifted with shift = delta - m^2, delta = max - m^2
+    const double m2 = pars->ferm_mass * pars->ferm_mass;
+    const double delta = max - m2;
+    const double norm = power_method(du, dph, m2 + (delta - m2), dr, dh, dp, iterations);
+    return max - norm;
+}
+
+double ker_find_max_eigenvalue_openacc(const su3_soa* u, ferm_param* pars, vec3_soa* loc_r, vec3_soa* loc_h,
+                                       vec3_soa* loc_p) {
+    return ostp_find_max_eigenvalue(u, pars, loc_r, loc_h, loc_p, nullptr);
+}
+
+double ker_find_min_eigenvalue_openacc(const su3_soa* u, ferm_param* pars, vec3_soa* loc_r, vec3_soa* loc_h,
+                                       vec3_soa* loc_p, double max) {
+    return ostp_find_min_eigenvalue(u, pars, loc_r, loc_h, loc_p, max, nullptr);
+}
+
+// find_min_max.c:100-121: min is m^2 (the power iteration for the minimum is
+// left unused there), max from the power method on loc_p1
+void find_min_max_eigenvalue_soloopenacc(const su3_soa* u, ferm_param* pars, vec3_soa* loc_r, vec3_soa* loc_h,
+                                         vec3_soa* loc_p1, vec3_soa* loc_p2, double* minmax) {
+    (void)loc_p2;
+    minmax[0] = pars->ferm_mass * pars->ferm_mass;
+    minmax[1] = ker_find_max_eigenvalue_openacc(u, pars, loc_r, loc_h, loc_p1);
+    if (verbosity_lv > 3)
+        printf("    Computed the min and max eigs : [ min= %.18lf; max= %.18lf ]  \n", minmax[0], minmax[1]);
 }
 
 // Extended entry points returning the diagnostics the reference only prints.

@@ -778,3 +778,49 @@ int or_eo_inversion(const or_geom *g, const void *u, const double *phases, doubl
     for (i = 0; i < n3; i++) oo[i] = f * (io[i] - po[i]);    /* fermionic_utilities.c:413-415 */
     return st;
 }
+
+/* ker_find_max_eigenvalue_openacc (OpenAcc/find_min_max.c:21-58): power
+ * iterations of M = m^2 - Deo Doe from the vector in loc_p (overwritten),
+ * stopping when the norm changes by <= 1e-5 relatively; returns the last
+ * norm, the iteration count in *iters.  shift != 0 is the min-eigenvalue
+ * variant's operator (find_min_max.c:61-97, fermion_matrix_multiplication_shifted). */
+static double power_loop(const or_geom *g, const void *u, const double *phases, double mass, double shift,
+                         void *loc_r, void *loc_h, void *loc_p, int *iters)
+{
+    const long n3 = 3 * g->sizeh;
+    or_dc *p = (or_dc *)loc_p, *r = (or_dc *)loc_r;
+    double norm = sqrt(or_l2norm2(g, loc_p)), old_norm;
+    int loop_count = 0;
+    do {
+        const double inorm = 1.0 / norm;
+        long i;
+        for (i = 0; i < n3; i++) p[i] = inorm * p[i];   /* multiply_fermion_x_doublefactor */
+        for (i = 0; i < n3; i++) r[i] = p[i];           /* assign_in_to_out */
+        old_norm = norm;
+        or_fermion_matrix_multiplication_shifted(g, u, loc_p, loc_r, loc_h, phases, mass, shift);
+        norm = sqrt(or_l2norm2(g, loc_p));
+        old_norm = fabs(old_norm - norm);
+        old_norm /= norm;
+        loop_count++;
+    } while (old_norm > 1.0e-5);
+    if (iters) *iters = loop_count;
+    return norm;
+}
+
+double or_find_max_eigenvalue(const or_geom *g, const void *u, const double *phases, double mass, void *loc_r,
+                              void *loc_h, void *loc_p, int *iters)
+{
+    return power_loop(g, u, phases, mass, 0.0, loc_r, loc_h, loc_p, iters);
+}
+
+/* ker_find_min_eigenvalue_openacc (find_min_max.c:61-97): the same loop on
+ * (m^2 + (max - m^2) - m^2) - Deo Doe, min = max - norm */
+double or_find_min_eigenvalue(const or_geom *g, const void *u, const double *phases, double mass, void *loc_r,
+                              void *loc_h, void *loc_p, double max, int *iters)
+{
+    const double m2 = mass * mass;
+    const double delta = max - m2;
+    const double norm = power_loop(g, u, phases, mass, delta - m2, loc_r, loc_h, loc_p, iters);
+    return max - norm;
+}
+

@@ -134,6 +134,11 @@ void or_ratapprox_rescale(const or_ratapprox *in, or_ratapprox *out, const doubl
 
 /* eo_inversion (Meas/ferm_meas.c:50-71): full-matrix solve through the
  * even-odd CG; returns the CG status */
+/* power method of find_min_max.c:21-97 (loc_p: start vector, overwritten) */
+double or_find_max_eigenvalue(const or_geom *g, const void *u, const double *phases, double mass, void *loc_r,
+                              void *loc_h, void *loc_p, int *iters);
+double or_find_min_eigenvalue(const or_geom *g, const void *u, const double *phases, double mass, void *loc_r,
+                              void *loc_h, void *loc_p, double max, int *iters);
 int or_eo_inversion(const or_geom *g, const void *u, const double *phases, double mass, double res, int max_cg,
                     const void *in_e, const void *in_o, void *out_e, void *out_o, void *phi_e, void *phi_o,
                     void *r, void *h, void *s, void *p, const or_tricks *tr, int *cg_return);

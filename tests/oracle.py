@@ -71,6 +71,8 @@ for name, res, args in [
     ("or_gen_vec3_z2", None, [_G, _P, C.c_ulonglong, C.c_uint]),
     ("or_gen_conf_cold", None, [_G, _P, _D, C.c_ulonglong]),
     ("or_gen_conf_expH", None, [_G, _P, _D, C.c_ulonglong]),
+    ("or_find_max_eigenvalue", _D, [_G, _P, _P, _D, _P, _P, _P, C.POINTER(_I)]),
+    ("or_find_min_eigenvalue", _D, [_G, _P, _P, _D, _P, _P, _P, _D, C.POINTER(_I)]),
     ("or_ratapprox_read", _I, [C.POINTER(RatApprox), C.c_char_p]),
     ("or_ratapprox_evaluate", _D, [C.POINTER(RatApprox), _D]),
     ("or_ratapprox_rescale", None, [C.POINTER(RatApprox), C.POINTER(RatApprox), _P]),
@@ -153,6 +155,19 @@ class Oracle:
         st = lib.or_ker_invert(self.gp, p(u), p(ph), mass, p(x), p(b), res, p(r), p(h), p(s), p(pp), max_cg, shift,
                                C.byref(tr), C.byref(cg))
         return st, cg.value, x
+
+    def find_max_eigenvalue(self, u, ph, mass, start):
+        """Power method of find_min_max.c:21-58 from `start`; (max, iterations)."""
+        r, h, pp = self.vec3(), self.vec3(), start.copy()
+        it = C.c_int(0)
+        v = lib.or_find_max_eigenvalue(self.gp, p(u), p(ph), mass, p(r), p(h), p(pp), C.byref(it))
+        return v, it.value
+
+    def find_min_eigenvalue(self, u, ph, mass, start, vmax):
+        r, h, pp = self.vec3(), self.vec3(), start.copy()
+        it = C.c_int(0)
+        v = lib.or_find_min_eigenvalue(self.gp, p(u), p(ph), mass, p(r), p(h), p(pp), vmax, C.byref(it))
+        return v, it.value
 
     def eo_inversion(self, u, ph, mass, b_e, b_o, res, max_cg=10000):
         x_e, x_o, phi_e, phi_o, r, h, s, pp = (self.vec3() for _ in range(8))

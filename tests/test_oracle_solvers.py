@@ -141,3 +141,50 @@ def test_remez_headerless_files_are_rejected(path):
     Order/Lambda header (rationalapprox.c:9-13,93-99)."""
     with pytest.raises(ValueError):
         read_remez(path)
+
+
+@pytest.fixture(scope="module")
+def dense44():
+    """M = m^2 - Deo Doe on a 4^4 expH field as a dense 384 x 384 matrix."""
+    o = Oracle(4, 4, 4, 4)
+    u = o.gen_expH(0.3, 1)
+    ph = o.phases()
+    mass = 0.05
+    n = 3 * o.sizeh
+    M = np.zeros((n, n), dtype=np.complex128)
+    for j in range(n):
+        e = np.zeros(n, dtype=np.complex128)
+        e[j] = 1.0
+        M[:, j] = o.mdagm(u, e.reshape(3, -1), ph, mass).ravel()
+    w, V = np.linalg.eigh((M + M.conj().T) / 2)
+    return o, u, ph, mass, M, w, V
+
+
+def test_power_method_max_eigenvalue(dense44):
+    """ker_find_max_eigenvalue_openacc restated: from a Gaussian start the
+    estimate approaches lambda_max from below (||M r|| <= lambda_max for unit r)
+    and stops within the 1e-5 change criterion; from the exact top eigenvector
+    it returns lambda_max after two iterations (the second sees no change)."""
+    o, u, ph, mass, M, w, V = dense44
+    assert np.max(np.abs(M - M.conj().T)) < 1e-14
+    b = o.gen_gauss(3)
+    v, it = o.find_max_eigenvalue(u, ph, mass, b)
+    assert w[-1] * (1 - 1e-3) <= v <= w[-1] * (1 + 1e-12) and it > 10
+    top = np.ascontiguousarray(V[:, -1].reshape(3, -1))
+    v2, it2 = o.find_max_eigenvalue(u, ph, mass, top)
+    assert abs(v2 - w[-1]) < 1e-12 * w[-1] and it2 == 2
+    # the mass term is in the operator: the same with m = 0.3 shifts lambda_max by 0.3^2 - 0.05^2
+    v3, _ = o.find_max_eigenvalue(u, ph, 0.3, top)
+    assert abs(v3 - (w[-1] + 0.09 - 0.0025)) < 1e-12 * w[-1]
+
+
+def test_power_method_min_formula(dense44):
+    """ker_find_min_eigenvalue_openacc restated: its operator is
+    (m^2 + (max - m^2) - m^2) - Deo Doe = M + (max - 2 m^2), whose top eigenvalue
+    is lambda_max + max - 2 m^2, so it returns 2 m^2 - lambda_max (the caller,
+    find_min_max_eigenvalue_soloopenacc, uses m^2 instead)."""
+    o, u, ph, mass, M, w, V = dense44
+    top = np.ascontiguousarray(V[:, -1].reshape(3, -1))
+    vmax = w[-1]
+    vmin, it = o.find_min_eigenvalue(u, ph, mass, top, vmax)
+    assert abs(vmin - (2 * mass * mass - w[-1])) < 1e-11 * w[-1] and it == 2
