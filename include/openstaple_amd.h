@@ -347,6 +347,20 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param *pars, RationalA
  * batch signature); returns its length, copies up to len-1 bytes into buf. */
 int ostp_autotune_report(char *buf, int len);
 
+/* ===================== fermion measurements (Meas/ferm_meas.c) ========== */
+/* Z2-noise chiral condensate, ferm_meas.c:282-320 for nvec noise vectors:
+ * noise k is ostp_generate_vec3_soa_z2noise(seed, stream 2k) on even sites and
+ * (seed, 2k+1) on odd sites; M^-1 eta by eo_inversion's even-odd split with
+ * the even systems solved together by the batched solver (half_inner: FP16
+ * storage, else FP32; FP64 reliable updates and true residual); cond[2k],
+ * cond[2k+1] = Re, Im of (<eta_e, x_e> + <eta_o, x_o>) degeneracy 0.25 / V.
+ * cg_return[k] may be NULL.  Returns the number of converged solves.  No
+ * single reference counterpart (fermion_measures does this per vector and
+ * writes it to its output file). */
+int ostp_chiral_condensate(const su3_soa *u, const su3_soa_f *u_f, ferm_param *pars, int nvec, uint64_t seed,
+                           double degeneracy, double res, int max_cg, int half_inner, double *cond,
+                           int *cg_return);
+
 /* ============================== eigenvalue bounds (OpenAcc/find_min_max.h) */
 /* find_min_max.h:9-14: power iterations of M = m^2 - Deo Doe from loc_p (a
  * Gaussian start vector, overwritten), until the norm changes by <= 1e-5

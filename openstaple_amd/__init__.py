@@ -207,6 +207,7 @@ _SIGS = {
     "inverter_wrapper": (_I, [InverterPackage, _P, _P, _P, _D, _I, _D, _I]),
     "inverter_multishift_wrapper": (_I, [InverterPackage, _P, _P, _P, _P, _D, _I, _I]),
     "ostp_multishift_mixed_solve": (_I, [InverterPackage, _P, _P, _P, _P, _D, _I, _I, _P, _P, _P]),
+    "ostp_chiral_condensate": (_I, [_P, _P, _P, _I, C.c_uint64, _D, _D, _I, _I, _P, _P]),
     "ker_find_max_eigenvalue_openacc": (_D, [_P, _P, _P, _P, _P]),
     "ker_find_min_eigenvalue_openacc": (_D, [_P, _P, _P, _P, _P, _D]),
     "find_min_max_eigenvalue_soloopenacc": (None, [_P, _P, _P, _P, _P, _P, _P]),

@@ -783,6 +783,63 @@ void eo_inversion(inverter_package ip, ferm_param* tfermions_parameters, double 
     combine_in1_minus_in2_allxfact(in_o, phi_o, (double)1 / m, out_o);
 }
 
+// Z2-noise chiral condensate (Meas/ferm_meas.c:282-320) for nvec noise vectors
+// at once: eo_inversion's even-odd split for every vector, the even systems
+// (m^2 - Deo Doe) x_e = m eta_e - Deo eta_o solved together by the batched
+// solver, x_o = (eta_o - Doe x_e) / m, and per vector
+// (<eta_e, x_e> + <eta_o, x_o>) * degeneracy * 0.25 / V.
+int ostp_chiral_condensate(const su3_soa* u, const su3_soa_f* u_f, ferm_param* pars, int nvec, uint64_t seed,
+                           double degeneracy, double res, int max_cg, int half_inner, double* cond,
+                           int* cg_return) {
+    const char* nm = "ostp_chiral_condensate";
+    if (!require_init(nm)) return 0;
+    if (nvec < 1 || !pars || !cond) {
+        fail("%s: invalid arguments", nm);
+        return 0;
+    }
+    const Lattice& L = ctx().lat;
+    const size_t vb = L.vec3_bytes();
+    // per vector: eta_e, eta_o, phi (source, then temporary), x_e, x_o
+    char* buf = nullptr;
+    if (!hip_ok(hipMalloc(&buf, 5 * vb * (size_t)nvec), "chiral condensate vectors")) return 0;
+    auto vec = [&](int k, int j) { return (vec3_soa*)(buf + ((size_t)k * 5 + j) * vb); };
+    const double m = pars->ferm_mass;
+    std::vector<ostp_batch_system> sys(nvec);
+    for (int k = 0; k < nvec; k++) {
+        vec3_soa *ee = vec(k, 0), *eo = vec(k, 1), *phi = vec(k, 2), *xe = vec(k, 3);
+        ostp_generate_vec3_soa_z2noise(ee, seed, 2u * (uint32_t)k);
+        ostp_generate_vec3_soa_z2noise(eo, seed, 2u * (uint32_t)k + 1u);
+        acc_Deo(u, phi, eo, pars->phases);
+        combine_in1_x_fact1_minus_in2_back_into_in2(ee, m, phi);
+        set_vec3_soa_to_zero(xe);
+        ostp_batch_system& q = sys[k];
+        for (int d = 0; d < 4; d++) q.nd[d] = L.nd[d];
+        q.par_temp = L.par_temp;
+        q.u = u;
+        q.u_f = u_f;
+        q.phases = pars->phases;
+        q.phases_f = pars->phases_f;
+        q.in = phi;
+        q.out = xe;
+        q.mass = m;
+    }
+    const int conv = batched_solve_internal(nvec, sys.data(), res, max_cg, 0.1, 16, cg_return, nullptr, nullptr,
+                                            half_inner != 0);
+    const double factor = degeneracy * 0.25 / ((double)L.nd[0] * L.nd[1] * L.nd[2] * L.nd[3]);
+    bool ok = true;
+    for (int k = 0; ok && k < nvec; k++) {
+        vec3_soa *ee = vec(k, 0), *eo = vec(k, 1), *tmp = vec(k, 2), *xe = vec(k, 3), *xo = vec(k, 4);
+        acc_Doe(u, tmp, xe, pars->phases);
+        combine_in1_minus_in2_allxfact(eo, tmp, (double)1 / m, xo);
+        double re_e = 0, im_e = 0, re_o = 0, im_o = 0;
+        ok = reduce_to_host(2, ee, xe, 0, &re_e, &im_e) && reduce_to_host(2, eo, xo, 0, &re_o, &im_o);
+        cond[2 * k] = (re_e + re_o) * factor;
+        cond[2 * k + 1] = (im_e + im_o) * factor;
+    }
+    (void)hipFree(buf);
+    return ok ? conv : 0;
+}
+
 int inverter_multishift_wrapper(inverter_package ip, ferm_param* pars, RationalApprox* approx, vec3_soa* out,
                                 const vec3_soa* in, double res, int max_cg, int convergence_importance) {
     const char* nm = "inverter_multishift_wrapper";

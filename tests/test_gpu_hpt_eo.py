@@ -115,3 +115,32 @@ def test_eo_inversion(shape, mass, mixed):
     scale = np.sqrt(np.linalg.norm(oe) ** 2 + np.linalg.norm(oo) ** 2)
     diff = np.sqrt(np.linalg.norm(ge - oe) ** 2 + np.linalg.norm(go - oo) ** 2)
     assert diff / scale <= 100 * res / mass
+
+
+@pytest.mark.parametrize("shape,mass,nvec,half", [((8, 8, 8, 8), 0.05, 4, 0), ((8, 8, 8, 8), 0.05, 4, 1),
+                                                  ((4, 6, 8, 10), 0.1, 3, 1)])
+def test_chiral_condensate_vs_oracle(shape, mass, nvec, half):
+    """ostp_chiral_condensate (ferm_meas.c:282-320 over nvec Z2 vectors, the even
+    systems solved together by the batched solver) against the oracle's
+    eo_inversion and scalar products on the same noise (generator streams 2k,
+    2k+1), to the precision the 1e-10 solves allow."""
+    o = Oracle(*shape)
+    init_lattice(shape)
+    u = o.gen_expH(0.3, 1)
+    ph = o.phases()
+    du, dph = Dev(u), Dev(ph)
+    duf, dphf = Dev(o.to_float_gauge(u)), Dev(ph.astype(np.float32))
+    fp = ost.ferm_param(mass, dph.p, dphf.p)
+    seed, deg = 31, 1.0
+    cond = (C.c_double * (2 * nvec))()
+    cg = (C.c_int * nvec)()
+    conv = lib.ostp_chiral_condensate(du.p, duf.p, C.byref(fp), nvec, seed, deg, 1e-10, 10000, half, cond, cg)
+    ost.check(0, "ostp_chiral_condensate")
+    assert conv == nvec
+    vol = float(np.prod(shape))
+    for k in range(nvec):
+        be, bo = o.gen_z2(seed, 2 * k), o.gen_z2(seed, 2 * k + 1)
+        st, _, xe, xo = o.eo_inversion(u, ph, mass, be, bo, 1e-12)
+        expect = (np.vdot(be, xe) + np.vdot(bo, xo)) * deg * 0.25 / vol
+        got = cond[2 * k] + 1j * cond[2 * k + 1]
+        assert abs(got - expect) <= 1e-7 * abs(expect), (k, got, expect)

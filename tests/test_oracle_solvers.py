@@ -188,3 +188,38 @@ def test_power_method_min_formula(dense44):
     vmax = w[-1]
     vmin, it = o.find_min_eigenvalue(u, ph, mass, top, vmax)
     assert abs(vmin - (2 * mass * mass - w[-1])) < 1e-11 * w[-1] and it == 2
+
+
+def test_eo_inversion_condensate_vs_dense_inverse():
+    """The chiral-condensate estimator eta^dag M^-1 eta (ferm_meas.c:282-307)
+    through the oracle's eo_inversion equals the one from the dense inverse of
+    the full staggered matrix M = m + D (D = Deo on odd -> even, Doe on even
+    -> odd) at 4^4; averaged over Z2 noise it estimates tr M^-1."""
+    o = Oracle(4, 4, 4, 4)
+    u = o.gen_expH(0.3, 1)
+    ph = o.phases()
+    mass = 0.1
+    n = 3 * o.sizeh
+    D = np.zeros((2 * n, 2 * n), dtype=np.complex128)
+    for j in range(n):
+        e = np.zeros((3, o.sizeh), dtype=np.complex128)
+        e.reshape(-1)[j] = 1.0
+        D[:n, n + j] = o.deo(u, e, ph).ravel()   # odd input j -> even output
+        D[n:, j] = o.doe(u, e, ph).ravel()       # even input j -> odd output
+    M = mass * np.eye(2 * n) + D
+    Minv = np.linalg.inv(M)
+    ests = []
+    for k in range(64):
+        be, bo = o.gen_z2(5, 2 * k), o.gen_z2(5, 2 * k + 1)
+        st, _, xe, xo = o.eo_inversion(u, ph, mass, be, bo, 1e-13)
+        eta = np.concatenate([be.ravel(), bo.ravel()])
+        est = np.vdot(be, xe) + np.vdot(bo, xo)
+        exact = np.vdot(eta, Minv @ eta)
+        assert st == 1 and abs(est - exact) <= 1e-9 * abs(exact)
+        ests.append(est)
+    ests = np.array(ests)
+    tr = np.trace(Minv)
+    # Z2 noise: unbiased estimator of the trace; 64 samples, 5 sigma
+    assert abs(ests.mean() - tr) <= 5 * ests.std() / np.sqrt(len(ests)), (ests.mean(), tr)
+    # D is anti-Hermitian, so Re tr M^-1 = sum m / (m^2 + |lambda|^2) > 0 and Im tr M^-1 = 0
+    assert tr.real > 0 and abs(tr.imag) < 1e-10 * abs(tr)
