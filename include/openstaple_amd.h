@@ -37,6 +37,7 @@ typedef struct su3_soa_t su3_soa;           /* struct_c_def.h:35-42 */
 typedef struct double_soa_t double_soa;     /* struct_c_def.h:25-27 */
 typedef struct tamat_soa_t tamat_soa;       /* struct_c_def.h:45-51 */
 typedef struct thmat_soa_t thmat_soa;       /* struct_c_def.h:52-58 */
+typedef struct dcomplex_soa_t dcomplex_soa; /* struct_c_def.h:21-23 */
 typedef struct vec3_soa_f_t vec3_soa_f;     /* generated sp_struct_c_def.h twin of vec3_soa */
 typedef struct su3_soa_f_t su3_soa_f;       /* generated twin of su3_soa */
 typedef struct float_soa_t float_soa;       /* generated twin of double_soa */
@@ -346,6 +347,18 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param *pars, RationalA
 /* Text of the autotuner's measurements so far (candidates and choice per
  * batch signature); returns its length, copies up to len-1 bytes into buf. */
 int ostp_autotune_report(char *buf, int len);
+
+/* ====================== gauge force and action (ipdot_gauge.h, su3_measurements.h, rectangles.h) */
+/* ipdot_gauge.h:15-17: local_staples = C_ZERO x plaquette staples (+ C_ONE x
+ * rectangle staples of types A, B, C for the tree-level Symanzik action set by
+ * ostp_set_stout's tlsm flag), times the defect coefficients with PAR_TEMP;
+ * tipdot = TA(U * staple).  One fused kernel. */
+void calc_ipdot_gauge_soloopenacc(const su3_soa *tconf_acc, su3_soa *local_staples, tamat_soa *tipdot);
+/* su3_measurements.h:14 and rectangles.h: sum over sites and planes of
+ * Re tr of the plaquettes / of the 2x1 and 1x2 rectangles (with the PAR_TEMP
+ * defect coefficients).  The scratch arguments are not written. */
+double calc_plaquette_soloopenacc(const su3_soa *tconf_acc, su3_soa *local_plaqs, dcomplex_soa *tr_local_plaqs);
+double calc_rettangolo_soloopenacc(const su3_soa *tconf_acc, su3_soa *local_plaqs, dcomplex_soa *tr_local_plaqs);
 
 /* ===================== fermion measurements (Meas/ferm_meas.c) ========== */
 /* Z2-noise chiral condensate, ferm_meas.c:282-320 for nvec noise vectors:

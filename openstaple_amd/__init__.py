@@ -208,6 +208,9 @@ _SIGS = {
     "inverter_multishift_wrapper": (_I, [InverterPackage, _P, _P, _P, _P, _D, _I, _I]),
     "ostp_multishift_mixed_solve": (_I, [InverterPackage, _P, _P, _P, _P, _D, _I, _I, _P, _P, _P]),
     "ostp_chiral_condensate": (_I, [_P, _P, _P, _I, C.c_uint64, _D, _D, _I, _I, _P, _P]),
+    "calc_ipdot_gauge_soloopenacc": (None, [_P, _P, _P]),
+    "calc_plaquette_soloopenacc": (_D, [_P, _P, _P]),
+    "calc_rettangolo_soloopenacc": (_D, [_P, _P, _P]),
     "ker_find_max_eigenvalue_openacc": (_D, [_P, _P, _P, _P, _P]),
     "ker_find_min_eigenvalue_openacc": (_D, [_P, _P, _P, _P, _P, _D]),
     "find_min_max_eigenvalue_soloopenacc": (None, [_P, _P, _P, _P, _P, _P, _P]),

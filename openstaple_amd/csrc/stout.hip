@@ -168,6 +168,236 @@ __device__ __forceinline__ void link_of(int gid, int sizeh, int& dir, int& idxh)
     idxh = gid - dir * sizeh;
 }
 
+
+// ============================================================ gauge force
+// Tree-level Symanzik (or Wilson) gauge force and action: plaquette staples
+// times C_ZERO (plaquettes.c calc_loc_staples_nnptrick_all), rectangle
+// staples of types A, B, C times C_ONE (rectangles.c:643-916), with the
+// PAR_TEMP defect coefficients K (products of the staple's links, then the
+// link's own K, su3_utilities.c:582-604), and ipdot = TA(U * staple)
+// (conf_times_staples_ta_part, su3_utilities.c:146-174).
+
+constexpr double kCOne = -0.25 * 0.33333333333333333333333;  // common_defines.h:75
+
+// y = x + a e_mu + b e_nu, periodic
+__device__ __forceinline__ void site_off(const int x[4], int mu, int a, int nu, int b, const LinkGeom& g, int y[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) y[i] = x[i];
+    y[mu] = (y[mu] + a + 2 * g.nd[mu]) % g.nd[mu];
+    y[nu] = (y[nu] + b + 2 * g.nd[nu]) % g.nd[nu];
+}
+
+// U_dir(y): rows 0,1 loaded, row 2 rebuilt; *k = its K when the layout has one
+__device__ __forceinline__ void link_y(const GaugePtrs& G, const LinkGeom& g, int dir, const int y[4], int pt, Su3& m,
+                                       double& k) {
+    const int par = (y[0] + y[1] + y[2] + y[3]) & 1;
+    const int i = snum(y, g);
+    const double2* U = G.u[2 * dir + par];
+    load_link(U, g.sizeh, i, m);
+    k = pt ? ((const double*)(U + 9 * (size_t)g.sizeh))[i] : 1.0;
+}
+
+// five-link staple term m1^(d1) m2^(d2) m3^+ m4^(d4) m5^(d5) for the three
+// shapes of rectangles.c: PPMMM (d = 0,0,1,1,1), PMMMP (0,1,1,1,0), MMMPP (1,1,1,0,0)
+enum Shape : int { kPPMMM = 0, kPMMMP = 1, kMMMPP = 2 };
+struct Hop {
+    int dir, a, b;  // link U_dir at x + a e_mu + b e_nu
+};
+template <int SH>
+__device__ __forceinline__ void five(const GaugePtrs& G, const LinkGeom& g, const int x[4], int mu, int nu, const Hop* h,
+                                     int pt, double c1, Su3& acc) {
+    // each link is loaded just before its product (few live matrices)
+    double k = 1.0;
+    auto ld = [&](int j, Su3& m) {
+        int y[4];
+        site_off(x, mu, h[j].a, nu, h[j].b, g, y);
+        double kj;
+        link_y(G, g, h[j].dir == 0 ? mu : nu, y, pt, m, kj);
+        k *= kj;
+    };
+    Su3 T, M;
+    if (SH == kPPMMM) {
+        ld(0, T);
+        ld(1, M);
+        su3_mul(T, M, T);
+        ld(2, M);
+        su3_mul_dag(T, M, T);
+        ld(3, M);
+        su3_mul_dag(T, M, T);
+        ld(4, M);
+        su3_mul_dag(T, M, T);
+    } else if (SH == kPMMMP) {
+        ld(0, T);
+        ld(1, M);
+        su3_mul_dag(T, M, T);
+        ld(2, M);
+        su3_mul_dag(T, M, T);
+        ld(3, M);
+        su3_mul_dag(T, M, T);
+        ld(4, M);
+        su3_mul(T, M, T);
+    } else {  // m1^+ m2^+ m3^+ m4 m5 = (m3 m2 m1)^+ m4 m5
+        ld(2, T);
+        ld(1, M);
+        su3_mul(T, M, T);
+        ld(0, M);
+        su3_mul(T, M, T);
+        ld(3, M);
+        su3_dag_mul(T, M, T);
+        ld(4, M);
+        su3_mul(T, M, T);
+    }
+    rebuild3row(T);
+    su3_scale_add(k * c1, T, acc);
+}
+
+// staple of link (x, mu), in the reference's accumulation order
+__device__ __forceinline__ void force_staple(const GaugePtrs& G, const LinkGeom& g, const int x[4], int mu, int pt,
+                                             int tlsm, double c0, Su3& S) {
+    su3_zero(S);
+    // plaquettes: right U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+, left U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
+#pragma unroll 1
+    for (int it = 0; it < 3; it++) {
+        const int nu = c_perp[mu][it];
+        int y[4];
+        Su3 A, B, C, T;
+        double ka, kb, kc;
+        site_off(x, mu, 1, nu, 0, g, y);
+        link_y(G, g, nu, y, pt, A, ka);
+        site_off(x, mu, 0, nu, 1, g, y);
+        link_y(G, g, mu, y, pt, B, kb);
+        link_y(G, g, nu, x, pt, C, kc);
+        su3_mul_dag(A, B, T);
+        su3_mul_dag(T, C, T);
+        rebuild3row(T);
+        su3_scale_add(ka * kb * kc * c0, T, S);
+        site_off(x, mu, 1, nu, -1, g, y);
+        link_y(G, g, nu, y, pt, A, ka);
+        site_off(x, mu, 0, nu, -1, g, y);
+        link_y(G, g, mu, y, pt, B, kb);
+        link_y(G, g, nu, y, pt, C, kc);
+        Su3 AB;
+        su3_mul(B, A, AB);
+        su3_dag_mul(AB, C, T);
+        rebuild3row(T);
+        su3_scale_add(ka * kb * kc * c0, T, S);
+    }
+    if (tlsm) {
+        // type A (rectangles.c:691-718)
+#pragma unroll 1
+        for (int it = 0; it < 3; it++) {
+            const int nu = c_perp[mu][it];
+            const Hop ar[5] = {{1, 1, 0}, {1, 1, 1}, {0, 0, 2}, {1, 0, 1}, {1, 0, 0}};
+            const Hop al[5] = {{1, 1, -1}, {1, 1, -2}, {0, 0, -2}, {1, 0, -2}, {1, 0, -1}};
+            five<kPPMMM>(G, g, x, mu, nu, ar, pt, kCOne, S);
+            five<kMMMPP>(G, g, x, mu, nu, al, pt, kCOne, S);
+        }
+        // type B (rectangles.c:780-807)
+#pragma unroll 1
+        for (int it = 0; it < 3; it++) {
+            const int nu = c_perp[mu][it];
+            const Hop br[5] = {{0, 1, 0}, {1, 2, 0}, {0, 1, 1}, {0, 0, 1}, {1, 0, 0}};
+            const Hop bl[5] = {{0, 1, 0}, {1, 2, -1}, {0, 1, -1}, {0, 0, -1}, {1, 0, -1}};
+            five<kPPMMM>(G, g, x, mu, nu, br, pt, kCOne, S);
+            five<kPMMMP>(G, g, x, mu, nu, bl, pt, kCOne, S);
+        }
+        // type C (rectangles.c:865-892)
+#pragma unroll 1
+        for (int it = 0; it < 3; it++) {
+            const int nu = c_perp[mu][it];
+            const Hop cr[5] = {{1, 1, 0}, {0, 0, 1}, {0, -1, 1}, {1, -1, 0}, {0, -1, 0}};
+            const Hop cl[5] = {{1, 1, -1}, {0, 0, -1}, {0, -1, -1}, {1, -1, -1}, {0, -1, 0}};
+            five<kPMMMP>(G, g, x, mu, nu, cr, pt, kCOne, S);
+            five<kMMMPP>(G, g, x, mu, nu, cl, pt, kCOne, S);
+        }
+    }
+    if (pt) {
+        Su3 U;
+        double k;
+        link_y(G, g, mu, x, pt, U, k);
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) S.c[r][c] = make_double2(k * S.c[r][c].x, k * S.c[r][c].y);
+    }
+}
+
+// one thread per link: staple (3 rows) into stap, ipdot = TA(U * staple)
+__global__ __launch_bounds__(kBlock) void k_gauge_force(GaugePtrs G, double2* __restrict__ stap, size_t su3_stride,
+                                                       void* __restrict__ ipdot, LinkGeom g, int pt, int tlsm,
+                                                       double c0) {
+    const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
+    if (gid >= 8 * g.sizeh) return;
+    int dir, idxh;
+    link_of(gid, g.sizeh, dir, idxh);
+    const int mu = dir >> 1;
+    int x[4];
+    decode(idxh, dir & 1, g, x);
+    Su3 S, U, X;
+    force_staple(G, g, x, mu, pt, tlsm, c0, S);
+    if (stap) store_rows((double2*)((char*)stap + dir * su3_stride), g.sizeh, idxh, S, 3);
+    load_link(G.u[dir], g.sizeh, idxh, U);
+    su3_mul(U, S, X);
+    Tamat q;
+    ta_part(X, 1.0, q);
+    ta_store(ipdot, dir, g.sizeh, idxh, q);
+}
+
+// per site (both parities): sum over the six planes of Re tr of the
+// plaquette (kind 0, plaquettes.c:14-78 times K) or of the 2x1 + 1x2
+// rectangles (kind 1, rectangles.c:16-160 times K); block partials
+__global__ __launch_bounds__(kBlock) void k_action_sum(GaugePtrs G, LinkGeom g, int pt, int kind, SolverCtl ctl) {
+    const int gid = blockIdx.x * kBlock + (int)threadIdx.x;
+    double v = 0.0;
+    if (gid < 2 * g.sizeh) {
+        const int parity = gid / g.sizeh, idxh = gid - parity * g.sizeh;
+        int x[4];
+        decode(idxh, parity, g, x);
+#pragma unroll 1
+        for (int mu = 0; mu < 3; mu++)
+#pragma unroll 1
+            for (int nu = mu + 1; nu < 4; nu++) {
+                int y[4];
+                Su3 A, B, C, T;
+                double ka, kb, kc, kd;
+                if (kind == 0) {
+                    // A = U_mu(x), B = U_nu(x+mu), C = U_mu(x+nu), D = U_nu(x): Re tr(A B C^+ D^+)
+                    link_y(G, g, mu, x, pt, A, ka);
+                    site_off(x, mu, 1, nu, 0, g, y);
+                    link_y(G, g, nu, y, pt, B, kb);
+                    site_off(x, mu, 0, nu, 1, g, y);
+                    link_y(G, g, mu, y, pt, C, kc);
+                    Su3 D;
+                    link_y(G, g, nu, x, pt, D, kd);
+                    su3_mul(A, B, T);
+                    su3_mul_dag(T, C, T);
+                    su3_mul_dag(T, D, T);
+                    v += (T.c[0][0].x + T.c[1][1].x + T.c[2][2].x) * (ka * kb * kc * kd);
+                } else {
+                    const Hop r21[6] = {{0, 0, 0}, {0, 1, 0}, {1, 2, 0}, {0, 1, 1}, {0, 0, 1}, {1, 0, 0}};
+                    const Hop r12[6] = {{0, 0, 0}, {1, 1, 0}, {1, 1, 1}, {0, 0, 2}, {1, 0, 1}, {1, 0, 0}};
+#pragma unroll
+                    for (int shape = 0; shape < 2; shape++) {
+                        const Hop* h = shape == 0 ? r21 : r12;
+                        double k = 1.0;
+#pragma unroll
+                        for (int j = 0; j < 6; j++) {
+                            Su3 M;
+                            site_off(x, mu, h[j].a, nu, h[j].b, g, y);
+                            double kj;
+                            link_y(G, g, h[j].dir == 0 ? mu : nu, y, pt, j == 0 ? T : M, kj);
+                            k *= kj;
+                            if (j == 1 || j == 2) su3_mul(T, M, T);
+                            else if (j >= 3) su3_mul_dag(T, M, T);
+                        }
+                        v += (T.c[0][0].x + T.c[1][1].x + T.c[2][2].x) * k;
+                    }
+                }
+            }
+    }
+    block_partial<1>(v, 0.0, ctl);
+}
+
 // Fused smearing: U'(rows 0,1) = exp(-QA) U; optional side outputs of the
 // reference's parking arrays (staples, QA, exp rows 0,1).
 __global__ __launch_bounds__(kBlock) void k_stout(GaugePtrs G, double2* __restrict__ uout, size_t su3_stride,
@@ -580,6 +810,30 @@ bool launch_sigma_chain(void* sigma, void* lam, void* qa, const void* u, double 
     return hip_ok(hipGetLastError(), "k_sigma");
 }
 
+bool launch_gauge_force(const void* u, void* stap, void* ipdot) {
+    const Lattice& L = ctx().lat;
+    hipLaunchKernelGGL(k_gauge_force, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u), (double2*)stap,
+                       L.su3_bytes(), ipdot, link_geom(), L.par_temp, g_tlsm ? 1 : 0, g_c_zero);
+    return hip_ok(hipGetLastError(), "k_gauge_force");
+}
+
+// kind 0: plaquettes, 1: rectangles; the sum returned on the host
+bool gauge_action_sum(const void* u, int kind, double* out) {
+    const Lattice& L = ctx().lat;
+    const int nb = blocks_for(2 * L.sizeh);
+    SolverCtl c = plain_ctl();
+    hipLaunchKernelGGL(k_action_sum, dim3(nb), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u), link_geom(),
+                       L.par_temp, kind, c);
+    if (!hip_ok(hipGetLastError(), "k_action_sum") || !launch_finalize(kFinStore, c, nb)) return false;
+    double v = 0.0;
+    if (!hip_ok(hipMemcpyAsync(&v, &ctx().state->sum, sizeof(double), hipMemcpyDeviceToHost, ctx().stream),
+                "action readback") ||
+        !hip_ok(hipStreamSynchronize(ctx().stream), "action readback"))
+        return false;
+    *out = v;
+    return true;
+}
+
 double stout_c_zero() { return g_c_zero; }
 
 }  // namespace ostp
@@ -733,6 +987,44 @@ void unitarize_conf(su3_soa* u) {
     hipLaunchKernelGGL(k_unitarize, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, (double2*)du, L.su3_bytes(),
                        (int)L.sizeh);
     if (hip_ok(hipGetLastError(), nm)) sync(nm);
+}
+
+// ipdot_gauge.c:361-372 (GAUGE_ACTION from ostp_set_stout's tlsm flag)
+void calc_ipdot_gauge_soloopenacc(const su3_soa* tconf_acc, su3_soa* local_staples, tamat_soa* tipdot) {
+    const char* nm = "calc_ipdot_gauge_soloopenacc";
+    if (!require_init(nm)) return;
+    const Lattice& L = ctx().lat;
+    const void* du = resolve(tconf_acc, 8 * L.su3_bytes(), nm);
+    void* ds = resolve(local_staples, 8 * L.su3_bytes(), nm);
+    void* dt = resolve(tipdot, 8 * (size_t)64 * L.sizeh, nm);
+    if (!du || !ds || !dt) return;
+    if (launch_gauge_force(du, ds, dt)) sync(nm);
+}
+
+// su3_measurements.c:191-211: sum over sites and the six planes of Re tr of
+// the plaquettes (times the defect coefficients with PAR_TEMP).  The
+// reference's scratch arguments are not written.
+double calc_plaquette_soloopenacc(const su3_soa* tconf_acc, su3_soa* local_plaqs, dcomplex_soa* tr_local_plaqs) {
+    (void)local_plaqs;
+    (void)tr_local_plaqs;
+    const char* nm = "calc_plaquette_soloopenacc";
+    if (!require_init(nm)) return NAN;
+    const void* du = resolve(tconf_acc, 8 * ctx().lat.su3_bytes(), nm);
+    double v = NAN;
+    if (du && !gauge_action_sum(du, 0, &v)) return NAN;
+    return v;
+}
+
+// rectangles.c:1890-1909: the same for the 2x1 and 1x2 rectangles
+double calc_rettangolo_soloopenacc(const su3_soa* tconf_acc, su3_soa* local_plaqs, dcomplex_soa* tr_local_plaqs) {
+    (void)local_plaqs;
+    (void)tr_local_plaqs;
+    const char* nm = "calc_rettangolo_soloopenacc";
+    if (!require_init(nm)) return NAN;
+    const void* du = resolve(tconf_acc, 8 * ctx().lat.su3_bytes(), nm);
+    double v = NAN;
+    if (du && !gauge_action_sum(du, 1, &v)) return NAN;
+    return v;
 }
 
 }  // extern "C"

@@ -19,6 +19,9 @@
 #include "OpenAcc/alloc_vars.h"
 #include "OpenAcc/fermionic_utilities.h"
 #include "RationalApprox/rationalapprox.h"
+#include "OpenAcc/plaquettes.h"
+#include "OpenAcc/rectangles.h"
+#include "OpenAcc/su3_measurements.h"
 
 int verbosity_lv = 0;
 action_param act_params;
@@ -101,3 +104,41 @@ void ref_sigma_prime_to_sigma(void *Sigma, void *Lambda, void *QA, const void *U
 }
 
 void ref_unitarize(void *u) { unitarize_conf((su3_soa *)u); }
+
+/* the tree-level Symanzik gauge force of calc_ipdot_gauge_soloopenacc_tlsm
+ * (OpenAcc/ipdot_gauge.c:157-181) composed from the same routines in the
+ * same order (ipdot_gauge.c itself pulls in the debug printers of DbgTools,
+ * whose MPI buffers do not build here); single rank, no PAR_TEMP */
+void ref_gauge_force_tlsm(const void *u, void *stap, void *ipdot)
+{
+    compute_nnp_and_nnm_openacc();
+    set_su3_soa_to_zero((su3_soa *)stap);
+    calc_loc_staples_nnptrick_all((const su3_soa *)u, (su3_soa *)stap);
+    calc_loc_improved_staples_typeA_nnptrick_all((const su3_soa *)u, (su3_soa *)stap);
+    calc_loc_improved_staples_typeB_nnptrick_all((const su3_soa *)u, (su3_soa *)stap);
+    calc_loc_improved_staples_typeC_nnptrick_all((const su3_soa *)u, (su3_soa *)stap);
+    conf_times_staples_ta_part((const su3_soa *)u, (const su3_soa *)stap, (tamat_soa *)ipdot);
+}
+
+/* calc_plaquette_soloopenacc (su3_measurements.c:191-211) and
+ * calc_rettangolo_soloopenacc (rectangles.c:1890-1909) with their scratch */
+double ref_plaquette(const void *u)
+{
+    compute_nnp_and_nnm_openacc();
+    su3_soa *lp = malloc(2 * sizeof(su3_soa));
+    dcomplex_soa *tr = malloc(2 * sizeof(dcomplex_soa));
+    double v = calc_plaquette_soloopenacc((su3_soa *)u, lp, tr);
+    free(lp); free(tr);
+    return v;
+}
+
+double ref_rectangles(const void *u)
+{
+    compute_nnp_and_nnm_openacc();
+    su3_soa *lp = malloc(2 * sizeof(su3_soa));
+    dcomplex_soa *tr = malloc(2 * sizeof(dcomplex_soa));
+    double v = calc_rettangolo_soloopenacc((const su3_soa *)u, lp, tr);
+    free(lp); free(tr);
+    return v;
+}
+

@@ -1,0 +1,170 @@
+"""Gauge force and gauge action on the GPU (calc_ipdot_gauge_soloopenacc,
+calc_plaquette_soloopenacc, calc_rettangolo_soloopenacc).
+
+Bar:
+  * tree-level Symanzik staples, ipdot and the plaquette / rectangle sums
+    against the reference's own plaquettes.c / rectangles.c / su3_utilities.c
+    compiled in oracle/_ref (composed as ipdot_gauge.c:157-181), 1e-13;
+  * independently of the reference: the staple of every sampled link is the
+    derivative of the action c0 sum Re tr P + c1 sum Re tr R along the su(3)
+    generators (central finite differences of the GPU action sums), for the
+    Symanzik and the Wilson action, with and without the PAR_TEMP defect
+    coefficients K; and ipdot = TA(U * staple) recomputed on the host.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import openstaple_amd as ost  # noqa: E402
+from openstaple_amd import lib  # noqa: E402
+from conftest import ref_lib  # noqa: E402
+from gpu_util import Dev, init_lattice  # noqa: E402
+from oracle import Oracle  # noqa: E402
+
+C0_TLSM, C1_TLSM = 5.0 * 0.33333333333333333333333, -0.25 * 0.33333333333333333333333
+
+
+def set_action(tlsm):
+    lib.ostp_set_stout(0.15, 2, 0.0, 0, 0, int(tlsm))
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+class Field:
+    """A gauge field in the reference layout with plane accessors."""
+
+    def __init__(self, shape, par_temp, seed):
+        self.o = o = Oracle(*shape, par_temp=par_temp)
+        self.shape, self.pt = shape, par_temp
+        self.u = o.gen_expH(0.4, seed)
+        if par_temp:
+            o.init_k(self.u, 0.6, 2, (1, 2, 0))
+        self.sizeh = o.sizeh
+        self.per = (144 + (8 if par_temp else 0)) * o.sizeh
+
+    def link(self, l, i):
+        m = np.zeros((3, 3), dtype=np.complex128)
+        b = self.u.view(np.uint8)
+        for r in range(3):
+            for c in range(3):
+                off = l * self.per + (3 * r + c) * 16 * self.sizeh + 16 * i
+                m[r, c] = np.frombuffer(b[off:off + 16].tobytes(), dtype=np.complex128)[0]
+        return m
+
+    def set_link(self, u, l, i, m):
+        for r in range(3):
+            for c in range(3):
+                off = l * self.per + (3 * r + c) * 16 * self.sizeh + 16 * i
+                u[off:off + 16] = np.frombuffer(np.complex128(m[r, c]).tobytes(), dtype=np.uint8)
+
+
+def gpu_force(f):
+    init_lattice(f.shape, par_temp=f.pt)
+    du = Dev(f.u)
+    ds = Dev(nbytes=8 * f.per)
+    dt = Dev(nbytes=8 * 64 * f.sizeh)
+    lib.calc_ipdot_gauge_soloopenacc(du.p, ds.p, dt.p)
+    ost.check(0, "calc_ipdot_gauge_soloopenacc")
+    stap = ds.get(np.zeros(8 * f.per, dtype=np.uint8))
+    ipd = dt.get(np.zeros(8 * 64 * f.sizeh, dtype=np.uint8))
+    return stap, ipd
+
+
+def staple_planes(f, buf):
+    """(8, 3, 3, sizeh) complex from an su3_soa[8] byte buffer."""
+    out = np.zeros((8, 3, 3, f.sizeh), dtype=np.complex128)
+    for l in range(8):
+        base = l * f.per
+        out[l] = np.frombuffer(buf[base:base + 144 * f.sizeh].tobytes(), dtype=np.complex128).reshape(3, 3, -1)
+    return out
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4, 4), (8, 8, 8, 8), (4, 6, 8, 10)])
+def test_force_and_action_vs_reference_build(shape):
+    ref = ref_lib(shape)
+    if ref is None:
+        pytest.skip("oracle/_ref not built")
+    ref.ref_plaquette.restype = C.c_double
+    ref.ref_rectangles.restype = C.c_double
+    f = Field(shape, False, 3)
+    set_action(True)
+    stap, ipd = gpu_force(f)
+    rstap = np.zeros_like(stap)
+    ripd = np.zeros_like(ipd)
+    ref.ref_gauge_force_tlsm(C.c_void_p(f.u.ctypes.data), C.c_void_p(rstap.ctypes.data),
+                             C.c_void_p(ripd.ctypes.data))
+    assert rel(staple_planes(f, stap), staple_planes(f, rstap)) < 1e-13
+    a, b = ipd.view(np.float64), ripd.view(np.float64)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-13
+    du = Dev(f.u)
+    p = lib.calc_plaquette_soloopenacc(du.p, None, None)
+    r = lib.calc_rettangolo_soloopenacc(du.p, None, None)
+    pr = ref.ref_plaquette(C.c_void_p(f.u.ctypes.data))
+    rr = ref.ref_rectangles(C.c_void_p(f.u.ctypes.data))
+    assert abs(p - pr) <= 1e-13 * abs(pr) and abs(r - rr) <= 1e-13 * abs(rr), (p, pr, r, rr)
+
+
+def gell_mann():
+    l = np.zeros((8, 3, 3), dtype=np.complex128)
+    l[0][0, 1] = l[0][1, 0] = 1
+    l[1][0, 1], l[1][1, 0] = -1j, 1j
+    l[2][0, 0], l[2][1, 1] = 1, -1
+    l[3][0, 2] = l[3][2, 0] = 1
+    l[4][0, 2], l[4][2, 0] = -1j, 1j
+    l[5][1, 2] = l[5][2, 1] = 1
+    l[6][1, 2], l[6][2, 1] = -1j, 1j
+    l[7] = np.diag([1, 1, -2]) / np.sqrt(3)
+    return l / 2
+
+
+def expm_h(h):
+    w, v = np.linalg.eigh(h)
+    return (v * np.exp(1j * w)) @ v.conj().T
+
+
+@pytest.mark.parametrize("tlsm,par_temp", [(1, 0), (0, 0), (1, 1), (0, 1)])
+def test_staple_is_action_derivative(tlsm, par_temp):
+    shape = (4, 4, 4, 6)
+    f = Field(shape, bool(par_temp), 5)
+    set_action(tlsm)
+    stap, ipd = gpu_force(f)
+    S = staple_planes(f, stap)
+    c0, c1 = (C0_TLSM, C1_TLSM) if tlsm else (1.0, 0.0)
+
+    def action(u):
+        du = Dev(u)
+        v = c0 * lib.calc_plaquette_soloopenacc(du.p, None, None)
+        if tlsm:
+            v += c1 * lib.calc_rettangolo_soloopenacc(du.p, None, None)
+        return v
+
+    T = gell_mann()
+    rng = np.random.default_rng(0)
+    eps = 1e-4
+    for l, i in [(0, 0), (3, 7), (5, 11), (6, int(rng.integers(f.sizeh))), (7, f.sizeh - 1)]:
+        U = f.link(l, i)
+        U[2] = np.conj(np.cross(U[0], U[1]))
+        X = U @ S[l][:, :, i]
+        # ipdot = TA(U S) (mat1_times_mat2_into_tamat3): c01, c02, c12, Im parts of the diagonal
+        ta = ipd.view(np.float64)
+        base = l * 8 * f.sizeh
+        c = ipd.view(np.complex128)[l * 4 * f.sizeh:l * 4 * f.sizeh + 3 * f.sizeh].reshape(3, -1)[:, i]
+        d = ta[base + 6 * f.sizeh + np.array([i, f.sizeh + i])]
+        tr3 = (X[0, 0].imag + X[1, 1].imag + X[2, 2].imag) / 3
+        assert np.allclose(c, 0.5 * (np.array([X[0, 1], X[0, 2], X[1, 2]]) - np.conj([X[1, 0], X[2, 0], X[2, 1]])),
+                           rtol=0, atol=1e-13 * np.abs(X).max())
+        assert np.allclose(d, [X[0, 0].imag - tr3, X[1, 1].imag - tr3], rtol=0, atol=1e-13 * np.abs(X).max())
+        for a in (0, 3, 7):
+            got = []
+            for s in (1, -1):
+                u2 = f.u.copy()
+                f.set_link(u2, l, i, expm_h(s * eps * T[a]) @ U)
+                got.append(action(u2))
+            fd = (got[0] - got[1]) / (2 * eps)
+            analytic = np.real(np.trace(1j * T[a] @ X))
+            assert abs(fd - analytic) <= 1e-6 * max(1.0, abs(analytic)), (l, i, a, fd, analytic)
