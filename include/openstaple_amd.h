@@ -48,6 +48,8 @@ typedef struct bf_param_t bf_param;         /* OpenAcc/backfield_parameters.h:5-
 typedef struct inv_tricks_t inv_tricks;     /* Include/inverter_tricks.h:4-11 */
 typedef struct rep_info_t rep_info;         /* Include/rep_info.h:4-11 */
 typedef struct defect_info_t defect_info;   /* OpenAcc/HPT_utilities.h:14-32 */
+typedef struct md_param_t md_param;         /* OpenAcc/md_parameters.h:6-19 */
+typedef struct action_param_t action_param; /* OpenAcc/action.h:6-18 */
 
 /* Reference globals the replaced code reads or writes.  The library defines
  * them with the reference defaults; a host program that defines them itself
@@ -55,6 +57,11 @@ typedef struct defect_info_t defect_info;   /* OpenAcc/HPT_utilities.h:14-32 */
  * precedence by ordinary symbol interposition. */
 extern int verbosity_lv;                    /* Include/setting_file_parser.c */
 extern int multishift_invert_iterations;    /* inverter_wrappers.h:14 */
+extern double deltas_Omelyan[7];            /* md_parameters.h:21 */
+extern float deltas_Omelyan_f[7];           /* md_parameters.h:22 */
+extern md_param md_parameters;              /* md_parameters.h:28 */
+extern action_param act_params;             /* action.h:22 (beta) */
+extern int nMdInversionPerformed;           /* md_parameters.h:26 */
 
 #define INVERTER_SUCCESS 1                  /* inverter_full.h:12-13 */
 #define INVERTER_FAILURE 0
@@ -348,6 +355,36 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param *pars, RationalA
  * batch signature); returns its length, copies up to len-1 bytes into buf. */
 int ostp_autotune_report(char *buf, int len);
 
+/* ============ molecular dynamics (su3_utilities.h, su3_measurements.h, md_integrator.h) */
+/* su3_utilities.h:53-57: mom[l] += fact * ipdot[l] in the sense
+ * P -= i fact TA (c01, c02, c12) and P_kk += fact TA_kk, for all 8 link arrays,
+ * fact = factor[id_factor] read on the host (the reference's deltas_Omelyan) */
+void mom_sum_mult(thmat_soa *mom, const tamat_soa *ipdot, const double *factor, int id_factor);
+/* su3_utilities.h:58-61: U <- exp(i factor[id_factor] P) U on rows 0 and 1,
+ * re-unitarised (row 2 is not written, as in the reference) */
+void mom_exp_times_conf_soloopenacc(su3_soa *tconf_acc, const thmat_soa *mom, const double *factor, int id_factor);
+/* su3_measurements.h: sum over the sites of link array mu (0..7) of Tr(P^2)/2;
+ * tr_local (work space of the reference) is not written */
+double calc_momenta_action(const thmat_soa *mom, double_soa *tr_local, const int mu);
+/* su3_measurements.h: P <- -P on all 8 link arrays */
+void invert_momenta(thmat_soa *mom);
+/* md_parameters.h:24: deltas_Omelyan[0..6] (and the float copies) from the
+ * MD parameters and act_params.beta; md_parameters itself is set by the host */
+void initialize_md_global_variables(md_param md_params);
+/* md_integrator.h: the gauge sub-integrator (2MN with md_parameters.gauge_scale
+ * steps over half an outer step).  Each force evaluation is fused with its
+ * momentum step; local_staples receives the staples of the last evaluation. */
+void multistep_2MN_gauge(su3_soa *tconf_acc, su3_soa *local_staples, tamat_soa *tipdot, thmat_soa *tmomenta);
+/* md_integrator.h (STOUT_FERMIONS build): one MD trajectory of
+ * md_parameters.no_md 2MN steps with the fermion force of every flavour and the
+ * gauge sub-integrator.  Not supported (error): act_params.topo_action == 1,
+ * md_parameters.extrapolateInvsForce. */
+void multistep_2MN_SOLOOPENACC(tamat_soa *tipdot_acc, su3_soa *tconf_acc, su3_soa *tstout_conf_acc_arr,
+                               su3_soa *tauxbis_conf_acc, su3_soa *taux_conf_acc, ferm_param *tfermions_parameters,
+                               int tNDiffFlavs, vec3_soa *ferm_in_acc, vec3_soa *tferm_shiftmulti_acc,
+                               inverter_package ip, thmat_soa *tmomenta, dcomplex_soa *local_sums, double res,
+                               const int max_cg);
+
 /* ====================== gauge force and action (ipdot_gauge.h, su3_measurements.h, rectangles.h) */
 /* ipdot_gauge.h:15-17: local_staples = C_ZERO x plaquette staples (+ C_ONE x
  * rectangle staples of types A, B, C for the tree-level Symanzik action set by
@@ -447,6 +484,10 @@ int ostp_fcg_solve(const su3_soa *u, ferm_param *pars, const su3_soa_f *u_f, con
  * the sequential dSFMT state, so every site is generated independently on the
  * device.  `stream` separates independent sources under one seed. */
 void ostp_generate_vec3_soa_gauss(vec3_soa *vect, uint64_t seed, uint32_t stream);
+/* generate_Momenta_gauss (random_assignement.c:145-185): the reference's
+ * construction of the 8 link momenta from eight variance-1/2 normals, drawn
+ * from the same counter-based stream as the other generators */
+void ostp_generate_Momenta_gauss(thmat_soa *mom8, uint64_t seed, uint32_t stream);
 void ostp_generate_vec3_soa_z2noise(vec3_soa *vect, uint64_t seed, uint32_t stream);
 void ostp_generate_Conf_cold(su3_soa *conf, double factor, uint64_t seed);
 /* U = exp(i*beta*H), H traceless Hermitian with standard-normal entries */

@@ -74,6 +74,31 @@ struct inverter_package_t { /* OpenAcc/inverter_package.h:12-29 */
     vec3_soa_f *out_f;
 };
 
+struct md_param_t {         /* OpenAcc/md_parameters.h:6-19 */
+    int no_md;
+    int gauge_scale;
+    double t;
+    double residue_metro;
+    double expected_max_eigenvalue;
+    int singlePrecMD;
+    double residue_md;
+    int max_cg_iterations;
+    int recycleInvsForce;
+    int extrapolateInvsForce;
+};
+
+struct action_param_t {     /* OpenAcc/action.h:6-18 */
+    double beta;
+    int stout_steps;
+    double stout_rho;
+    int topo_action;
+    double barrier;
+    double width;
+    char topo_file_path[20];
+    int topo_stout_steps;
+    double topo_rho;
+};
+
 struct bf_param_t {         /* OpenAcc/backfield_parameters.h:5-9 */
     double ex, ey, ez, bx, by, bz;
 };

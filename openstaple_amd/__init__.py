@@ -88,6 +88,35 @@ class InverterPackage(C.Structure):  # OpenAcc/inverter_package.h:12-29
     ]
 
 
+class MdParam(C.Structure):  # OpenAcc/md_parameters.h:6-19
+    _fields_ = [
+        ("no_md", C.c_int),
+        ("gauge_scale", C.c_int),
+        ("t", C.c_double),
+        ("residue_metro", C.c_double),
+        ("expected_max_eigenvalue", C.c_double),
+        ("singlePrecMD", C.c_int),
+        ("residue_md", C.c_double),
+        ("max_cg_iterations", C.c_int),
+        ("recycleInvsForce", C.c_int),
+        ("extrapolateInvsForce", C.c_int),
+    ]
+
+
+class ActionParam(C.Structure):  # OpenAcc/action.h:6-18
+    _fields_ = [
+        ("beta", C.c_double),
+        ("stout_steps", C.c_int),
+        ("stout_rho", C.c_double),
+        ("topo_action", C.c_int),
+        ("barrier", C.c_double),
+        ("width", C.c_double),
+        ("topo_file_path", C.c_char * 20),
+        ("topo_stout_steps", C.c_int),
+        ("topo_rho", C.c_double),
+    ]
+
+
 class BfParam(C.Structure):  # OpenAcc/backfield_parameters.h:5-9
     _fields_ = [(n, C.c_double) for n in ("ex", "ey", "ez", "bx", "by", "bz")]
 
@@ -209,6 +238,14 @@ _SIGS = {
     "ostp_multishift_mixed_solve": (_I, [InverterPackage, _P, _P, _P, _P, _D, _I, _I, _P, _P, _P]),
     "ostp_chiral_condensate": (_I, [_P, _P, _P, _I, C.c_uint64, _D, _D, _I, _I, _P, _P]),
     "calc_ipdot_gauge_soloopenacc": (None, [_P, _P, _P]),
+    "mom_sum_mult": (None, [_P, _P, _P, _I]),
+    "mom_exp_times_conf_soloopenacc": (None, [_P, _P, _P, _I]),
+    "calc_momenta_action": (_D, [_P, _P, _I]),
+    "invert_momenta": (None, [_P]),
+    "initialize_md_global_variables": (None, [MdParam]),
+    "multistep_2MN_gauge": (None, [_P, _P, _P, _P]),
+    "multistep_2MN_SOLOOPENACC": (None, [_P, _P, _P, _P, _P, _P, _I, _P, _P, InverterPackage, _P, _P, _D, _I]),
+    "ostp_generate_Momenta_gauss": (None, [_P, C.c_uint64, C.c_uint32]),
     "calc_plaquette_soloopenacc": (_D, [_P, _P, _P]),
     "calc_rettangolo_soloopenacc": (_D, [_P, _P, _P]),
     "ker_find_max_eigenvalue_openacc": (_D, [_P, _P, _P, _P, _P]),

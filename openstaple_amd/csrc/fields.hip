@@ -144,7 +144,7 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
     }
 }
 
-enum { kTagGauss = 1, kTagZ2 = 2, kTagCold = 3, kTagExpH = 4 };
+enum { kTagGauss = 1, kTagZ2 = 2, kTagCold = 3, kTagExpH = 4, kTagMom = 5 };
 
 __device__ __forceinline__ void draw(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t w[4]) {
     w[0] = a; w[1] = b; w[2] = c; w[3] = d;
@@ -232,6 +232,35 @@ __global__ __launch_bounds__(kBlock) void k_gen_conf(double2* __restrict__ u, si
     for (int r = 0; r < 3; r++)
         for (int c = 0; c < 3; c++) U[(long)(3 * r + c) * sizeh + i] = m.c[r][c];
     if (par_temp) ((double*)(U + 9L * sizeh))[i] = 1.0;
+}
+
+// generate_Momenta_gauss (random_assignement.c:145-185): eight normals of
+// variance 1/2 per link (four two_double_gauss pairs), combined into the
+// traceless Hermitian momentum the reference builds from them
+__global__ __launch_bounds__(kBlock) void k_gen_momenta(void* __restrict__ mom, int sizeh, uint64_t seed,
+                                                        uint32_t stream) {
+#pragma clang fp contract(off)
+    const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (gid >= 8L * sizeh) return;
+    const int link = (int)(gid / sizeh), i = (int)(gid - (long)link * sizeh);
+    double r[8];
+    for (int k = 0; k < 4; k++) {
+        uint32_t w[4];
+        draw(seed, (uint32_t)i, (uint32_t)(4 * link + k), stream, kTagMom, w);
+        const double u1 = u53(w[0], w[1]), u2 = u53(w[2], w[3]);
+        const double phi = (2 * 3.14159265358979323846) * u2;
+        const double radius = sqrt(-1.0 * log(1.0 - u1));
+        r[2 * k] = radius * cos(phi);
+        r[2 * k + 1] = radius * sin(phi);
+    }
+    const double one_by_sqrt3 = 1.0 / sqrt(3.0);
+    Thmat h;
+    h.rc00 = r[2] + r[7] * one_by_sqrt3;
+    h.rc11 = -r[2] + r[7] * one_by_sqrt3;
+    h.c01 = make_double2(r[0], -r[1]);
+    h.c02 = make_double2(r[3], -r[4]);
+    h.c12 = make_double2(r[5], -r[6]);
+    th_store(mom, link, sizeh, i, h);
 }
 
 // plain copy (roofline reference for the HBM bandwidth of the box)
@@ -327,6 +356,17 @@ void ostp_generate_Conf_cold(su3_soa* conf, double factor, uint64_t seed) {
 }
 void ostp_generate_Conf_expH(su3_soa* conf, double beta, uint64_t seed) {
     gen_conf(conf, beta, seed, 1, "ostp_generate_Conf_expH");
+}
+
+void ostp_generate_Momenta_gauss(thmat_soa* mom8, uint64_t seed, uint32_t stream) {
+    const char* nm = "ostp_generate_Momenta_gauss";
+    if (!require_init(nm)) return;
+    const Lattice& L = ctx().lat;
+    void* d = resolve(mom8, (size_t)8 * 64 * L.sizeh, nm);
+    if (!d) return;
+    hipLaunchKernelGGL(k_gen_momenta, dim3(blocks_for(8 * L.sizeh)), dim3(kBlock), 0, ctx().stream, d, (int)L.sizeh,
+                       seed, stream);
+    if (hip_ok(hipGetLastError(), nm)) hip_ok(hipStreamSynchronize(ctx().stream), nm);
 }
 
 double ostp_time_copy(void* dst, const void* src, size_t bytes, int reps) {

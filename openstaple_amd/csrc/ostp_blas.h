@@ -33,6 +33,14 @@ bool launch_dot(const void* a, const void* b, int complex_dot, const SolverCtl& 
 bool reduce_to_host(int kind, const void* a, const void* b, int single, double* re, double* im);
 SolverCtl plain_ctl();
 
+// Gauge force and action (stout.hip).  launch_gauge_force: staples (when stap
+// is not null) and ipdot of the gauge action; with mom, the momentum step
+// mom_sum_mult(mom, ipdot, fact) fused into the same kernel.
+bool launch_gauge_force(const void* u, void* stap, void* ipdot, void* mom, double fact);
+// kind 0: sum of Re tr of the plaquettes, 1: of the 2x1 and 1x2 rectangles
+bool gauge_action_sum(const void* u, int kind, double* out);
+double stout_c_zero();
+
 // Batched even-odd solves (batch.hip); the C entry points ostp_batched_mixed_solve
 // (half = false) and ostp_batched_half_solve (half = true) call it as well.
 int batched_solve_internal(int nsys, const ostp_batch_system* sys, double res, int max_cg, double mixed_delta,

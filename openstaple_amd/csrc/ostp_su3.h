@@ -221,4 +221,61 @@ __device__ __forceinline__ void ta_part(const Su3& X, double s, Tamat& q) {
     q.ic11 = s * (X.c[1][1].y - third * tr);
 }
 
+// ---------------------------------------------- tamat_soa / thmat_soa access
+// tamat / thmat soa: c01,c02,c12 complex planes then two real planes
+struct TaPtr {
+    double2* c;   // 3 planes
+    double* d;    // 2 planes
+};
+__device__ __forceinline__ TaPtr ta_at(void* base, int link, int sizeh) {
+    char* b = (char*)base + (size_t)link * 64 * sizeh;
+    return TaPtr{(double2*)b, (double*)(b + (size_t)48 * sizeh)};
+}
+__device__ __forceinline__ void ta_load(const void* base, int link, int sizeh, int i, Tamat& q) {
+    TaPtr p = ta_at(const_cast<void*>(base), link, sizeh);
+    q.c01 = p.c[i];
+    q.c02 = p.c[sizeh + i];
+    q.c12 = p.c[2 * sizeh + i];
+    q.ic00 = p.d[i];
+    q.ic11 = p.d[sizeh + i];
+}
+__device__ __forceinline__ void ta_store(void* base, int link, int sizeh, int i, const Tamat& q) {
+    TaPtr p = ta_at(base, link, sizeh);
+    p.c[i] = q.c01;
+    p.c[sizeh + i] = q.c02;
+    p.c[2 * sizeh + i] = q.c12;
+    p.d[i] = q.ic00;
+    p.d[sizeh + i] = q.ic11;
+}
+__device__ __forceinline__ void th_load(const void* base, int link, int sizeh, int i, Thmat& l) {
+    TaPtr p = ta_at(const_cast<void*>(base), link, sizeh);
+    l.c01 = p.c[i];
+    l.c02 = p.c[sizeh + i];
+    l.c12 = p.c[2 * sizeh + i];
+    l.rc00 = p.d[i];
+    l.rc11 = p.d[sizeh + i];
+}
+__device__ __forceinline__ void th_store(void* base, int link, int sizeh, int i, const Thmat& l) {
+    TaPtr p = ta_at(base, link, sizeh);
+    p.c[i] = l.c01;
+    p.c[sizeh + i] = l.c02;
+    p.c[2 * sizeh + i] = l.c12;
+    p.d[i] = l.rc00;
+    p.d[sizeh + i] = l.rc11;
+}
+
+// thm -= (i fact) tam on c01, c02, c12; rc00/rc11 += fact ic00/ic11, rounded
+// like the reference's complex product (no fused multiply-add)
+__device__ __forceinline__ void mom_step(void* mom, int link, int sizeh, int i, const Tamat& q, double fact) {
+#pragma clang fp contract(off)
+    Thmat h;
+    th_load(mom, link, sizeh, i, h);
+    h.c01 = make_double2(h.c01.x + fact * q.c01.y, h.c01.y - fact * q.c01.x);
+    h.c02 = make_double2(h.c02.x + fact * q.c02.y, h.c02.y - fact * q.c02.x);
+    h.c12 = make_double2(h.c12.x + fact * q.c12.y, h.c12.y - fact * q.c12.x);
+    h.rc00 = h.rc00 + fact * q.ic00;
+    h.rc11 = h.rc11 + fact * q.ic11;
+    th_store(mom, link, sizeh, i, h);
+}
+
 }  // namespace ostp

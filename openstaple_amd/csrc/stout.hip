@@ -89,48 +89,6 @@ __device__ __forceinline__ void store_rows(double2* __restrict__ U, int sizeh, i
         for (int c = 0; c < 3; c++) U[(3 * r + c) * sizeh + i] = m.c[r][c];
 }
 
-// tamat / thmat soa: c01,c02,c12 complex planes then two real planes
-struct TaPtr {
-    double2* c;   // 3 planes
-    double* d;    // 2 planes
-};
-__device__ __forceinline__ TaPtr ta_at(void* base, int link, int sizeh) {
-    char* b = (char*)base + (size_t)link * 64 * sizeh;
-    return TaPtr{(double2*)b, (double*)(b + (size_t)48 * sizeh)};
-}
-__device__ __forceinline__ void ta_load(const void* base, int link, int sizeh, int i, Tamat& q) {
-    TaPtr p = ta_at(const_cast<void*>(base), link, sizeh);
-    q.c01 = p.c[i];
-    q.c02 = p.c[sizeh + i];
-    q.c12 = p.c[2 * sizeh + i];
-    q.ic00 = p.d[i];
-    q.ic11 = p.d[sizeh + i];
-}
-__device__ __forceinline__ void ta_store(void* base, int link, int sizeh, int i, const Tamat& q) {
-    TaPtr p = ta_at(base, link, sizeh);
-    p.c[i] = q.c01;
-    p.c[sizeh + i] = q.c02;
-    p.c[2 * sizeh + i] = q.c12;
-    p.d[i] = q.ic00;
-    p.d[sizeh + i] = q.ic11;
-}
-__device__ __forceinline__ void th_load(const void* base, int link, int sizeh, int i, Thmat& l) {
-    TaPtr p = ta_at(const_cast<void*>(base), link, sizeh);
-    l.c01 = p.c[i];
-    l.c02 = p.c[sizeh + i];
-    l.c12 = p.c[2 * sizeh + i];
-    l.rc00 = p.d[i];
-    l.rc11 = p.d[sizeh + i];
-}
-__device__ __forceinline__ void th_store(void* base, int link, int sizeh, int i, const Thmat& l) {
-    TaPtr p = ta_at(base, link, sizeh);
-    p.c[i] = l.c01;
-    p.c[sizeh + i] = l.c02;
-    p.c[2 * sizeh + i] = l.c12;
-    p.d[i] = l.rc00;
-    p.d[sizeh + i] = l.rc11;
-}
-
 __device__ __constant__ int c_perp[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
 
 // sum of the 6 staples of link (x, mu) times c0 (plaquettes.c:196-259)
@@ -322,10 +280,12 @@ __device__ __forceinline__ void force_staple(const GaugePtrs& G, const LinkGeom&
     }
 }
 
-// one thread per link: staple (3 rows) into stap, ipdot = TA(U * staple)
+// one thread per link: staple (3 rows) into stap, ipdot = TA(U * staple);
+// with mom, also the momentum step of mom_sum_mult (su3_utilities.h:1383-1394)
+// mom -= i fact ipdot on the complex entries, mom += fact ipdot on the diagonal
 __global__ __launch_bounds__(kBlock) void k_gauge_force(GaugePtrs G, double2* __restrict__ stap, size_t su3_stride,
                                                        void* __restrict__ ipdot, LinkGeom g, int pt, int tlsm,
-                                                       double c0) {
+                                                       double c0, void* __restrict__ mom, double fact) {
     const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
     if (gid >= 8 * g.sizeh) return;
     int dir, idxh;
@@ -341,6 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_gauge_force(GaugePtrs G, double2* __
     Tamat q;
     ta_part(X, 1.0, q);
     ta_store(ipdot, dir, g.sizeh, idxh, q);
+    if (mom) mom_step(mom, dir, g.sizeh, idxh, q, fact);
 }
 
 // per site (both parities): sum over the six planes of Re tr of the
@@ -810,10 +771,10 @@ bool launch_sigma_chain(void* sigma, void* lam, void* qa, const void* u, double 
     return hip_ok(hipGetLastError(), "k_sigma");
 }
 
-bool launch_gauge_force(const void* u, void* stap, void* ipdot) {
+bool launch_gauge_force(const void* u, void* stap, void* ipdot, void* mom, double fact) {
     const Lattice& L = ctx().lat;
     hipLaunchKernelGGL(k_gauge_force, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u), (double2*)stap,
-                       L.su3_bytes(), ipdot, link_geom(), L.par_temp, g_tlsm ? 1 : 0, g_c_zero);
+                       L.su3_bytes(), ipdot, link_geom(), L.par_temp, g_tlsm ? 1 : 0, g_c_zero, mom, fact);
     return hip_ok(hipGetLastError(), "k_gauge_force");
 }
 
@@ -998,7 +959,7 @@ void calc_ipdot_gauge_soloopenacc(const su3_soa* tconf_acc, su3_soa* local_stapl
     void* ds = resolve(local_staples, 8 * L.su3_bytes(), nm);
     void* dt = resolve(tipdot, 8 * (size_t)64 * L.sizeh, nm);
     if (!du || !ds || !dt) return;
-    if (launch_gauge_force(du, ds, dt)) sync(nm);
+    if (launch_gauge_force(du, ds, dt, nullptr, 0.0)) sync(nm);
 }
 
 // su3_measurements.c:191-211: sum over sites and the six planes of Re tr of

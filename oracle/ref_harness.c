@@ -22,6 +22,8 @@
 #include "OpenAcc/plaquettes.h"
 #include "OpenAcc/rectangles.h"
 #include "OpenAcc/su3_measurements.h"
+#include "OpenAcc/su3_utilities.h"
+#include "OpenAcc/md_parameters.h"
 
 int verbosity_lv = 0;
 action_param act_params;
@@ -142,3 +144,64 @@ double ref_rectangles(const void *u)
     return v;
 }
 
+
+/* mom_sum_mult and mom_exp_times_conf_soloopenacc (su3_utilities.c:241-307)
+ * with a one-entry factor array, calc_momenta_action (su3_measurements.c:138-168) */
+void ref_mom_sum_mult(void *mom, const void *ipdot, double fact)
+{
+    double f[1] = {fact};
+    mom_sum_mult((thmat_soa *)mom, (const tamat_soa *)ipdot, f, 0);
+}
+
+void ref_mom_exp(void *u, const void *mom, double delta)
+{
+    double f[1] = {delta};
+    mom_exp_times_conf_soloopenacc((su3_soa *)u, (const thmat_soa *)mom, f, 0);
+}
+
+double ref_momenta_action(const void *mom, int mu)
+{
+    double_soa *tr = malloc(sizeof(double_soa));
+    double v = calc_momenta_action((const thmat_soa *)mom, tr, mu);
+    free(tr);
+    return v;
+}
+
+/* deltas_Omelyan of the compiled initialize_md_global_variables (md_parameters.c:11-52) */
+void ref_md_deltas(double beta, int no_md, int gauge_scale, double t, double *out)
+{
+    md_param p;
+    memset(&p, 0, sizeof p);
+    p.no_md = no_md;
+    p.gauge_scale = gauge_scale;
+    p.t = t;
+    act_params.beta = beta;
+    initialize_md_global_variables(p);
+    md_parameters = p;
+    memcpy(out, deltas_Omelyan, sizeof deltas_Omelyan);
+}
+
+/* one sub-step of multistep_2MN_gauge_bloc (md_integrator.c:256-345) */
+static void ref_gauge_bloc(void *u, void *stap, void *ipdot, void *mom, int idx)
+{
+    ref_gauge_force_tlsm(u, stap, ipdot);
+    mom_sum_mult((thmat_soa *)mom, (const tamat_soa *)ipdot, deltas_Omelyan, idx);
+    mom_exp_times_conf_soloopenacc((su3_soa *)u, (const thmat_soa *)mom, deltas_Omelyan, 4);
+}
+
+/* multistep_2MN_gauge (md_integrator.c:347-376) composed from the compiled
+ * force, momentum and link routines, deltas from ref_md_deltas (md_integrator.c
+ * itself needs the generated FP32 headers and does not build here) */
+void ref_md_gauge(void *u, void *stap, void *ipdot, void *mom, double beta, int no_md, int gauge_scale, double t)
+{
+    double d[7];
+    ref_md_deltas(beta, no_md, gauge_scale, t, d);
+    ref_gauge_bloc(u, stap, ipdot, mom, 3);
+    for (int md = 1; md < gauge_scale; md++) {
+        ref_gauge_bloc(u, stap, ipdot, mom, 5);
+        ref_gauge_bloc(u, stap, ipdot, mom, 6);
+    }
+    ref_gauge_bloc(u, stap, ipdot, mom, 5);
+    ref_gauge_force_tlsm(u, stap, ipdot);
+    mom_sum_mult((thmat_soa *)mom, (const tamat_soa *)ipdot, deltas_Omelyan, 3);
+}
