@@ -50,6 +50,7 @@ typedef struct rep_info_t rep_info;         /* Include/rep_info.h:4-11 */
 typedef struct defect_info_t defect_info;   /* OpenAcc/HPT_utilities.h:14-32 */
 typedef struct md_param_t md_param;         /* OpenAcc/md_parameters.h:6-19 */
 typedef struct action_param_t action_param; /* OpenAcc/action.h:6-18 */
+typedef struct alloc_settings_t alloc_settings; /* OpenAcc/alloc_settings.h:4-20 */
 
 /* Reference globals the replaced code reads or writes.  The library defines
  * them with the reference defaults; a host program that defines them itself
@@ -62,6 +63,16 @@ extern float deltas_Omelyan_f[7];           /* md_parameters.h:22 */
 extern md_param md_parameters;              /* md_parameters.h:28 */
 extern action_param act_params;             /* action.h:22 (beta) */
 extern int nMdInversionPerformed;           /* md_parameters.h:26 */
+/* fields the RHMC update reads (OpenAcc/alloc_vars.h:20-76, alloc_settings.h:23,
+ * Include/fermion_parameters.h:46); null until the host points them at its own */
+extern su3_soa *conf_acc_bkp, *aux_conf_acc, *auxbis_conf_acc, *gstout_conf_acc_arr, *gconf_as_fermionmatrix;
+extern thmat_soa *momenta;
+extern tamat_soa *ipdot_acc;
+extern vec3_soa *ferm_chi_acc, *ferm_phi_acc, *ferm_shiftmulti_acc, *kloc_r, *kloc_h, *kloc_s, *kloc_p, *k_p_shiftferm;
+extern dcomplex_soa *local_sums;
+extern double_soa *d_local_sums;
+extern ferm_param *fermions_parameters;
+extern alloc_settings alloc_info;
 
 #define INVERTER_SUCCESS 1                  /* inverter_full.h:12-13 */
 #define INVERTER_FAILURE 0
@@ -384,6 +395,37 @@ void multistep_2MN_SOLOOPENACC(tamat_soa *tipdot_acc, su3_soa *tconf_acc, su3_so
                                int tNDiffFlavs, vec3_soa *ferm_in_acc, vec3_soa *tferm_shiftmulti_acc,
                                inverter_package ip, thmat_soa *tmomenta, dcomplex_soa *local_sums, double res,
                                const int max_cg);
+
+/* ====================================== RHMC update (update_versatile.h) */
+/* update_versatile.c:44-580 (STOUT_FERMIONS, double-precision MD): one RHMC
+ * trajectory on tconf_acc with the reference's steps, actions and Metropolis
+ * test; returns acc + 1 if accepted (or metro == 0), else acc.  Reads
+ * act_params.beta, md_parameters, alloc_info (NDiffFlavs, maxApproxOrder),
+ * fermions_parameters (masses, phases, mother approximations) and the field
+ * globals above.  Not supported (error): md_parameters.singlePrecMD,
+ * act_params.topo_action, the NORANDOM and reversibility debug modes. */
+int UPDATE_SOLOACC_UNOSTEP_VERSATILE(su3_soa *tconf_acc, double res_metro, double res_md, int id_iter, int acc,
+                                     int metro, int max_cg);
+
+/* No reference counterpart: the random streams of the update.  Momenta,
+ * pseudofermions and power-method starts draw stream first_stream, first_stream
+ * + 1, ... of the counter-based generator with this seed; the Metropolis
+ * uniform comes from fn(ctx) when set (e.g. the host's casuale), else from the
+ * same seed. */
+typedef double (*ostp_uniform_fn)(void *ctx);
+void ostp_set_rng_seed(uint64_t seed, uint32_t first_stream);
+void ostp_set_metropolis_uniform(ostp_uniform_fn fn, void *ctx);
+/* the actions and the Metropolis decision of the last update (the values the
+ * reference prints) */
+typedef struct ostp_rhmc_info_t {
+    int iteration;
+    int accepted;
+    int md_multishift_iterations;
+    double action_in, action_mom_in, action_ferm_in;    /* gauge action beta/3 (c0 P + c1 R) */
+    double action_fin, action_mom_fin, action_ferm_fin;
+    double delta_S, p1, p2;                             /* p2 = -1: no draw (delta_S < 0) */
+} ostp_rhmc_info;
+int ostp_rhmc_last_info(ostp_rhmc_info *out);
 
 /* ====================== gauge force and action (ipdot_gauge.h, su3_measurements.h, rectangles.h) */
 /* ipdot_gauge.h:15-17: local_staples = C_ZERO x plaquette staples (+ C_ONE x

@@ -99,6 +99,20 @@ struct action_param_t {     /* OpenAcc/action.h:6-18 */
     double topo_rho;
 };
 
+struct alloc_settings_t {   /* OpenAcc/alloc_settings.h:4-20 */
+    int NDiffFlavs;
+    int NPS_tot;
+    int conf_acc_size;
+    int maxNeededShifts;
+    int maxApproxOrder;
+    int revTestAllocations;
+    int diagnosticsAllocations;
+    int singlePrecCoreAllocations;
+    int singlePrecExtendedAllocations;
+    int stoutAllocations;
+    int num_replicas;
+};
+
 struct bf_param_t {         /* OpenAcc/backfield_parameters.h:5-9 */
     double ex, ey, ez, bx, by, bz;
 };

@@ -117,6 +117,33 @@ class ActionParam(C.Structure):  # OpenAcc/action.h:6-18
     ]
 
 
+class AllocSettings(C.Structure):  # OpenAcc/alloc_settings.h:4-20
+    _fields_ = [(n, C.c_int) for n in (
+        "NDiffFlavs", "NPS_tot", "conf_acc_size", "maxNeededShifts", "maxApproxOrder", "revTestAllocations",
+        "diagnosticsAllocations", "singlePrecCoreAllocations", "singlePrecExtendedAllocations", "stoutAllocations",
+        "num_replicas")]
+
+
+class RhmcInfo(C.Structure):  # ostp_rhmc_info
+    _fields_ = [
+        ("iteration", C.c_int),
+        ("accepted", C.c_int),
+        ("md_multishift_iterations", C.c_int),
+        ("action_in", C.c_double),
+        ("action_mom_in", C.c_double),
+        ("action_ferm_in", C.c_double),
+        ("action_fin", C.c_double),
+        ("action_mom_fin", C.c_double),
+        ("action_ferm_fin", C.c_double),
+        ("delta_S", C.c_double),
+        ("p1", C.c_double),
+        ("p2", C.c_double),
+    ]
+
+
+UNIFORM_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
+
+
 class BfParam(C.Structure):  # OpenAcc/backfield_parameters.h:5-9
     _fields_ = [(n, C.c_double) for n in ("ex", "ey", "ez", "bx", "by", "bz")]
 
@@ -246,6 +273,10 @@ _SIGS = {
     "multistep_2MN_gauge": (None, [_P, _P, _P, _P]),
     "multistep_2MN_SOLOOPENACC": (None, [_P, _P, _P, _P, _P, _P, _I, _P, _P, InverterPackage, _P, _P, _D, _I]),
     "ostp_generate_Momenta_gauss": (None, [_P, C.c_uint64, C.c_uint32]),
+    "UPDATE_SOLOACC_UNOSTEP_VERSATILE": (_I, [_P, _D, _D, _I, _I, _I, _I]),
+    "ostp_set_rng_seed": (None, [C.c_uint64, C.c_uint32]),
+    "ostp_set_metropolis_uniform": (None, [UNIFORM_FN, _P]),
+    "ostp_rhmc_last_info": (_I, [C.POINTER(RhmcInfo)]),
     "calc_plaquette_soloopenacc": (_D, [_P, _P, _P]),
     "calc_rettangolo_soloopenacc": (_D, [_P, _P, _P]),
     "ker_find_max_eigenvalue_openacc": (_D, [_P, _P, _P, _P, _P]),

@@ -40,6 +40,9 @@ bool launch_gauge_force(const void* u, void* stap, void* ipdot, void* mom, doubl
 // kind 0: sum of Re tr of the plaquettes, 1: of the 2x1 and 1x2 rectangles
 bool gauge_action_sum(const void* u, int kind, double* out);
 double stout_c_zero();
+bool stout_tlsm();
+// momenta action (md.hip): sum of Tr(P^2)/2 over link arrays [first, first + nlinks)
+bool momenta_action(const void* mom, int first, int nlinks, double* out);
 
 // Batched even-odd solves (batch.hip); the C entry points ostp_batched_mixed_solve
 // (half = false) and ostp_batched_half_solve (half = true) call it as well.

@@ -796,6 +796,7 @@ bool gauge_action_sum(const void* u, int kind, double* out) {
 }
 
 double stout_c_zero() { return g_c_zero; }
+bool stout_tlsm() { return g_tlsm; }
 
 }  // namespace ostp
 
