@@ -55,6 +55,10 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve workload")
     ap.add_argument("--iters-per-graph", type=int, default=16,
                     help="iterations captured per HIP graph in the batched driver (0: direct launches, for profilers)")
+    ap.add_argument("--no-rhmc", action="store_true", help="skip the RHMC trajectory workload")
+    ap.add_argument("--rhmc-only", action="store_true", help="run only the RHMC trajectory workload")
+    ap.add_argument("--rhmc-lattice", type=str, default="48,48,48,12")
+    ap.add_argument("--rhmc-steps", type=int, default=10, help="Omelyan steps per trajectory")
     ap.add_argument("--dslash-only", action="store_true",
                     help="only the timed Dslash loop and the copy kernel (for counter runs)")
     return ap.parse_args()
@@ -69,6 +73,9 @@ def main():
     import openstaple_amd as ost
     from openstaple_amd import lib
 
+    if args.rhmc_only:
+        print(json.dumps({"rhmc": run_rhmc(ost, lib, args)}))
+        return
     shape = tuple(int(x) for x in args.lattice.split(","))
     L = ost.Lattice(*shape)
     L.init()
@@ -188,6 +195,8 @@ def main():
         result["solves"] = run_solves(ost, lib, L, u, ph, b, uf, phf, args)
     if not args.no_batch:
         result["batched"] = run_batched(ost, lib, args)
+    if not args.no_rhmc:
+        result["rhmc"] = run_rhmc(ost, lib, args)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(shape, args.cpu_seconds)
@@ -524,6 +533,111 @@ def run_batched(ost, lib, args):
     out["autotune"] = rep.value.decode().strip().splitlines()[-1] if rep.value else ""
     ost.check(0, "batched")
     return out
+
+
+# SU(3) products in the tree-level Symanzik force of one link: per plane
+# (3 of them) 2 plaquette staples x 2 products and 6 rectangle staples x 4
+# products, plus U * staple; 198 flops per 3x3 complex product (27 complex
+# multiplies, 18 complex additions).  Row reconstructions and the staple sums
+# are not counted.
+GAUGE_FORCE_FLOPS_PER_LINK = (3 * (2 * 2 + 6 * 4) + 1) * 198
+FP64_PEAK_TFLOPS = 78.6
+
+
+def run_rhmc(ost, lib, args):
+    """One RHMC trajectory (UPDATE_SOLOACC_UNOSTEP_VERSATILE) on 48^3 x 12:
+    tree-level Symanzik gauge action at beta = 3.8, 2 stout steps rho = 0.15,
+    Nf = 2+1 staggered with m_l = 0.0015 (degeneracy 2) and m_s = 0.04, one
+    pseudofermion each, order-19 approximations (x^(1/4), x^(-1/2) on
+    [1e-7, 1] for the light quark, x^(1/8), x^(-1/4) on [4e-7, 1] for the
+    strange), args.rhmc_steps Omelyan steps of trajectory length 1 with 5 gauge
+    sub-steps, FP64 multishift solves (1e-8 in the force, 1e-10 for the heat
+    bath and the acceptance), links from the generate_Conf_cold rule with
+    eps = 0.5 (seed 7), seeded momenta and pseudofermions.  Also the device
+    time of the gauge-force kernel on the same links."""
+    golden = ROOT / "tests" / "golden"
+
+    def dm(n):
+        p = lib.ostp_malloc(n)
+        if not p:
+            raise MemoryError(lib.ostp_last_error())
+        lib.ostp_memset(p, 0, n)
+        return p
+
+    shape = tuple(int(x) for x in args.rhmc_lattice.split(","))
+    L = ost.Lattice(*shape)
+    L.init()
+    lib.ostp_set_stout(0.15, 2, 0.0, 0, 0, 1)
+    lib.ostp_set_inverter_tricks(0, 0, 0.1, 10000)
+    sizeh, gb, vb, tb = L.sizeh, 8 * L.su3_bytes, L.vec3_bytes, 8 * 64 * L.sizeh
+    conf = dm(gb)
+    lib.ostp_generate_Conf_cold(conf, 0.5, 7)
+    sizes = {"conf_acc_bkp": gb, "aux_conf_acc": gb, "auxbis_conf_acc": gb, "gstout_conf_acc_arr": 2 * gb,
+             "momenta": tb, "ipdot_acc": tb, "ferm_phi_acc": 2 * vb, "ferm_chi_acc": 2 * vb,
+             "ferm_shiftmulti_acc": 19 * vb, "k_p_shiftferm": 19 * vb, "kloc_r": vb, "kloc_h": vb, "kloc_s": vb,
+             "kloc_p": vb}
+    bufs = {k: dm(n) for k, n in sizes.items()}
+    for k, p in bufs.items():
+        C.c_void_p.in_dll(lib, k).value = p
+    ph = dm(8 * 8 * sizeh)
+    lib.calc_u1_phases(ph, ost.BfParam(0, 0, 0, 0, 0, 0), 0.0, 0.0)
+    fps = (ost.FermParam * 2)()
+    for i, (mass, deg, fi, md) in enumerate([
+            (0.0015, 2, "approx_1_over_4_order_19_mloglm_7.0.REMEZ", "approx_-1_over_2_order_19_mloglm_7.0.REMEZ"),
+            (0.04, 1, "approx_1_over_8_order_19_mloglm_6.4.REMEZ", "approx_-1_over_4_order_19_mloglm_6.4.REMEZ")]):
+        fp = fps[i]
+        fp.ferm_mass, fp.degeneracy, fp.number_of_ps, fp.index_of_the_first_ps = mass, deg, 1, i
+        fp.phases = ph
+        fp.approx_fi_mother = ost.read_remez(str(golden / fi))
+        fp.approx_md_mother = ost.read_remez(str(golden / md))
+        fp.approx_li_mother = ost.read_remez(str(golden / md))
+    C.c_void_p.in_dll(lib, "fermions_parameters").value = C.addressof(fps)
+    ai = ost.AllocSettings.in_dll(lib, "alloc_info")
+    ai.NDiffFlavs, ai.NPS_tot, ai.maxApproxOrder, ai.maxNeededShifts = 2, 2, 19, 19
+    ost.ActionParam.in_dll(lib, "act_params").beta = 3.8
+    mdp = ost.MdParam(no_md=args.rhmc_steps, gauge_scale=5, t=1.0)
+    C.memmove(C.addressof(ost.MdParam.in_dll(lib, "md_parameters")), C.addressof(mdp),
+                   C.sizeof(mdp))
+    lib.initialize_md_global_variables(mdp)
+    ost.check(0, "rhmc setup")
+
+    reps = 20
+    us_force = lib.ostp_time_gauge_force(conf, None, bufs["ipdot_acc"], None, 0.0, reps)
+    us_force_mom = lib.ostp_time_gauge_force(conf, None, bufs["ipdot_acc"], bufs["momenta"], 0.0, reps)
+    ost.check(0, "gauge force timing")
+    flops = GAUGE_FORCE_FLOPS_PER_LINK * 8 * sizeh
+    tflops = flops / (us_force * 1e-6) / 1e12
+
+    lib.ostp_set_rng_seed(2026, 0)
+    t0 = time.perf_counter()
+    acc = lib.UPDATE_SOLOACC_UNOSTEP_VERSATILE(conf, 1e-10, 1e-8, 0, 0, 1, 50000)
+    t1 = time.perf_counter()
+    ost.check(0, "UPDATE_SOLOACC_UNOSTEP_VERSATILE")
+    info = ost.RhmcInfo()
+    lib.ostp_rhmc_last_info(C.byref(info))
+    for k in bufs:
+        C.c_void_p.in_dll(lib, k).value = None
+    C.c_void_p.in_dll(lib, "fermions_parameters").value = None
+    for p in list(bufs.values()) + [conf, ph]:
+        lib.ostp_free(p)
+    return {
+        "workload": "RHMC trajectory %s, TLSM beta=3.8, 2 stout rho=0.15, Nf=2+1 m_l=0.0015 m_s=0.04, order 19, "
+                    "%d Omelyan steps x 5 gauge sub-steps, tau=1, FP64 multishift (force 1e-8, "
+                    "accept 1e-10)" % ("x".join(map(str, shape)), args.rhmc_steps),
+        "s_per_trajectory": round(t1 - t0, 3),
+        "accepted": int(acc),
+        "delta_S": info.delta_S,
+        "md_multishift_iterations": info.md_multishift_iterations,
+        "gauge_force": {
+            "kernel": "k_gauge_force (tree-level Symanzik staples + TA(U S))",
+            "us_per_launch": round(us_force, 1),
+            "us_per_launch_with_momentum_step": round(us_force_mom, 1),
+            "flops_per_link": GAUGE_FORCE_FLOPS_PER_LINK,
+            "achieved_TFLOPs": round(tflops, 2),
+            "peak_TFLOPs": FP64_PEAK_TFLOPS,
+            "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+        },
+    }
 
 
 def cpu_baseline(shape, seconds):

@@ -438,6 +438,11 @@ void calc_ipdot_gauge_soloopenacc(const su3_soa *tconf_acc, su3_soa *local_stapl
  * defect coefficients).  The scratch arguments are not written. */
 double calc_plaquette_soloopenacc(const su3_soa *tconf_acc, su3_soa *local_plaqs, dcomplex_soa *tr_local_plaqs);
 double calc_rettangolo_soloopenacc(const su3_soa *tconf_acc, su3_soa *local_plaqs, dcomplex_soa *tr_local_plaqs);
+/* No reference counterpart: device microseconds per launch of the gauge-force
+ * kernel (staples when stap is not null, ipdot, and the fused momentum step
+ * mom_sum_mult(mom, ipdot, fact) when mom is not null), averaged over reps */
+double ostp_time_gauge_force(const su3_soa *u, su3_soa *stap, tamat_soa *ipdot, thmat_soa *mom, double fact,
+                             int reps);
 
 /* ===================== fermion measurements (Meas/ferm_meas.c) ========== */
 /* Z2-noise chiral condensate, ferm_meas.c:282-320 for nvec noise vectors:

@@ -265,6 +265,7 @@ _SIGS = {
     "ostp_multishift_mixed_solve": (_I, [InverterPackage, _P, _P, _P, _P, _D, _I, _I, _P, _P, _P]),
     "ostp_chiral_condensate": (_I, [_P, _P, _P, _I, C.c_uint64, _D, _D, _I, _I, _P, _P]),
     "calc_ipdot_gauge_soloopenacc": (None, [_P, _P, _P]),
+    "ostp_time_gauge_force": (_D, [_P, _P, _P, _P, _D, _I]),
     "mom_sum_mult": (None, [_P, _P, _P, _I]),
     "mom_exp_times_conf_soloopenacc": (None, [_P, _P, _P, _I]),
     "calc_momenta_action": (_D, [_P, _P, _I]),

@@ -37,8 +37,13 @@ struct LinkGeom {
     int sizeh;
 };
 
+// the 8 link arrays of a gauge field, contiguous at a fixed stride: a link
+// array's address is arithmetic on a per-lane index, not a load from an
+// array of pointers
 struct GaugePtrs {
-    const double2* u[8];  // r0.c0 plane of each link array; rows at +3k*sizeh, cols at +c*sizeh
+    const char* base;  // r0.c0 plane of link array 0; rows at +3k*sizeh, cols at +c*sizeh
+    size_t stride;     // bytes between link arrays
+    __device__ __forceinline__ const double2* at(int l) const { return (const double2*)(base + (size_t)l * stride); }
 };
 
 __device__ __forceinline__ void decode(int idxh, int parity, const LinkGeom& g, int d[4]) {
@@ -101,18 +106,18 @@ __device__ __forceinline__ void staple_sum(const GaugePtrs& G, const LinkGeom& g
         const int nu = c_perp[mu][it];
         Su3 A, B, C, T;
         // right: U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
-        load_link(G.u[2 * nu + !parity], sh, i_pmu, A);
-        load_link(G.u[2 * mu + !parity], sh, shifted(d, nu, 1, 0, 0, g), B);
-        load_link(G.u[2 * nu + parity], sh, idxh, C);
+        load_link(G.at(2 * nu + !parity), sh, i_pmu, A);
+        load_link(G.at(2 * mu + !parity), sh, shifted(d, nu, 1, 0, 0, g), B);
+        load_link(G.at(2 * nu + parity), sh, idxh, C);
         su3_mul_dag(A, B, T);
         su3_mul_dag(T, C, T);
         rebuild3row(T);
         su3_scale_add(c0, T, acc);
         // left: U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
         const int i_mnu = shifted(d, nu, -1, 0, 0, g);
-        load_link(G.u[2 * nu + parity], sh, shifted(d, mu, 1, nu, -1, g), A);
-        load_link(G.u[2 * mu + !parity], sh, i_mnu, B);
-        load_link(G.u[2 * nu + !parity], sh, i_mnu, C);
+        load_link(G.at(2 * nu + parity), sh, shifted(d, mu, 1, nu, -1, g), A);
+        load_link(G.at(2 * mu + !parity), sh, i_mnu, B);
+        load_link(G.at(2 * nu + !parity), sh, i_mnu, C);
         Su3 AB;
         su3_mul(B, A, AB);      // A^+ B^+ = (B A)^+
         su3_dag_mul(AB, C, T);  // (B A)^+ C
@@ -137,22 +142,68 @@ __device__ __forceinline__ void link_of(int gid, int sizeh, int& dir, int& idxh)
 
 constexpr double kCOne = -0.25 * 0.33333333333333333333333;  // common_defines.h:75
 
-// y = x + a e_mu + b e_nu, periodic
+// y = x + a e_mu + b e_nu, periodic (|a|, |b| <= 2 and every extent >= 2)
 __device__ __forceinline__ void site_off(const int x[4], int mu, int a, int nu, int b, const LinkGeom& g, int y[4]) {
 #pragma unroll
     for (int i = 0; i < 4; i++) y[i] = x[i];
-    y[mu] = (y[mu] + a + 2 * g.nd[mu]) % g.nd[mu];
-    y[nu] = (y[nu] + b + 2 * g.nd[nu]) % g.nd[nu];
+    int v = y[mu] + a;
+    v += v < 0 ? g.nd[mu] : 0;
+    v -= v >= g.nd[mu] ? g.nd[mu] : 0;
+    y[mu] = v;
+    v = y[nu] + b;
+    v += v < 0 ? g.nd[nu] : 0;
+    v -= v >= g.nd[nu] ? g.nd[nu] : 0;
+    y[nu] = v;
 }
 
-// U_dir(y): rows 0,1 loaded, row 2 rebuilt; *k = its K when the layout has one
-__device__ __forceinline__ void link_y(const GaugePtrs& G, const LinkGeom& g, int dir, const int y[4], int pt, Su3& m,
-                                       double& k) {
+// U_dir(y): rows 0,1 loaded (row 2 is left for the caller to rebuild when a
+// product needs it); *k = its K when the layout has one
+__device__ __forceinline__ void link2_y(const GaugePtrs& G, const LinkGeom& g, int dir, const int y[4], int pt, Su3& m,
+                                        double& k) {
     const int par = (y[0] + y[1] + y[2] + y[3]) & 1;
     const int i = snum(y, g);
-    const double2* U = G.u[2 * dir + par];
-    load_link(U, g.sizeh, i, m);
+    const double2* U = G.at(2 * dir + par);
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) m.c[r][c] = U[(size_t)(3 * r + c) * g.sizeh + i];
     k = pt ? ((const double*)(U + 9 * (size_t)g.sizeh))[i] : 1.0;
+}
+__device__ __forceinline__ void link_y(const GaugePtrs& G, const LinkGeom& g, int dir, const int y[4], int pt, Su3& m,
+                                       double& k) {
+    link2_y(G, g, dir, y, pt, m, k);
+    rebuild3row(m);
+}
+
+// rows 0,1 of A B, A B^+ (A: rows 0,1; B: all rows); O may alias A
+__device__ __forceinline__ void mul_r2(const Su3& A, const Su3& B, Su3& O) {
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        double2 o[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) o[c] = cfma(A.c[r][2], B.c[2][c], cfma(A.c[r][1], B.c[1][c], cmul(A.c[r][0], B.c[0][c])));
+#pragma unroll
+        for (int c = 0; c < 3; c++) O.c[r][c] = o[c];
+    }
+}
+__device__ __forceinline__ void mul_dag_r2(const Su3& A, const Su3& B, Su3& O) {
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        double2 o[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            o[c] = cfma(A.c[r][2], cconj(B.c[c][2]), cfma(A.c[r][1], cconj(B.c[c][1]), cmul(A.c[r][0], cconj(B.c[c][0]))));
+#pragma unroll
+        for (int c = 0; c < 3; c++) O.c[r][c] = o[c];
+    }
+}
+// rows 0,1 of A^+ B (A, B: all rows); O must not alias A
+__device__ __forceinline__ void dag_mul_r2(const Su3& A, const Su3& B, Su3& O) {
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            O.c[r][c] = cfma(cconj(A.c[2][r]), B.c[2][c], cfma(cconj(A.c[1][r]), B.c[1][c], cmul(cconj(A.c[0][r]), B.c[0][c])));
 }
 
 // five-link staple term m1^(d1) m2^(d2) m3^+ m4^(d4) m5^(d5) for the three
@@ -161,147 +212,205 @@ enum Shape : int { kPPMMM = 0, kPMMMP = 1, kMMMPP = 2 };
 struct Hop {
     int dir, a, b;  // link U_dir at x + a e_mu + b e_nu
 };
+// rows 0,1 of m^+ (m: all rows)
+__device__ __forceinline__ void dag_r2(const Su3& m, Su3& O) {
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) O.c[r][c] = cconj(m.c[c][r]);
+}
+// acc += s T (transposed = 0) or s T^+ (transposed = 1), T with all rows
+__device__ __forceinline__ void add_term(double s, const Su3& T, int transposed, Su3& acc) {
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double2 v = transposed ? cconj(T.c[c][r]) : T.c[r][c];
+            acc.c[r][c] = make_double2(acc.c[r][c].x + s * v.x, acc.c[r][c].y + s * v.y);
+        }
+}
+// Each term is a chain of SU(3) products that carries only rows 0,1 and
+// rebuilds row 2 at the end.  MMMPP is formed as its conjugate
+// m5^+ m4^+ m3 m2 m1, a chain of the same kind, and added transposed.
 template <int SH>
 __device__ __forceinline__ void five(const GaugePtrs& G, const LinkGeom& g, const int x[4], int mu, int nu, const Hop* h,
                                      int pt, double c1, Su3& acc) {
-    // each link is loaded just before its product (few live matrices)
     double k = 1.0;
-    auto ld = [&](int j, Su3& m) {
+    auto ld = [&](int j, Su3& m, bool full) {
         int y[4];
         site_off(x, mu, h[j].a, nu, h[j].b, g, y);
         double kj;
-        link_y(G, g, h[j].dir == 0 ? mu : nu, y, pt, m, kj);
+        link2_y(G, g, h[j].dir == 0 ? mu : nu, y, pt, m, kj);
+        if (full) rebuild3row(m);
         k *= kj;
     };
+    // dagger pattern of the chain, in chain order
+    constexpr int d1 = SH == kPPMMM ? 0 : SH == kPMMMP ? 0 : 1;
+    constexpr int d2 = SH == kPPMMM ? 0 : 1;
+    constexpr int d4 = SH == kPMMMP ? 1 : SH == kPPMMM ? 1 : 0;
+    constexpr int d5 = SH == kPMMMP ? 0 : SH == kPPMMM ? 1 : 0;
     Su3 T, M;
-    if (SH == kPPMMM) {
-        ld(0, T);
-        ld(1, M);
-        su3_mul(T, M, T);
-        ld(2, M);
-        su3_mul_dag(T, M, T);
-        ld(3, M);
-        su3_mul_dag(T, M, T);
-        ld(4, M);
-        su3_mul_dag(T, M, T);
-    } else if (SH == kPMMMP) {
-        ld(0, T);
-        ld(1, M);
-        su3_mul_dag(T, M, T);
-        ld(2, M);
-        su3_mul_dag(T, M, T);
-        ld(3, M);
-        su3_mul_dag(T, M, T);
-        ld(4, M);
-        su3_mul(T, M, T);
-    } else {  // m1^+ m2^+ m3^+ m4 m5 = (m3 m2 m1)^+ m4 m5
-        ld(2, T);
-        ld(1, M);
-        su3_mul(T, M, T);
-        ld(0, M);
-        su3_mul(T, M, T);
-        ld(3, M);
-        su3_dag_mul(T, M, T);
-        ld(4, M);
-        su3_mul(T, M, T);
+    if (SH != kMMMPP) {
+        ld(0, T, false);
+        ld(1, M, true);
+        if (d2) mul_dag_r2(T, M, T); else mul_r2(T, M, T);
+        ld(2, M, true);
+        mul_dag_r2(T, M, T);
+        ld(3, M, true);
+        if (d4) mul_dag_r2(T, M, T); else mul_r2(T, M, T);
+        ld(4, M, true);
+        if (d5) mul_dag_r2(T, M, T); else mul_r2(T, M, T);
+    } else {
+        (void)d1;
+        ld(4, M, true);
+        dag_r2(M, T);
+        ld(3, M, true);
+        mul_dag_r2(T, M, T);
+        ld(2, M, true);
+        mul_r2(T, M, T);
+        ld(1, M, true);
+        mul_r2(T, M, T);
+        ld(0, M, true);
+        mul_r2(T, M, T);
     }
     rebuild3row(T);
-    su3_scale_add(k * c1, T, acc);
+    add_term(k * c1, T, SH == kMMMPP, acc);
 }
 
-// staple of link (x, mu), in the reference's accumulation order
-__device__ __forceinline__ void force_staple(const GaugePtrs& G, const LinkGeom& g, const int x[4], int mu, int pt,
-                                             int tlsm, double c0, Su3& S) {
-    su3_zero(S);
-    // plaquettes: right U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+, left U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
-#pragma unroll 1
-    for (int it = 0; it < 3; it++) {
-        const int nu = c_perp[mu][it];
+// rectangle staple terms A right/left, B right/left, C right/left
+// (rectangles.c:691-718, 780-807, 865-892): the five links and the shape
+__device__ __constant__ Hop c_rect_hops[6][5] = {
+    {{1, 1, 0}, {1, 1, 1}, {0, 0, 2}, {1, 0, 1}, {1, 0, 0}},
+    {{1, 1, -1}, {1, 1, -2}, {0, 0, -2}, {1, 0, -2}, {1, 0, -1}},
+    {{0, 1, 0}, {1, 2, 0}, {0, 1, 1}, {0, 0, 1}, {1, 0, 0}},
+    {{0, 1, 0}, {1, 2, -1}, {0, 1, -1}, {0, 0, -1}, {1, 0, -1}},
+    {{1, 1, 0}, {0, 0, 1}, {0, -1, 1}, {1, -1, 0}, {0, -1, 0}},
+    {{1, 1, -1}, {0, 0, -1}, {0, -1, -1}, {1, -1, -1}, {0, -1, 0}},
+};
+__device__ __constant__ int c_rect_shape[6] = {kPPMMM, kMMMPP, kPPMMM, kPMMMP, kPMMMP, kMMMPP};
+
+// the staple terms of link (x, mu) in the plane (mu, nu), in the reference's
+// order within the plane: plaquettes right and left times C_ZERO (plaquettes.c:
+// 83-195), rectangles of types A, B, C right and left times C_ONE (rectangles.c:
+// 691-718, 780-807, 865-892); K products of each term's links when pt
+__device__ __forceinline__ void plane_staple(const GaugePtrs& G, const LinkGeom& g, const int x[4], int mu, int nu,
+                                             int pt, int tlsm, double c0, Su3& S) {
+    {
+        // right: U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
         int y[4];
-        Su3 A, B, C, T;
+        Su3 A, B, T;
         double ka, kb, kc;
         site_off(x, mu, 1, nu, 0, g, y);
-        link_y(G, g, nu, y, pt, A, ka);
+        link2_y(G, g, nu, y, pt, A, ka);
         site_off(x, mu, 0, nu, 1, g, y);
         link_y(G, g, mu, y, pt, B, kb);
-        link_y(G, g, nu, x, pt, C, kc);
-        su3_mul_dag(A, B, T);
-        su3_mul_dag(T, C, T);
+        mul_dag_r2(A, B, T);
+        link_y(G, g, nu, x, pt, B, kc);
+        mul_dag_r2(T, B, T);
         rebuild3row(T);
-        su3_scale_add(ka * kb * kc * c0, T, S);
-        site_off(x, mu, 1, nu, -1, g, y);
-        link_y(G, g, nu, y, pt, A, ka);
+        add_term(ka * kb * kc * c0, T, 0, S);
+        // left: U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu), formed as its conjugate
+        // U_nu(x-nu)^+ U_mu(x-nu) U_nu(x+mu-nu) and added transposed
         site_off(x, mu, 0, nu, -1, g, y);
+        link_y(G, g, nu, y, pt, B, kc);
+        dag_r2(B, T);
         link_y(G, g, mu, y, pt, B, kb);
-        link_y(G, g, nu, y, pt, C, kc);
-        Su3 AB;
-        su3_mul(B, A, AB);
-        su3_dag_mul(AB, C, T);
+        mul_r2(T, B, T);
+        site_off(x, mu, 1, nu, -1, g, y);
+        link_y(G, g, nu, y, pt, B, ka);
+        mul_r2(T, B, T);
         rebuild3row(T);
-        su3_scale_add(ka * kb * kc * c0, T, S);
+        add_term(ka * kb * kc * c0, T, 1, S);
     }
     if (tlsm) {
-        // type A (rectangles.c:691-718)
+        // one term at a time (a runtime loop keeps the live matrices few)
 #pragma unroll 1
-        for (int it = 0; it < 3; it++) {
-            const int nu = c_perp[mu][it];
-            const Hop ar[5] = {{1, 1, 0}, {1, 1, 1}, {0, 0, 2}, {1, 0, 1}, {1, 0, 0}};
-            const Hop al[5] = {{1, 1, -1}, {1, 1, -2}, {0, 0, -2}, {1, 0, -2}, {1, 0, -1}};
-            five<kPPMMM>(G, g, x, mu, nu, ar, pt, kCOne, S);
-            five<kMMMPP>(G, g, x, mu, nu, al, pt, kCOne, S);
-        }
-        // type B (rectangles.c:780-807)
-#pragma unroll 1
-        for (int it = 0; it < 3; it++) {
-            const int nu = c_perp[mu][it];
-            const Hop br[5] = {{0, 1, 0}, {1, 2, 0}, {0, 1, 1}, {0, 0, 1}, {1, 0, 0}};
-            const Hop bl[5] = {{0, 1, 0}, {1, 2, -1}, {0, 1, -1}, {0, 0, -1}, {1, 0, -1}};
-            five<kPPMMM>(G, g, x, mu, nu, br, pt, kCOne, S);
-            five<kPMMMP>(G, g, x, mu, nu, bl, pt, kCOne, S);
-        }
-        // type C (rectangles.c:865-892)
-#pragma unroll 1
-        for (int it = 0; it < 3; it++) {
-            const int nu = c_perp[mu][it];
-            const Hop cr[5] = {{1, 1, 0}, {0, 0, 1}, {0, -1, 1}, {1, -1, 0}, {0, -1, 0}};
-            const Hop cl[5] = {{1, 1, -1}, {0, 0, -1}, {0, -1, -1}, {1, -1, -1}, {0, -1, 0}};
-            five<kPMMMP>(G, g, x, mu, nu, cr, pt, kCOne, S);
-            five<kMMMPP>(G, g, x, mu, nu, cl, pt, kCOne, S);
+        for (int t = 0; t < 6; t++) {
+            const Hop* h = c_rect_hops[t];
+            switch (c_rect_shape[t]) {
+                case kPPMMM: five<kPPMMM>(G, g, x, mu, nu, h, pt, kCOne, S); break;
+                case kPMMMP: five<kPMMMP>(G, g, x, mu, nu, h, pt, kCOne, S); break;
+                default: five<kMMMPP>(G, g, x, mu, nu, h, pt, kCOne, S); break;
+            }
         }
     }
-    if (pt) {
-        Su3 U;
+}
+
+// Gauge force, one thread per (link, plane): a block is three waves over the
+// same 64 links, wave w summing the staple terms of plane nu = perp(mu)[w].
+// Each wave forms TA(U S_w) (TA is linear, so the planes' parts add up to
+// TA(U S)); the three parts (and, when requested, the staples) are summed
+// through LDS in plane order and multiplied by K_mu(x).  Wave 0 then writes
+// ipdot and, with mom, the momentum step of mom_sum_mult
+// (su3_utilities.h:1383-1394).
+constexpr int kGfLinks = 64;
+__global__ __launch_bounds__(3 * kGfLinks) void k_gauge_force(GaugePtrs G, double2* __restrict__ stap,
+                                                             size_t su3_stride, void* __restrict__ ipdot, LinkGeom g,
+                                                             int pt, int tlsm, double c0, void* __restrict__ mom,
+                                                             double fact) {
+    __shared__ double red[3][18][kGfLinks];
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int gid = xcd_block_id() * kGfLinks + lane;
+    const bool live = gid < 8 * g.sizeh;
+    int dir = 0, idxh = 0, mu = 0;
+    int x[4] = {0, 0, 0, 0};
+    Su3 S;
+    su3_zero(S);
+    Tamat q = {};
+    if (live) {
+        link_of(gid, g.sizeh, dir, idxh);
+        mu = dir >> 1;
+        decode(idxh, dir & 1, g, x);
+        plane_staple(G, g, x, mu, c_perp[mu][w], pt, tlsm, c0, S);
+        Su3 U, X;
         double k;
         link_y(G, g, mu, x, pt, U, k);
+        su3_mul(U, S, X);
+        ta_part(X, 1.0, q);
+    }
+    red[w][0][lane] = q.c01.x;
+    red[w][1][lane] = q.c01.y;
+    red[w][2][lane] = q.c02.x;
+    red[w][3][lane] = q.c02.y;
+    red[w][4][lane] = q.c12.x;
+    red[w][5][lane] = q.c12.y;
+    red[w][6][lane] = q.ic00;
+    red[w][7][lane] = q.ic11;
+    __syncthreads();
+    const double kmu = (live && pt) ? ((const double*)(G.at(dir) + 9 * (size_t)g.sizeh))[idxh] : 1.0;
+    if (w == 0 && live) {
+        double t[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) t[j] = kmu * (red[0][j][lane] + red[1][j][lane] + red[2][j][lane]);
+        Tamat o;
+        o.c01 = make_double2(t[0], t[1]);
+        o.c02 = make_double2(t[2], t[3]);
+        o.c12 = make_double2(t[4], t[5]);
+        o.ic00 = t[6];
+        o.ic11 = t[7];
+        ta_store(ipdot, dir, g.sizeh, idxh, o);
+        if (mom) mom_step(mom, dir, g.sizeh, idxh, o, fact);
+    }
+    if (stap) {
+        __syncthreads();
 #pragma unroll
         for (int r = 0; r < 3; r++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) S.c[r][c] = make_double2(k * S.c[r][c].x, k * S.c[r][c].y);
+            for (int c = 0; c < 3; c++) {
+                red[w][2 * (3 * r + c)][lane] = S.c[r][c].x;
+                red[w][2 * (3 * r + c) + 1][lane] = S.c[r][c].y;
+            }
+        __syncthreads();
+        if (w == 0 && live) {
+            double2* O = (double2*)((char*)stap + dir * su3_stride);
+#pragma unroll
+            for (int e = 0; e < 9; e++) {
+                const double re = kmu * (red[0][2 * e][lane] + red[1][2 * e][lane] + red[2][2 * e][lane]);
+                const double im = kmu * (red[0][2 * e + 1][lane] + red[1][2 * e + 1][lane] + red[2][2 * e + 1][lane]);
+                O[(size_t)e * g.sizeh + idxh] = make_double2(re, im);
+            }
+        }
     }
-}
-
-// one thread per link: staple (3 rows) into stap, ipdot = TA(U * staple);
-// with mom, also the momentum step of mom_sum_mult (su3_utilities.h:1383-1394)
-// mom -= i fact ipdot on the complex entries, mom += fact ipdot on the diagonal
-__global__ __launch_bounds__(kBlock) void k_gauge_force(GaugePtrs G, double2* __restrict__ stap, size_t su3_stride,
-                                                       void* __restrict__ ipdot, LinkGeom g, int pt, int tlsm,
-                                                       double c0, void* __restrict__ mom, double fact) {
-    const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
-    if (gid >= 8 * g.sizeh) return;
-    int dir, idxh;
-    link_of(gid, g.sizeh, dir, idxh);
-    const int mu = dir >> 1;
-    int x[4];
-    decode(idxh, dir & 1, g, x);
-    Su3 S, U, X;
-    force_staple(G, g, x, mu, pt, tlsm, c0, S);
-    if (stap) store_rows((double2*)((char*)stap + dir * su3_stride), g.sizeh, idxh, S, 3);
-    load_link(G.u[dir], g.sizeh, idxh, U);
-    su3_mul(U, S, X);
-    Tamat q;
-    ta_part(X, 1.0, q);
-    ta_store(ipdot, dir, g.sizeh, idxh, q);
-    if (mom) mom_step(mom, dir, g.sizeh, idxh, q, fact);
 }
 
 // per site (both parities): sum over the six planes of Re tr of the
@@ -373,7 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_stout(GaugePtrs G, double2* __restri
     decode(idxh, parity, g, d);
     Su3 S, U, X, E, O;
     staple_sum(G, g, d, idxh, mu, parity, c0, S);
-    load_link(G.u[dir], g.sizeh, idxh, U);
+    load_link(G.at(dir), g.sizeh, idxh, U);
     su3_mul(U, S, X);
     Tamat q;
     ta_part(X, rho / c0, q);
@@ -423,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void k_rho_ta(GaugePtrs G, const double2* _
     int dir, idxh;
     link_of(gid, g.sizeh, dir, idxh);
     Su3 U, S, X;
-    load_link(G.u[dir], g.sizeh, idxh, U);
+    load_link(G.at(dir), g.sizeh, idxh, U);
     load_gl3((const double2*)((const char*)stap + dir * su3_stride), g.sizeh, idxh, S);
     su3_mul(U, S, X);
     Tamat q;
@@ -443,7 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_exp_times_conf(GaugePtrs G, const vo
     ta_load(qa, dir, g.sizeh, idxh, q);
     Su3 E, U, O;
     ch_exp_minus_qa(q, E);
-    load_link(G.u[dir], g.sizeh, idxh, U);
+    load_link(G.at(dir), g.sizeh, idxh, U);
 #pragma unroll
     for (int r = 0; r < 2; r++)
 #pragma unroll
@@ -603,7 +712,7 @@ __global__ __launch_bounds__(kBlock) void k_qa_lambda(GaugePtrs G, const double2
     int dir, idxh;
     link_of(gid, g.sizeh, dir, idxh);
     Su3 U, S;
-    load_link(G.u[dir], g.sizeh, idxh, U);
+    load_link(G.at(dir), g.sizeh, idxh, U);
     Tamat q;
     if (qa_given) {
         ta_load(qa, dir, g.sizeh, idxh, q);
@@ -652,9 +761,9 @@ __global__ __launch_bounds__(kBlock) void k_sigma(GaugePtrs G, const void* __res
         const int i_pnu = shifted(d, nu, 1, 0, 0, g);
         Su3 A, B, C, AB, ABC, X, L1, L2;
         // right part: A = U_nu(x+mu), B = U_mu(x+nu)^+, C = U_nu(x)^+
-        load_link(G.u[2 * nu + !parity], sh, i_pmu, A);
-        load_link(G.u[2 * mu + !parity], sh, i_pnu, B);
-        load_link(G.u[2 * nu + parity], sh, idxh, C);
+        load_link(G.at(2 * nu + !parity), sh, i_pmu, A);
+        load_link(G.at(2 * mu + !parity), sh, i_pnu, B);
+        load_link(G.at(2 * nu + parity), sh, idxh, C);
         su3_mul_dag(A, B, AB);
         su3_mul_dag(AB, C, ABC);
         // i rho ABC (L_mu(x) - L_nu(x))
@@ -680,9 +789,9 @@ __global__ __launch_bounds__(kBlock) void k_sigma(GaugePtrs G, const void* __res
         // left part: A = U_nu(x+mu-nu)^+, B = U_mu(x-nu)^+, C = U_nu(x-nu)
         const int i_mnu = shifted(d, nu, -1, 0, 0, g);
         const int i_pmu_mnu = shifted(d, mu, 1, nu, -1, g);
-        load_link(G.u[2 * nu + parity], sh, i_pmu_mnu, A);
-        load_link(G.u[2 * mu + !parity], sh, i_mnu, B);
-        load_link(G.u[2 * nu + !parity], sh, i_mnu, C);
+        load_link(G.at(2 * nu + parity), sh, i_pmu_mnu, A);
+        load_link(G.at(2 * mu + !parity), sh, i_mnu, B);
+        load_link(G.at(2 * nu + !parity), sh, i_mnu, C);
         su3_mul(B, A, AB);  // (A^+ B^+) = (B A)^+
         // i rho A^+ B^+ (L_nu(x-nu) - L_mu(x-nu)) C
         th_load(lam, 2 * nu + !parity, sh, i_mnu, lt);
@@ -750,7 +859,8 @@ static LinkGeom link_geom() {
 static GaugePtrs gauge_ptrs(const void* u) {
     GaugePtrs G;
     const size_t st = ctx().lat.su3_bytes();
-    for (int l = 0; l < 8; l++) G.u[l] = (const double2*)((const char*)u + l * st);
+    G.base = (const char*)u;
+    G.stride = st;
     return G;
 }
 static int nblk8() { return blocks_for(8 * ctx().lat.sizeh); }
@@ -773,7 +883,8 @@ bool launch_sigma_chain(void* sigma, void* lam, void* qa, const void* u, double 
 
 bool launch_gauge_force(const void* u, void* stap, void* ipdot, void* mom, double fact) {
     const Lattice& L = ctx().lat;
-    hipLaunchKernelGGL(k_gauge_force, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u), (double2*)stap,
+    const int nb = blocks_for(8 * L.sizeh, kGfLinks);
+    hipLaunchKernelGGL(k_gauge_force, dim3(nb), dim3(3 * kGfLinks), 0, ctx().stream, gauge_ptrs(u), (double2*)stap,
                        L.su3_bytes(), ipdot, link_geom(), L.par_temp, g_tlsm ? 1 : 0, g_c_zero, mom, fact);
     return hip_ok(hipGetLastError(), "k_gauge_force");
 }
@@ -987,6 +1098,33 @@ double calc_rettangolo_soloopenacc(const su3_soa* tconf_acc, su3_soa* local_plaq
     double v = NAN;
     if (du && !gauge_action_sum(du, 1, &v)) return NAN;
     return v;
+}
+
+// device time per launch of the gauge-force kernel (HIP events on the
+// library stream), with the fused momentum step when mom is not null
+double ostp_time_gauge_force(const su3_soa* u, su3_soa* stap, tamat_soa* ipdot, thmat_soa* mom, double fact,
+                             int reps) {
+    const char* nm = "ostp_time_gauge_force";
+    if (!require_init(nm)) return -1.0;
+    const Lattice& L = ctx().lat;
+    const size_t gb = 8 * L.su3_bytes(), tb = (size_t)8 * 64 * L.sizeh;
+    const void* du = resolve(u, gb, nm);
+    void* ds = stap ? resolve(stap, gb, nm) : nullptr;
+    void* di = resolve(ipdot, tb, nm);
+    void* dm = mom ? resolve(mom, tb, nm) : nullptr;
+    if (!du || (stap && !ds) || !di || (mom && !dm) || reps <= 0) return -1.0;
+    hipEvent_t e0, e1;
+    if (!hip_ok(hipEventCreate(&e0), "event") || !hip_ok(hipEventCreate(&e1), "event")) return -1.0;
+    (void)hipEventRecord(e0, ctx().stream);
+    for (int r = 0; r < reps; r++)
+        if (!launch_gauge_force(du, ds, di, dm, fact)) return -1.0;
+    (void)hipEventRecord(e1, ctx().stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 1000.0 * (double)ms / reps;
 }
 
 }  // extern "C"
