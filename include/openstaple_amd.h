@@ -51,6 +51,7 @@ typedef struct defect_info_t defect_info;   /* OpenAcc/HPT_utilities.h:14-32 */
 typedef struct md_param_t md_param;         /* OpenAcc/md_parameters.h:6-19 */
 typedef struct action_param_t action_param; /* OpenAcc/action.h:6-18 */
 typedef struct alloc_settings_t alloc_settings; /* OpenAcc/alloc_settings.h:4-20 */
+typedef struct global_su3_soa_t global_su3_soa; /* struct_c_def.h:68-72: r0, r1, r2, no K plane */
 
 /* Reference globals the replaced code reads or writes.  The library defines
  * them with the reference defaults; a host program that defines them itself
@@ -426,6 +427,29 @@ typedef struct ostp_rhmc_info_t {
     double delta_S, p1, p2;                             /* p2 = -1: no draw (delta_S < 0) */
 } ostp_rhmc_info;
 int ostp_rhmc_last_info(ostp_rhmc_info *out);
+
+/* ============================== field files (OpenAcc/io.h, DbgTools/dbgtools.h) */
+/* Host arrays, as in the reference (a device address is staged through a host
+ * copy); the lattice comes from ostp_init or, for programs without a GPU,
+ * from ostp_set_lattice_host.  Formats, record order, byte order and return
+ * codes are the reference's (0 ok, 1 file missing, 2 malformed). */
+int ostp_set_lattice_host(int nd0, int nd1, int nd2, int nd3, int par_temp);
+/* io.c:35-108, io.c:111-207 */
+int print_su3_soa_ASCII(const global_su3_soa *conf_rw, const char *nomefile, int conf_id_iter,
+                        const double_soa *back_phases);
+int read_su3_soa_ASCII(global_su3_soa *conf, const char *nomefile, int *conf_id_iter);
+/* io.c:257-412, io.c:415-547: LIME records ildg-format, MD_traj, input-file,
+ * ildg-binary-data (big-endian FP64, t z y x order, directions x y z t) */
+int print_su3_soa_ildg_binary(const global_su3_soa *conf, const char *nomefile, int conf_id_iter);
+int read_su3_soa_ildg_binary(global_su3_soa *conf, const char *nomefile, int *conf_id_iter);
+/* dbgtools.c:1193-1236 */
+void print_vec3_soa(const vec3_soa *fermion, const char *nomefile);
+int  read_vec3_soa(vec3_soa *fermion, const char *nomefile);
+/* save_conf_wrapper / read_conf_wrapper (io.h:51-150) on the library's
+ * su3_soa[8] (device or registered address; the K plane is not written) */
+int ostp_save_conf(const su3_soa *conf, const char *nomefile, int conf_id_iter, int use_ildg);
+int ostp_read_conf(su3_soa *conf, const char *nomefile, int *conf_id_iter, int use_ildg);
+extern char input_file_str[300 * 500];      /* Include/setting_file_parser.h:11 */
 
 /* ====================== gauge force and action (ipdot_gauge.h, su3_measurements.h, rectangles.h) */
 /* ipdot_gauge.h:15-17: local_staples = C_ZERO x plaquette staples (+ C_ONE x

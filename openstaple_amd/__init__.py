@@ -266,6 +266,15 @@ _SIGS = {
     "ostp_chiral_condensate": (_I, [_P, _P, _P, _I, C.c_uint64, _D, _D, _I, _I, _P, _P]),
     "calc_ipdot_gauge_soloopenacc": (None, [_P, _P, _P]),
     "ostp_time_gauge_force": (_D, [_P, _P, _P, _P, _D, _I]),
+    "ostp_set_lattice_host": (_I, [_I, _I, _I, _I, _I]),
+    "print_su3_soa_ASCII": (_I, [_P, C.c_char_p, _I, _P]),
+    "read_su3_soa_ASCII": (_I, [_P, C.c_char_p, C.POINTER(_I)]),
+    "print_su3_soa_ildg_binary": (_I, [_P, C.c_char_p, _I]),
+    "read_su3_soa_ildg_binary": (_I, [_P, C.c_char_p, C.POINTER(_I)]),
+    "print_vec3_soa": (None, [_P, C.c_char_p]),
+    "read_vec3_soa": (_I, [_P, C.c_char_p]),
+    "ostp_save_conf": (_I, [_P, C.c_char_p, _I, _I]),
+    "ostp_read_conf": (_I, [_P, C.c_char_p, C.POINTER(_I), _I]),
     "mom_sum_mult": (None, [_P, _P, _P, _I]),
     "mom_exp_times_conf_soloopenacc": (None, [_P, _P, _P, _I]),
     "calc_momenta_action": (_D, [_P, _P, _I]),

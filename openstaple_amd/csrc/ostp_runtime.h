@@ -49,6 +49,10 @@ struct Context {
     SolverState* state = nullptr;   // device
     SolverState* state_host = nullptr;  // pinned mirror
     int max_blocks = 0;
+    // extents for the host-only entry points (field files) set by
+    // ostp_set_lattice_host; ostp_init clears them
+    Lattice host_lat{};
+    bool host_lat_set = false;
 };
 
 Context& ctx();

@@ -168,6 +168,7 @@ int ostp_init(int nd0, int nd1, int nd2, int nd3, int par_temp) {
     }
     c.lat = L;
     c.initialized = true;
+    c.host_lat_set = false;
     return 0;
 }
 
