@@ -208,6 +208,12 @@ int ostp_cg_solve(const su3_soa *u, ferm_param *pars, vec3_soa *solution, const 
 int ostp_mixed_solve(inverter_package ip, ferm_param *pars, vec3_soa *solution, const vec3_soa *in,
                      double res, int max_cg, double shift, double mixed_delta, int *cg_return,
                      int *reliable_updates, double *true_res);
+/* No reference counterpart: the CG-M keeps the base residuals of a window of
+ * iterations and applies every shift's x_k / p_k update once per window
+ * (the scalar recurrence is unchanged).  w = 0: window from the free memory
+ * (up to 64; per-iteration updates below 8), 1: per-iteration updates,
+ * 2..64: that window.  Returns the previous setting. */
+int ostp_set_multishift_window(int w);
 int ostp_multishift_solve(const su3_soa *u, ferm_param *pars, RationalApprox *approx, vec3_soa *out,
                           const vec3_soa *in, double residuo, vec3_soa *loc_r, vec3_soa *loc_h,
                           vec3_soa *loc_s, vec3_soa *loc_p, vec3_soa *shiftferm, int max_cg, int single,

@@ -267,6 +267,7 @@ _SIGS = {
     "calc_ipdot_gauge_soloopenacc": (None, [_P, _P, _P]),
     "ostp_time_gauge_force": (_D, [_P, _P, _P, _P, _D, _I]),
     "ostp_set_lattice_host": (_I, [_I, _I, _I, _I, _I]),
+    "ostp_set_multishift_window": (_I, [_I]),
     "print_su3_soa_ASCII": (_I, [_P, C.c_char_p, _I, _P]),
     "read_su3_soa_ASCII": (_I, [_P, C.c_char_p, C.POINTER(_I)]),
     "print_su3_soa_ildg_binary": (_I, [_P, C.c_char_p, _I]),

@@ -126,6 +126,152 @@ __global__ __launch_bounds__(kBlock) void k_ms_update(const C* __restrict__ r, C
     }
 }
 
+// ---------------------------------------------------------------------------
+// CG-M with the shift updates applied once per window of W iterations.
+//
+// Within a window every shifted p_k and x_k is a linear combination of its
+// value at the window start and of the window's base residuals r_0..r_{W-1}:
+//   p_k(t) = B_k(t) p_k(0) + sum_{j<t} D_kj(t) r_j,   B(t+1) = gamma_k B(t),
+//            D_kj(t+1) = gamma_k D_kj(t) (j < t), D_kt(t+1) = zeta_k
+//   x_k(W) = x_k(0) + a_k p_k(0) + sum_j c_kj r_j,    a_k -= omega_k B(t),
+//            c_kj -= omega_k D_kj(t)
+// with the per-iteration scalars of the reference's recurrence
+// (inverter_multishift_full.c:116-171; a frozen shift contributes nothing).
+// The recurrence itself never reads x_k or p_k, so iteration counts and
+// convergence flags are those of the per-iteration updates; the residuals
+// are kept (W x 48 B per half-site) and one pass per window updates x_k and
+// p_k: per iteration that is about 2 vectors of traffic instead of 4 per
+// active shift.
+constexpr int kMsWinMax = 64;
+struct MsWin {
+    int t;                                  // iterations in the current window
+    int active[kMaxShifts];                 // shift moved in this window
+    double a[kMaxShifts], B[kMaxShifts];
+    double c[kMaxShifts][kMsWinMax], D[kMaxShifts][kMsWinMax];
+};
+
+// r_new = r_old + omega s ; lambda = |r_new|^2
+template <typename C>
+__global__ __launch_bounds__(kBlock) void k_msw_r(const C* __restrict__ r_old, C* __restrict__ r_new,
+                                                  const C* __restrict__ s, int sizeh, SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const double om = ctl.st->omega;
+    double v = 0.0;
+    if (i < sizeh) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long k = (long)c * sizeh + i;
+            const C rr = r_old[k], ss = s[k];
+            const C rn = mkc<C>((double)rr.x + om * (double)ss.x, (double)rr.y + om * (double)ss.y);
+            r_new[k] = rn;
+            v += (double)rn.x * rn.x + (double)rn.y * rn.y;
+        }
+    }
+    block_partial<1>(v, 0.0, ctl);
+}
+
+// p = gammag p + r; block 0 also advances the window coefficients of every
+// active shift by this iteration's omega_k, gamma_k, zeta_k
+template <typename C>
+__global__ __launch_bounds__(kBlock) void k_msw_p(C* __restrict__ p, const C* __restrict__ r, long n3, MsWin* w,
+                                                  SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const SolverState* st = ctl.st;
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n3) {
+        const double g = st->gammag;
+        const C rr = r[i], pp = p[i];
+        p[i] = mkc<C>((double)pp.x * g + (double)rr.x, (double)pp.y * g + (double)rr.y);
+    }
+    if (blockIdx.x != 0) return;
+    const int t = w->t, cur = st->flag_cur, order = st->order;
+    const int per = t + 2;  // j < t, the new column t, and the scalar pair
+    for (int e = threadIdx.x; e < order * per; e += kBlock) {
+        const int k = e / per, j = e - k * per;
+        if (st->flags[cur][k] != 1) continue;
+        const double om = st->omegas[k], gk = st->gammas[k];
+        if (j < t) {
+            w->c[k][j] -= om * w->D[k][j];
+            w->D[k][j] *= gk;
+        } else if (j == t) {
+            w->D[k][t] = st->zeta_iii[k];
+        } else {
+            w->a[k] -= om * w->B[k];
+            w->B[k] *= gk;
+            w->active[k] = 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) w->t = t + 1;
+}
+
+// x_k += a_k p_k + sum_j c_kj r_j ; p_k = B_k p_k + sum_j D_kj r_j for the
+// shifts that moved, four shifts per pass over the residuals
+template <typename C>
+__global__ __launch_bounds__(kBlock) void k_msw_flush(C* __restrict__ xs, C* __restrict__ ps,
+                                                      const C* __restrict__ hist, long n3, int order,
+                                                      const MsWin* __restrict__ w, SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    const int t = w->t;
+    if (t == 0) return;
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n3) return;
+    for (int k0 = 0; k0 < order; k0 += 4) {
+        double2 ax[4], ap[4];
+        bool on[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = k0 + q;
+            on[q] = k < order && w->active[k];
+            ax[q] = make_double2(0.0, 0.0);
+            ap[q] = make_double2(0.0, 0.0);
+            if (on[q]) {
+                const C xv = xs[k * n3 + i], pv = ps[k * n3 + i];
+                const double a = w->a[k], b = w->B[k];
+                ax[q] = make_double2((double)xv.x + a * (double)pv.x, (double)xv.y + a * (double)pv.y);
+                ap[q] = make_double2(b * (double)pv.x, b * (double)pv.y);
+            }
+        }
+        if (!(on[0] || on[1] || on[2] || on[3])) continue;
+        for (int j = 0; j < t; j++) {
+            const C rv = hist[(long)j * n3 + i];
+            const double rx = (double)rv.x, ry = (double)rv.y;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (!on[q]) continue;
+                const double c = w->c[k0 + q][j], d = w->D[k0 + q][j];
+                ax[q].x += c * rx;
+                ax[q].y += c * ry;
+                ap[q].x += d * rx;
+                ap[q].y += d * ry;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (on[q]) {
+                xs[(k0 + q) * n3 + i] = mkc<C>(ax[q].x, ax[q].y);
+                ps[(k0 + q) * n3 + i] = mkc<C>(ap[q].x, ap[q].y);
+            }
+    }
+}
+
+// a fresh window: coefficients of the identity map
+__global__ __launch_bounds__(kBlock) void k_msw_reset(MsWin* w, SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    for (int e = threadIdx.x; e < kMaxShifts * kMsWinMax; e += kBlock) {
+        const int k = e / kMsWinMax, j = e - k * kMsWinMax;
+        w->c[k][j] = 0.0;
+        w->D[k][j] = 0.0;
+    }
+    for (int k = threadIdx.x; k < kMaxShifts; k += kBlock) {
+        w->a[k] = 0.0;
+        w->B[k] = 1.0;
+        w->active[k] = 0;
+    }
+    if (threadIdx.x == 0) w->t = 0;
+}
+
 // FP32 CG-M with FP64 reliable updates: r += omega s and, for the base
 // system's solution (needed for the FP64 residual), x0_f -= omega p; |r|^2
 __global__ __launch_bounds__(kBlock) void k_msx_r(float2* __restrict__ r, const float2* __restrict__ s,
@@ -514,6 +660,27 @@ struct MsBufs {
     void *out, *r, *h, *s, *p, *shiftferm;
 };
 
+static void* g_msw_hist = nullptr;
+static size_t g_msw_hist_bytes = 0;
+static MsWin* g_msw = nullptr;
+static int g_msw_forced = 0;  // 0 automatic, 1 per-iteration updates, 2..64 fixed
+
+// window length: as many residuals as fit in a quarter of the free memory,
+// at most 64 (fewer than 8: per-iteration updates)
+static int msw_window(size_t vb) {
+    if (g_msw_forced == 1) return 0;
+    if (g_msw_forced >= 2) return g_msw_forced;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        free_b = 0;
+    }
+    const size_t have = g_msw_hist_bytes + free_b / 4;
+    size_t w = vb ? have / vb : 0;
+    if (w > (size_t)kMsWinMax) w = kMsWinMax;
+    return w >= 8 ? (int)w : 0;
+}
+
 template <typename R>
 int multishift_solve(const MsBufs& B, double mass, int order, const double* shifts, double residuo, int max_cg,
                      int* cg_out, double* shift_res) {
@@ -559,17 +726,60 @@ int multishift_solve(const MsBufs& B, double mass, int order, const double* shif
     if (order == 0) s.status = kDone;  // the reference's loop runs once; nothing to update without shifts
     if (!push_state()) return INVERTER_FAILURE;
 
+    // residual window (falls back to per-iteration shift updates when fewer
+    // than 8 residuals fit)
+    const int W = msw_window(vb);
+    char* hist = nullptr;
+    if (W >= 2) {
+        hist = (char*)g_msw_hist;
+        if (g_msw_hist_bytes < (size_t)W * vb) {
+            if (g_msw_hist) (void)hipFree(g_msw_hist);
+            g_msw_hist = nullptr;
+            g_msw_hist_bytes = 0;
+            if (!hip_ok(hipMalloc(&g_msw_hist, (size_t)W * vb), "multishift residual window")) return INVERTER_FAILURE;
+            g_msw_hist_bytes = (size_t)W * vb;
+        }
+        hist = (char*)g_msw_hist;
+        if (!g_msw && !hip_ok(hipMalloc(&g_msw, sizeof(MsWin)), "multishift window")) return INVERTER_FAILURE;
+        hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kAlways));
+    }
+    auto slot = [&](int j) { return (C*)(hist + (size_t)((j - 1) % W) * vb); };  // residual of iteration j (1-based)
     auto iteration = [&](int j) -> bool {
         if (!launch_mdagm<R>(B.u, B.ph, B.p, B.h, B.s, m2, 2, kFinMsAlpha, ctl_of(kIfRunning))) return false;
-        hipLaunchKernelGGL((k_ms_r<C>), dim3(nbh), dim3(kBlock), 0, S(), (C*)B.r, (const C*)B.s, (int)sizeh,
+        if (!hist) {
+            hipLaunchKernelGGL((k_ms_r<C>), dim3(nbh), dim3(kBlock), 0, S(), (C*)B.r, (const C*)B.s, (int)sizeh,
+                               ctl_of(kIfRunning));
+            if (!launch_finalize(kFinMsLambda, ctl_of(kIfRunning), nbh)) return false;
+            hipLaunchKernelGGL((k_ms_update<C>), dim3(nb3), dim3(kBlock), 0, S(), (const C*)B.r, (C*)B.p,
+                               (C*)B.out, (C*)B.shiftferm, vstride, n3, ctl_of(kIfIter, j));
+            return check_launch("multishift iteration");
+        }
+        const C* r_old = j == 1 ? (const C*)B.r : slot(j - 1);
+        C* r_new = slot(j);
+        hipLaunchKernelGGL((k_msw_r<C>), dim3(nbh), dim3(kBlock), 0, S(), r_old, r_new, (const C*)B.s, (int)sizeh,
                            ctl_of(kIfRunning));
         if (!launch_finalize(kFinMsLambda, ctl_of(kIfRunning), nbh)) return false;
-        hipLaunchKernelGGL((k_ms_update<C>), dim3(nb3), dim3(kBlock), 0, S(), (const C*)B.r, (C*)B.p, (C*)B.out,
-                           (C*)B.shiftferm, vstride, n3, ctl_of(kIfIter, j));
+        hipLaunchKernelGGL((k_msw_p<C>), dim3(nb3), dim3(kBlock), 0, S(), (C*)B.p, (const C*)r_new, n3, g_msw,
+                           ctl_of(kIfIter, j));
+        if (j % W == 0) {
+            // only when iteration j ran: a window cut short is flushed once, at the end
+            hipLaunchKernelGGL((k_msw_flush<C>), dim3(nb3), dim3(kBlock), 0, S(), (C*)B.out, (C*)B.shiftferm,
+                               (const C*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kIfIter, j));
+            hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kIfIter, j));
+        }
         return check_launch("multishift iteration");
     };
     if (!run_batches(iteration)) return INVERTER_FAILURE;
     const int cg = H().cg;
+    if (hist) {
+        // the partial last window, and the final residual where the per-iteration
+        // version leaves it
+        hipLaunchKernelGGL((k_msw_flush<C>), dim3(nb3), dim3(kBlock), 0, S(), (C*)B.out, (C*)B.shiftferm,
+                           (const C*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kAlways));
+        if (cg > 0 && !hip_ok(hipMemcpyAsync(B.r, slot(cg), vb, hipMemcpyDeviceToDevice, S()), "final residual"))
+            return INVERTER_FAILURE;
+        if (!check_launch("multishift window flush")) return INVERTER_FAILURE;
+    }
     multishift_invert_iterations += cg;
     if (cg == max_cg) printf("WARNING: maximum number of iterations reached in invert\n");
 
@@ -1152,6 +1362,13 @@ int ostp_mixed_solve(inverter_package ip, ferm_param* pars, vec3_soa* solution, 
         !B.pf || !B.outf)
         return INVERTER_FAILURE;
     return mixed_solve(B, pars->ferm_mass, shift, res, max_cg, mixed_delta, cg_return, reliable_updates, true_res);
+}
+
+int ostp_set_multishift_window(int w) {
+    if (w < 0 || w > kMsWinMax) return fail("ostp_set_multishift_window: %d outside 0..%d", w, kMsWinMax) ? 0 : 1;
+    const int prev = g_msw_forced;
+    g_msw_forced = w;
+    return prev;
 }
 
 int ostp_multishift_solve(const su3_soa* u, ferm_param* pars, RationalApprox* approx, vec3_soa* out,
