@@ -403,6 +403,52 @@ void multistep_2MN_SOLOOPENACC(tamat_soa *tipdot_acc, su3_soa *tconf_acc, su3_so
                                inverter_package ip, thmat_soa *tmomenta, dcomplex_soa *local_sums, double res,
                                const int max_cg);
 
+/* ========================= fermion measurements (Meas/ferm_meas.h and helpers) */
+/* baryon_number_utilities.h:13-31: dM/dmu and d2M/dmu2 hop operators of one
+ * machine direction (arrays indexed by the direction of time), output even
+ * (eo) or odd (oe) sites; 0.5 (U in(x+mu) +/- U^+ in(x-mu)) with the phases */
+void dM_dmu_eo0(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void dM_dmu_eo1(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void dM_dmu_eo2(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void dM_dmu_eo3(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void dM_dmu_oe0(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void dM_dmu_oe1(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void dM_dmu_oe2(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void dM_dmu_oe3(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void d2M_dmu2_eo0(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void d2M_dmu2_eo1(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void d2M_dmu2_eo2(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void d2M_dmu2_eo3(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void d2M_dmu2_oe0(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void d2M_dmu2_oe1(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void d2M_dmu2_oe2(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+void d2M_dmu2_oe3(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *backfield);
+extern void (*dM_dmu_eo[4])(const su3_soa *, vec3_soa *, const vec3_soa *, const double_soa *);
+extern void (*dM_dmu_oe[4])(const su3_soa *, vec3_soa *, const vec3_soa *, const double_soa *);
+extern void (*d2M_dmu2_eo[4])(const su3_soa *, vec3_soa *, const vec3_soa *, const double_soa *);
+extern void (*d2M_dmu2_oe[4])(const su3_soa *, vec3_soa *, const vec3_soa *, const double_soa *);
+/* field_times_fermion_matrix.h: Deo / Doe with every hop's phase multiplied by
+ * the complex field field_re + i field_im (e.g. i dphase/db_z) */
+void acc_Deo_wf(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *phases,
+                const double_soa *field_re, const double_soa *field_im);
+void acc_Doe_wf(const su3_soa *u, vec3_soa *out, const vec3_soa *in, const double_soa *phases,
+                const double_soa *field_re, const double_soa *field_im);
+/* magnetic_susceptibility_utilities.h: i dphase/db_z of the fermion's phases
+ * (re = 0; im from the unbounded phases at b_z = 1 and 0) */
+void idphase_dbz(double_soa *idphi_dbz_re, double_soa *idphi_dbz_im, ferm_param *fpar);
+/* ferm_meas.c:230-460, one flavour, all noise vectors' inversions batched:
+ * out[12 k + 2 q + (0, 1)] = (re, im) x degeneracy / (4 V) for vector k and
+ * q = 0 chiral condensate <r, M^-1 r>, 1 quark number <r, dM/dmu M^-1 r>,
+ * 2 magnetisation -<r, dM/db M^-1 r> (with pars->mag_re/mag_im, else NaN),
+ * 3 -<r, M^-2 r> (k < n_chiral_susc), 4 <r, d2M/dmu2 M^-1 r> and
+ * 5 -<r, (dM/dmu M^-1)^2 r> (k < n_qn_susc); entries not measured are NaN.
+ * r: Z2 noise from streams 2k (even) and 2k+1 (odd) of seed.  Returns nvec
+ * less the number of inversions that did not converge (floored at 0; nvec
+ * when every inversion converged), 0 on an error. */
+int ostp_fermion_measures(const su3_soa *u, const su3_soa_f *u_f, ferm_param *pars, int nvec, int n_chiral_susc,
+                          int n_qn_susc, uint64_t seed, double res, int max_cg, int half_inner, double *out,
+                          int *cg_return);
+
 /* ====================================== RHMC update (update_versatile.h) */
 /* update_versatile.c:44-580 (STOUT_FERMIONS, double-precision MD): one RHMC
  * trajectory on tconf_acc with the reference's steps, actions and Metropolis

@@ -268,6 +268,12 @@ _SIGS = {
     "ostp_time_gauge_force": (_D, [_P, _P, _P, _P, _D, _I]),
     "ostp_set_lattice_host": (_I, [_I, _I, _I, _I, _I]),
     "ostp_set_multishift_window": (_I, [_I]),
+    **{f"{n}{k}": (None, [_P, _P, _P, _P]) for n in ("dM_dmu_eo", "dM_dmu_oe", "d2M_dmu2_eo", "d2M_dmu2_oe")
+       for k in range(4)},
+    "acc_Deo_wf": (None, [_P, _P, _P, _P, _P, _P]),
+    "acc_Doe_wf": (None, [_P, _P, _P, _P, _P, _P]),
+    "idphase_dbz": (None, [_P, _P, _P]),
+    "ostp_fermion_measures": (_I, [_P, _P, _P, _I, _I, _I, C.c_uint64, _D, _I, _I, _P, C.POINTER(_I)]),
     "print_su3_soa_ASCII": (_I, [_P, C.c_char_p, _I, _P]),
     "read_su3_soa_ASCII": (_I, [_P, C.c_char_p, C.POINTER(_I)]),
     "print_su3_soa_ildg_binary": (_I, [_P, C.c_char_p, _I]),

@@ -64,7 +64,10 @@ __device__ __forceinline__ double rebound_2pi(double a) {
     return a * (2 * 3.14159265358979323846);
 }
 
-__global__ __launch_bounds__(kBlock) void k_u1_phases(double* __restrict__ phases, GeomArgs g, BfArgs bf) {
+// bounded: calc_u1_phases (reduced to [-1/2, 1/2] turns, times 2 pi);
+// otherwise calc_u1_phases_unb_no2pi (backfield.c:20-160): turns, unreduced
+__global__ __launch_bounds__(kBlock) void k_u1_phases(double* __restrict__ phases, GeomArgs g, BfArgs bf,
+                                                      int bounded) {
 #pragma clang fp contract(off)
     const int nd0 = g.nd[0], nd1 = g.nd[1], nd2 = g.nd[2], nd3 = g.nd[3];
     const long vol = (long)nd0 * nd1 * nd2 * nd3;
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_u1_phases(double* __restrict__ phase
     }
     arg *= bf.ferm_charge;
     // KSphaseX == 1 everywhere
-    phases[(long)(xmap * 2 + parity) * sh + idxh] = rebound_2pi(arg);
+    phases[(long)(xmap * 2 + parity) * sh + idxh] = bounded ? rebound_2pi(arg) : arg;
 
     arg = (x - tnx / 2 + 1) * bf.bz / (tnx * tny);
     if (Y == tny) {
@@ -105,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_u1_phases(double* __restrict__ phase
     }
     arg *= bf.ferm_charge;
     if ((1 - 2 * (x & 1)) == -1) arg += 0.5;
-    phases[(long)(ymap * 2 + parity) * sh + idxh] = rebound_2pi(arg);
+    phases[(long)(ymap * 2 + parity) * sh + idxh] = bounded ? rebound_2pi(arg) : arg;
 
     arg = (y - tny / 2 + 1) * bf.bx / (tny * tnz);
     if (Z == tnz) {
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_u1_phases(double* __restrict__ phase
     }
     arg *= bf.ferm_charge;
     if ((1 - 2 * ((x + y) & 1)) == -1) arg += 0.5;
-    phases[(long)(zmap * 2 + parity) * sh + idxh] = rebound_2pi(arg);
+    phases[(long)(zmap * 2 + parity) * sh + idxh] = bounded ? rebound_2pi(arg) : arg;
 
     arg = (z - tnz / 2 + 1) * bf.ez / (tnz * tnt);
     arg += (y - tny / 2 + 1) * bf.ey / (tny * tnt);
@@ -123,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_u1_phases(double* __restrict__ phase
     if ((1 - 2 * ((x + y + z) & 1)) == -1) arg += 0.5;
     arg += chpotphase * 0.5;
     if (T == tnt) arg += 0.5;
-    phases[(long)(tmap * 2 + parity) * sh + idxh] = rebound_2pi(arg);
+    phases[(long)(tmap * 2 + parity) * sh + idxh] = bounded ? rebound_2pi(arg) : arg;
 }
 
 __global__ __launch_bounds__(kBlock) void k_d2f_real(const double* __restrict__ d, float* __restrict__ f, long n) {
@@ -283,6 +286,16 @@ static GeomArgs geom_args() {
 
 using namespace ostp;
 
+namespace ostp {
+// calc_u1_phases_unb_no2pi on a device array of 8 double_soa; waits
+bool u1_phases_unbounded(double* phases, bf_param b, double im_chem_pot, double charge) {
+    BfArgs bf{b.ex, b.ey, b.ez, b.bx, b.by, b.bz, im_chem_pot, charge};
+    hipLaunchKernelGGL(k_u1_phases, dim3(blocks_for(2 * ctx().lat.sizeh)), dim3(kBlock), 0, ctx().stream, phases,
+                       geom_args(), bf, 0);
+    return hip_ok(hipGetLastError(), "k_u1_phases") && hip_ok(hipStreamSynchronize(ctx().stream), "k_u1_phases");
+}
+}  // namespace ostp
+
 extern "C" {
 
 int ostp_compute_nnp_and_nnm(int* nnp, int* nnm) {
@@ -306,7 +319,8 @@ void calc_u1_phases(double_soa* phases, bf_param bf_pars, double im_chem_pot, do
     double* d = (double*)resolve(phases, 8 * L.dsoa_bytes(), "calc_u1_phases");
     if (!d) return;
     BfArgs bf{bf_pars.ex, bf_pars.ey, bf_pars.ez, bf_pars.bx, bf_pars.by, bf_pars.bz, im_chem_pot, ferm_charge};
-    hipLaunchKernelGGL(k_u1_phases, dim3(blocks_for(2 * L.sizeh)), dim3(kBlock), 0, ctx().stream, d, geom_args(), bf);
+    hipLaunchKernelGGL(k_u1_phases, dim3(blocks_for(2 * L.sizeh)), dim3(kBlock), 0, ctx().stream, d, geom_args(), bf,
+                       1);
     if (!hip_ok(hipGetLastError(), "k_u1_phases") || !hip_ok(hipStreamSynchronize(ctx().stream), "sync")) return;
     // a registered host array is updated too, as the reference computes it on the host
     if (ostp_acc_is_present(phases, 8 * L.dsoa_bytes()) && (void*)d != (void*)phases)

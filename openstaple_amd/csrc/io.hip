@@ -78,10 +78,13 @@ struct HostView {
         dev = const_cast<void*>(ptr);
         buf.resize(bytes);
         p = buf.data();
-        ok = !copy_in || hip_ok(hipMemcpy(p, ptr, bytes, hipMemcpyDeviceToHost), nm);
+        // on the library stream, after the kernels queued there
+        ok = !copy_in || ostp_memcpy_dtoh(p, ptr, bytes) == 0;
+        (void)nm;
     }
     bool flush(size_t bytes, const char* nm) {
-        return !dev || hip_ok(hipMemcpy(dev, p, bytes, hipMemcpyHostToDevice), nm);
+        (void)nm;
+        return !dev || ostp_memcpy_htod(dev, p, bytes) == 0;
     }
 };
 
@@ -514,8 +517,7 @@ int ostp_save_conf(const su3_soa* conf, const char* nomefile, int conf_id_iter, 
     if (!d) return 2;
     std::vector<char> host(8 * gstride);
     for (int l = 0; l < 8; l++)
-        if (!hip_ok(hipMemcpy(host.data() + l * gstride, d + l * L.su3_bytes(), gstride, hipMemcpyDeviceToHost), nm))
-            return 2;
+        if (ostp_memcpy_dtoh(host.data() + l * gstride, d + l * L.su3_bytes(), gstride) != 0) return 2;
     return use_ildg ? print_su3_soa_ildg_binary((const global_su3_soa*)host.data(), nomefile, conf_id_iter)
                     : print_su3_soa_ASCII((const global_su3_soa*)host.data(), nomefile, conf_id_iter, nullptr);
 }
@@ -533,8 +535,7 @@ int ostp_read_conf(su3_soa* conf, const char* nomefile, int* conf_id_iter, int u
                            : read_su3_soa_ASCII((global_su3_soa*)host.data(), nomefile, conf_id_iter);
     if (e) return e;
     for (int l = 0; l < 8; l++)
-        if (!hip_ok(hipMemcpy(d + l * L.su3_bytes(), host.data() + l * gstride, gstride, hipMemcpyHostToDevice), nm))
-            return 2;
+        if (ostp_memcpy_htod(d + l * L.su3_bytes(), host.data() + l * gstride, gstride) != 0) return 2;
     // a registered host array gets the same contents, as after the reference's
     // host-side read
     if (ostp_acc_is_present(conf, 8 * L.su3_bytes()) && (void*)d != (void*)conf)

@@ -44,6 +44,9 @@ bool stout_tlsm();
 // momenta action (md.hip): sum of Tr(P^2)/2 over link arrays [first, first + nlinks)
 bool momenta_action(const void* mom, int first, int nlinks, double* out);
 
+// U(1) phases without the reduction and the 2 pi (fields.hip)
+bool u1_phases_unbounded(double* phases, bf_param b, double im_chem_pot, double charge);
+
 // Batched even-odd solves (batch.hip); the C entry points ostp_batched_mixed_solve
 // (half = false) and ostp_batched_half_solve (half = true) call it as well.
 int batched_solve_internal(int nsys, const ostp_batch_system* sys, double res, int max_cg, double mixed_delta,

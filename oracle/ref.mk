@@ -9,7 +9,7 @@ LATTICES ?= 4x4x4x4 8x8x8x8 4x6x8x10
 SRCS := OpenAcc/geometry.c OpenAcc/backfield.c OpenAcc/fermionic_utilities.c \
         RationalApprox/rationalapprox.c Mpi/multidev.c OpenAcc/stouting.c \
         OpenAcc/su3_utilities.c OpenAcc/plaquettes.c OpenAcc/su3_measurements.c \
-        OpenAcc/rectangles.c OpenAcc/md_parameters.c
+        OpenAcc/rectangles.c OpenAcc/md_parameters.c Meas/baryon_number_utilities.c
 TILES := -DDEODOEGANG3=8 -DDEODOETILE0=8 -DDEODOETILE1=8 -DDEODOETILE2=8 \
          -DSTAPGANG3=8 -DSTAPTILE0=8 -DSTAPTILE1=8 -DSTAPTILE2=8 \
          -DIMPSTAPGANG3=8 -DIMPSTAPTILE0=8 -DIMPSTAPTILE1=8 -DIMPSTAPTILE2=8 \

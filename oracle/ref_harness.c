@@ -24,6 +24,7 @@
 #include "OpenAcc/su3_measurements.h"
 #include "OpenAcc/su3_utilities.h"
 #include "OpenAcc/md_parameters.h"
+#include "Meas/baryon_number_utilities.h"
 
 int verbosity_lv = 0;
 action_param act_params;
@@ -204,4 +205,31 @@ void ref_md_gauge(void *u, void *stap, void *ipdot, void *mom, double beta, int 
     ref_gauge_bloc(u, stap, ipdot, mom, 5);
     ref_gauge_force_tlsm(u, stap, ipdot);
     mom_sum_mult((thmat_soa *)mom, (const tamat_soa *)ipdot, deltas_Omelyan, 3);
+}
+
+/* dM/dmu (kind 0) and d2M/dmu2 (kind 1) of machine direction dir, output even
+ * (eo = 0) or odd (eo = 1) sites (Meas/baryon_number_utilities.c) */
+void ref_dmu(int kind, int eo, int dir, const void *u, void *out, const void *in, const void *ph)
+{
+    compute_nnp_and_nnm_openacc();
+    void (*f)(const su3_soa *, vec3_soa *, const vec3_soa *, const double_soa *) =
+        kind == 0 ? (eo == 0 ? dM_dmu_eo[dir] : dM_dmu_oe[dir]) : (eo == 0 ? d2M_dmu2_eo[dir] : d2M_dmu2_oe[dir]);
+    f((const su3_soa *)u, (vec3_soa *)out, (const vec3_soa *)in, (const double_soa *)ph);
+}
+
+/* idphase_dbz (magnetic_susceptibility_utilities.c:13-35) composed from the
+ * compiled backfield.c routines it calls (that file needs generated headers) */
+void ref_idphase_dbz(void *re, void *im, const int *map, double charge)
+{
+    int nd[4] = {nd0, nd1, nd2, nd3};
+    geom_par.xmap = map[0]; geom_par.ymap = map[1]; geom_par.zmap = map[2]; geom_par.tmap = map[3];
+    geom_par.gnx = nd[map[0]]; geom_par.gny = nd[map[1]]; geom_par.gnz = nd[map[2]]; geom_par.gnt = nd[map[3]];
+    bf_param zero = {0, 0, 0, 0, 0, 0};
+    bf_param bz1 = zero;
+    bz1.bz += 1;
+    calc_u1_phases_unb_no2pi((double_soa *)im, bz1, 0, charge);
+    calc_u1_phases_unb_no2pi((double_soa *)re, zero, 0, charge);
+    phase_diff_in_place((double_soa *)im, (double_soa *)re);
+    mult_u1_phases((double_soa *)im, 2 * 3.14159265358979323846);
+    set_double_soa_to_zero((double_soa *)re);
 }

@@ -18,6 +18,7 @@ def declared_functions():
         txt = h.read_text()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         txt = re.sub(r"^typedef[^;]*;", "", txt, flags=re.M | re.S)  # type and callback typedefs
+        txt = re.sub(r'^extern\s+(?!"C")[^;]*;', "", txt, flags=re.M | re.S)  # global variables
         for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", txt, flags=re.M):
             name = m.group(1)
             if name in ("if", "while", "for", "return", "sizeof", "defined"):
@@ -50,8 +51,18 @@ def test_library_exports_every_declared_function(lib):
 
 
 def test_reference_globals_are_exported(lib):
-    for g in ("verbosity_lv", "inverter_tricks", "multishift_invert_iterations"):
+    for g in ("verbosity_lv", "inverter_tricks", "multishift_invert_iterations", "deltas_Omelyan", "md_parameters",
+              "act_params", "nMdInversionPerformed", "conf_acc_bkp", "aux_conf_acc", "auxbis_conf_acc",
+              "gstout_conf_acc_arr", "gconf_as_fermionmatrix", "momenta", "ipdot_acc", "ferm_chi_acc",
+              "ferm_phi_acc", "ferm_shiftmulti_acc", "kloc_r", "kloc_h", "kloc_s", "kloc_p", "k_p_shiftferm",
+              "local_sums", "d_local_sums", "fermions_parameters", "alloc_info", "input_file_str", "dM_dmu_eo",
+              "dM_dmu_oe", "d2M_dmu2_eo", "d2M_dmu2_oe"):
         ctypes.c_int.in_dll(lib, g)
+    # the derivative tables point at the exported functions
+    for name in ("dM_dmu_eo", "dM_dmu_oe", "d2M_dmu2_eo", "d2M_dmu2_oe"):
+        table = (ctypes.c_void_p * 4).in_dll(lib, name)
+        for k in range(4):
+            assert table[k] == ctypes.cast(getattr(lib, "%s%d" % (name, k)), ctypes.c_void_p).value
 
 
 def test_python_binding_covers_the_exports():
