@@ -229,18 +229,53 @@ __device__ __forceinline__ void add_term(double s, const Su3& T, int transposed,
             acc.c[r][c] = make_double2(acc.c[r][c].x + s * v.x, acc.c[r][c].y + s * v.y);
         }
 }
+// The links of a wave's staple terms.  A wave of the gauge force covers 64
+// sites of one link array (direction mu, parity p) and one plane nu, so mu,
+// nu, p, the strides and the link array of every hop are wave-uniform and
+// live in scalar registers; a lane keeps its site's lexicographic index and
+// its mu, nu coordinates, and a neighbour x + a e_mu + b e_nu is
+// (lex + a s_mu + b s_nu, wrapped) / 2 in the link array of parity
+// p + a + b.  Loads are a scalar base plus a 32-bit lane offset.
+struct PlaneCtx {
+    const char* base;
+    size_t stride;
+    unsigned plane;  // bytes per colour plane (16 sizeh)
+    int mu, nu, p, sm, sn, ndm, ndn;
+    int lex, xm, xn;
+};
+__device__ __forceinline__ int pc_idx(const PlaneCtx& c, int a, int b) {
+    const int ym = c.xm + a, yn = c.xn + b;
+    int l = c.lex + a * c.sm + b * c.sn;
+    l += ym < 0 ? c.ndm * c.sm : (ym >= c.ndm ? -c.ndm * c.sm : 0);
+    l += yn < 0 ? c.ndn * c.sn : (yn >= c.ndn ? -c.ndn * c.sn : 0);
+    return l >> 1;
+}
+// rows 0,1 of U_d(x + a e_mu + b e_nu), d = mu (which = 0) or nu (1); *k its K
+__device__ __forceinline__ void pc_link2(const PlaneCtx& c, int which, int a, int b, int pt, Su3& m, double& k) {
+    const int arr = __builtin_amdgcn_readfirstlane(2 * (which ? c.nu : c.mu) + ((c.p + a + b) & 1));
+    const char* U = c.base + (size_t)arr * c.stride;
+    const unsigned i16 = (unsigned)pc_idx(c, a, b) * 16u;
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int col = 0; col < 3; col++)
+            m.c[r][col] = *(const double2*)(U + (i16 + (unsigned)(3 * r + col) * c.plane));
+    k = pt ? *(const double*)(U + (size_t)9 * c.plane + (i16 >> 1)) : 1.0;
+}
+__device__ __forceinline__ void pc_link(const PlaneCtx& c, int which, int a, int b, int pt, Su3& m, double& k) {
+    pc_link2(c, which, a, b, pt, m, k);
+    rebuild3row(m);
+}
+
 // Each term is a chain of SU(3) products that carries only rows 0,1 and
 // rebuilds row 2 at the end.  MMMPP is formed as its conjugate
 // m5^+ m4^+ m3 m2 m1, a chain of the same kind, and added transposed.
 template <int SH>
-__device__ __forceinline__ void five(const GaugePtrs& G, const LinkGeom& g, const int x[4], int mu, int nu, const Hop* h,
-                                     int pt, double c1, Su3& acc) {
+__device__ __forceinline__ void five(const PlaneCtx& pc, const Hop* h, int pt, double c1, Su3& acc) {
     double k = 1.0;
     auto ld = [&](int j, Su3& m, bool full) {
-        int y[4];
-        site_off(x, mu, h[j].a, nu, h[j].b, g, y);
         double kj;
-        link2_y(G, g, h[j].dir == 0 ? mu : nu, y, pt, m, kj);
+        pc_link2(pc, h[j].dir, h[j].a, h[j].b, pt, m, kj);
         if (full) rebuild3row(m);
         k *= kj;
     };
@@ -293,31 +328,25 @@ __device__ __constant__ int c_rect_shape[6] = {kPPMMM, kMMMPP, kPPMMM, kPMMMP, k
 // order within the plane: plaquettes right and left times C_ZERO (plaquettes.c:
 // 83-195), rectangles of types A, B, C right and left times C_ONE (rectangles.c:
 // 691-718, 780-807, 865-892); K products of each term's links when pt
-__device__ __forceinline__ void plane_staple(const GaugePtrs& G, const LinkGeom& g, const int x[4], int mu, int nu,
-                                             int pt, int tlsm, double c0, Su3& S) {
+__device__ __forceinline__ void plane_staple(const PlaneCtx& pc, int pt, int tlsm, double c0, Su3& S) {
     {
         // right: U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
-        int y[4];
         Su3 A, B, T;
         double ka, kb, kc;
-        site_off(x, mu, 1, nu, 0, g, y);
-        link2_y(G, g, nu, y, pt, A, ka);
-        site_off(x, mu, 0, nu, 1, g, y);
-        link_y(G, g, mu, y, pt, B, kb);
+        pc_link2(pc, 1, 1, 0, pt, A, ka);
+        pc_link(pc, 0, 0, 1, pt, B, kb);
         mul_dag_r2(A, B, T);
-        link_y(G, g, nu, x, pt, B, kc);
+        pc_link(pc, 1, 0, 0, pt, B, kc);
         mul_dag_r2(T, B, T);
         rebuild3row(T);
         add_term(ka * kb * kc * c0, T, 0, S);
         // left: U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu), formed as its conjugate
         // U_nu(x-nu)^+ U_mu(x-nu) U_nu(x+mu-nu) and added transposed
-        site_off(x, mu, 0, nu, -1, g, y);
-        link_y(G, g, nu, y, pt, B, kc);
+        pc_link(pc, 1, 0, -1, pt, B, kc);
         dag_r2(B, T);
-        link_y(G, g, mu, y, pt, B, kb);
+        pc_link(pc, 0, 0, -1, pt, B, kb);
         mul_r2(T, B, T);
-        site_off(x, mu, 1, nu, -1, g, y);
-        link_y(G, g, nu, y, pt, B, ka);
+        pc_link(pc, 1, 1, -1, pt, B, ka);
         mul_r2(T, B, T);
         rebuild3row(T);
         add_term(ka * kb * kc * c0, T, 1, S);
@@ -328,9 +357,9 @@ __device__ __forceinline__ void plane_staple(const GaugePtrs& G, const LinkGeom&
         for (int t = 0; t < 6; t++) {
             const Hop* h = c_rect_hops[t];
             switch (c_rect_shape[t]) {
-                case kPPMMM: five<kPPMMM>(G, g, x, mu, nu, h, pt, kCOne, S); break;
-                case kPMMMP: five<kPMMMP>(G, g, x, mu, nu, h, pt, kCOne, S); break;
-                default: five<kMMMPP>(G, g, x, mu, nu, h, pt, kCOne, S); break;
+                case kPPMMM: five<kPPMMM>(pc, h, pt, kCOne, S); break;
+                case kPMMMP: five<kPMMMP>(pc, h, pt, kCOne, S); break;
+                default: five<kMMMPP>(pc, h, pt, kCOne, S); break;
             }
         }
     }
@@ -349,22 +378,41 @@ __global__ __launch_bounds__(3 * kGfLinks) void k_gauge_force(GaugePtrs G, doubl
                                                              int pt, int tlsm, double c0, void* __restrict__ mom,
                                                              double fact) {
     __shared__ double red[3][18][kGfLinks];
-    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
-    const int gid = xcd_block_id() * kGfLinks + lane;
-    const bool live = gid < 8 * g.sizeh;
-    int dir = 0, idxh = 0, mu = 0;
-    int x[4] = {0, 0, 0, 0};
+    const int lane = (int)threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    // a block covers 64 half-sites of one link array: dir is uniform
+    const int nbpd = (g.sizeh + kGfLinks - 1) / kGfLinks;
+    const int blk = xcd_block_id();
+    const int dir = __builtin_amdgcn_readfirstlane(blk / nbpd);
+    const int idxh = (blk - dir * nbpd) * kGfLinks + lane;
+    const bool live = idxh < g.sizeh;
+    const int mu = dir >> 1;
     Su3 S;
     su3_zero(S);
     Tamat q = {};
     if (live) {
-        link_of(gid, g.sizeh, dir, idxh);
-        mu = dir >> 1;
+        int x[4];
         decode(idxh, dir & 1, g, x);
-        plane_staple(G, g, x, mu, c_perp[mu][w], pt, tlsm, c0, S);
+        PlaneCtx pc;
+        pc.base = G.base;
+        pc.stride = G.stride;
+        pc.plane = 16u * (unsigned)g.sizeh;
+        pc.mu = mu;
+        pc.nu = __builtin_amdgcn_readfirstlane(c_perp[mu][w]);
+        pc.p = dir & 1;
+        const int st[4] = {1, g.nd[0], g.nd[0] * g.nd[1], g.nd[0] * g.nd[1] * g.nd[2]};
+        auto sel = [](const int* v, int i) { return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3]; };
+        pc.sm = sel(st, pc.mu);
+        pc.sn = sel(st, pc.nu);
+        pc.ndm = sel(g.nd, pc.mu);
+        pc.ndn = sel(g.nd, pc.nu);
+        pc.lex = x[0] + st[1] * x[1] + st[2] * x[2] + st[3] * x[3];
+        pc.xm = sel(x, pc.mu);
+        pc.xn = sel(x, pc.nu);
+        plane_staple(pc, pt, tlsm, c0, S);
         Su3 U, X;
         double k;
-        link_y(G, g, mu, x, pt, U, k);
+        pc_link(pc, 0, 0, 0, pt, U, k);
         su3_mul(U, S, X);
         ta_part(X, 1.0, q);
     }
@@ -883,7 +931,10 @@ bool launch_sigma_chain(void* sigma, void* lam, void* qa, const void* u, double 
 
 bool launch_gauge_force(const void* u, void* stap, void* ipdot, void* mom, double fact) {
     const Lattice& L = ctx().lat;
-    const int nb = blocks_for(8 * L.sizeh, kGfLinks);
+    // lane offsets within a link array are 32-bit (up to 6 planes of 16 sizeh bytes)
+    if ((size_t)96 * L.sizeh >= ((size_t)1 << 32)) return fail("gauge force: lattice too large (sizeh %ld)", (long)L.sizeh);
+    // whole blocks per link array (the kernel takes the direction from the block)
+    const int nb = 8 * (int)((L.sizeh + kGfLinks - 1) / kGfLinks);
     hipLaunchKernelGGL(k_gauge_force, dim3(nb), dim3(3 * kGfLinks), 0, ctx().stream, gauge_ptrs(u), (double2*)stap,
                        L.su3_bytes(), ipdot, link_geom(), L.par_temp, g_tlsm ? 1 : 0, g_c_zero, mom, fact);
     return hip_ok(hipGetLastError(), "k_gauge_force");
