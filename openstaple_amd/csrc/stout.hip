@@ -328,40 +328,124 @@ __device__ __constant__ int c_rect_shape[6] = {kPPMMM, kMMMPP, kPPMMM, kPMMMP, k
 // order within the plane: plaquettes right and left times C_ZERO (plaquettes.c:
 // 83-195), rectangles of types A, B, C right and left times C_ONE (rectangles.c:
 // 691-718, 780-807, 865-892); K products of each term's links when pt
-__device__ __forceinline__ void plane_staple(const PlaneCtx& pc, int pt, int tlsm, double c0, Su3& S) {
-    {
-        // right: U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
-        Su3 A, B, T;
-        double ka, kb, kc;
-        pc_link2(pc, 1, 1, 0, pt, A, ka);
-        pc_link(pc, 0, 0, 1, pt, B, kb);
-        mul_dag_r2(A, B, T);
-        pc_link(pc, 1, 0, 0, pt, B, kc);
-        mul_dag_r2(T, B, T);
-        rebuild3row(T);
-        add_term(ka * kb * kc * c0, T, 0, S);
-        // left: U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu), formed as its conjugate
-        // U_nu(x-nu)^+ U_mu(x-nu) U_nu(x+mu-nu) and added transposed
-        pc_link(pc, 1, 0, -1, pt, B, kc);
-        dag_r2(B, T);
-        pc_link(pc, 0, 0, -1, pt, B, kb);
-        mul_r2(T, B, T);
-        pc_link(pc, 1, 1, -1, pt, B, ka);
-        mul_r2(T, B, T);
-        rebuild3row(T);
-        add_term(ka * kb * kc * c0, T, 1, S);
+// A link of a staple term: U_d(x + a e_mu + b e_nu), d = mu (which = 0) or
+// nu (1), daggered when dag.
+struct HopD {
+    int which, a, b, dag;
+};
+// T = L0 L1 ... L(N-1) with all rows; *k the product of the links' K
+template <int N>
+__device__ __forceinline__ void chain(const PlaneCtx& pc, const HopD (&h)[N], int pt, Su3& T, double& k) {
+    Su3 M;
+    double kj;
+    if (h[0].dag) {
+        pc_link(pc, h[0].which, h[0].a, h[0].b, pt, M, kj);
+        dag_r2(M, T);
+    } else {
+        pc_link2(pc, h[0].which, h[0].a, h[0].b, pt, T, kj);
     }
-    if (tlsm) {
-        // one term at a time (a runtime loop keeps the live matrices few)
+    k = kj;
+#pragma unroll
+    for (int j = 1; j < N; j++) {
+        pc_link(pc, h[j].which, h[j].a, h[j].b, pt, M, kj);
+        if (h[j].dag) mul_dag_r2(T, M, T); else mul_r2(T, M, T);
+        k *= kj;
+    }
+    rebuild3row(T);
+}
+// sl[e kGfLinks] (e = 2 (3 r + c) + re/im) += s T: a thread's staple sum in its
+// own LDS slots, which keeps 36 registers free for the product chains
+constexpr int kGfLinks = 64;
+__device__ __forceinline__ void lds_axpy(double s, const Su3& T, double* sl) {
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int e = 2 * (3 * r + c);
+            sl[e * kGfLinks] = fma(s, T.c[r][c].x, sl[e * kGfLinks]);
+            sl[(e + 1) * kGfLinks] = fma(s, T.c[r][c].y, sl[(e + 1) * kGfLinks]);
+        }
+}
+// acc += s T (all rows)
+__device__ __forceinline__ void axpy3(double s, const Su3& T, Su3& acc) {
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            acc.c[r][c] = make_double2(fma(s, T.c[r][c].x, acc.c[r][c].x), fma(s, T.c[r][c].y, acc.c[r][c].y));
+}
+
+// The staple terms of link (x, mu) on one side of the plane (mu, nu): the
+// plaquette a m n and the rectangles of types A, B, C on that side
+// (plaquettes.c:83-195, rectangles.c:691-718, 780-807, 865-892), each
+// times C_ZERO or C_ONE and the K of its links.  On the upper side
+//   a = U_nu(x+mu), m = U_mu(x+nu)^+, n = U_nu(x)^+,
+//   A = a Q0 n, B = Q2 m n, C = a m Q4 with
+//   Q0 = U_nu(x+mu+nu) U_mu(x+2nu)^+ U_nu(x+nu)^+,
+//   Q2 = U_mu(x+mu) U_nu(x+2mu) U_mu(x+mu+nu)^+,
+//   Q4 = U_mu(x-mu+nu)^+ U_nu(x-mu)^+ U_mu(x-mu),
+// and the lower side is its mirror in nu (a = U_nu(x+mu-nu)^+,
+// m = U_mu(x-nu)^+, n = U_nu(x-nu)).  The four terms share factors:
+//   (c0 a m + c1 a Q0 + c1 Q2 m) n + c1 (a m) Q4,
+// 14 link loads and 207 complex multiply-adds instead of 18 and 252 for
+// the terms taken one by one; the sum is the same up to rounding.
+// c_side[side] = {a, m, n, Q0[3], Q2[3], Q4[3]}
+__device__ __constant__ HopD c_side[2][12] = {
+    {{1, 1, 0, 0}, {0, 0, 1, 1}, {1, 0, 0, 1},
+     {1, 1, 1, 0}, {0, 0, 2, 1}, {1, 0, 1, 1},
+     {0, 1, 0, 0}, {1, 2, 0, 0}, {0, 1, 1, 1},
+     {0, -1, 1, 1}, {1, -1, 0, 1}, {0, -1, 0, 0}},
+    {{1, 1, -1, 1}, {0, 0, -1, 1}, {1, 0, -1, 0},
+     {1, 1, -2, 1}, {0, 0, -2, 1}, {1, 0, -2, 0},
+     {0, 1, 0, 0}, {1, 2, -1, 1}, {0, 1, -1, 1},
+     {0, -1, -1, 1}, {1, -1, -1, 0}, {0, -1, 0, 0}},
+};
+
+__device__ __forceinline__ void plane_staple(const PlaneCtx& pc, int pt, int tlsm, double c0, double* sl) {
 #pragma unroll 1
-        for (int t = 0; t < 6; t++) {
-            const Hop* h = c_rect_hops[t];
-            switch (c_rect_shape[t]) {
-                case kPPMMM: five<kPPMMM>(pc, h, pt, kCOne, S); break;
-                case kPMMMP: five<kPMMMP>(pc, h, pt, kCOne, S); break;
-                default: five<kMMMPP>(pc, h, pt, kCOne, S); break;
+    for (int side = 0; side < 2; side++) {
+        const HopD* d = c_side[side];
+        Su3 G, T;
+        double k;
+        su3_zero(G);
+        {
+            // a m: into G with c0, and with Q4 into S
+            Su3 am;
+            double k_am;
+            const HopD h[2] = {d[0], d[1]};
+            chain(pc, h, pt, am, k_am);
+            axpy3(c0 * k_am, am, G);
+            if (tlsm) {
+                const HopD h4[3] = {d[9], d[10], d[11]};
+                chain(pc, h4, pt, T, k);
+                Su3 X;
+                mul_r2(am, T, X);
+                rebuild3row(X);
+                lds_axpy(kCOne * k_am * k, X, sl);
             }
         }
+        if (tlsm) {
+            const HopD h0[4] = {d[0], d[3], d[4], d[5]};
+            chain(pc, h0, pt, T, k);
+            axpy3(kCOne * k, T, G);
+            const HopD h2[4] = {d[6], d[7], d[8], d[1]};
+            chain(pc, h2, pt, T, k);
+            axpy3(kCOne * k, T, G);
+        }
+        // (...) n
+        Su3 n;
+        double kn;
+        pc_link(pc, d[2].which, d[2].a, d[2].b, pt, T, kn);
+        if (d[2].dag) {
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) n.c[r][c] = cconj(T.c[c][r]);
+        } else {
+            n = T;
+        }
+        su3_mul(G, n, T);
+        lds_axpy(kn, T, sl);
     }
 }
 
@@ -372,8 +456,7 @@ __device__ __forceinline__ void plane_staple(const PlaneCtx& pc, int pt, int tls
 // through LDS in plane order and multiplied by K_mu(x).  Wave 0 then writes
 // ipdot and, with mom, the momentum step of mom_sum_mult
 // (su3_utilities.h:1383-1394).
-constexpr int kGfLinks = 64;
-__global__ __launch_bounds__(3 * kGfLinks) void k_gauge_force(GaugePtrs G, double2* __restrict__ stap,
+__global__ __launch_bounds__(3 * kGfLinks, 2) void k_gauge_force(GaugePtrs G, double2* __restrict__ stap,
                                                              size_t su3_stride, void* __restrict__ ipdot, LinkGeom g,
                                                              int pt, int tlsm, double c0, void* __restrict__ mom,
                                                              double fact) {
@@ -387,8 +470,9 @@ __global__ __launch_bounds__(3 * kGfLinks) void k_gauge_force(GaugePtrs G, doubl
     const int idxh = (blk - dir * nbpd) * kGfLinks + lane;
     const bool live = idxh < g.sizeh;
     const int mu = dir >> 1;
-    Su3 S;
-    su3_zero(S);
+    double* sl = &red[w][0][lane];
+#pragma unroll
+    for (int e = 0; e < 18; e++) sl[e * kGfLinks] = 0.0;
     Tamat q = {};
     if (live) {
         int x[4];
@@ -409,13 +493,33 @@ __global__ __launch_bounds__(3 * kGfLinks) void k_gauge_force(GaugePtrs G, doubl
         pc.lex = x[0] + st[1] * x[1] + st[2] * x[2] + st[3] * x[3];
         pc.xm = sel(x, pc.mu);
         pc.xn = sel(x, pc.nu);
-        plane_staple(pc, pt, tlsm, c0, S);
-        Su3 U, X;
+        plane_staple(pc, pt, tlsm, c0, sl);
+        Su3 S, U, X;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                S.c[r][c] = make_double2(sl[2 * (3 * r + c) * kGfLinks], sl[(2 * (3 * r + c) + 1) * kGfLinks]);
         double k;
         pc_link(pc, 0, 0, 0, pt, U, k);
         su3_mul(U, S, X);
         ta_part(X, 1.0, q);
     }
+    const double kmu = (live && pt) ? ((const double*)(G.at(dir) + 9 * (size_t)g.sizeh))[idxh] : 1.0;
+    if (stap) {
+        // the three planes' staple sums are in red
+        __syncthreads();
+        if (w == 0 && live) {
+            double2* O = (double2*)((char*)stap + dir * su3_stride);
+#pragma unroll
+            for (int e = 0; e < 9; e++) {
+                const double re = kmu * (red[0][2 * e][lane] + red[1][2 * e][lane] + red[2][2 * e][lane]);
+                const double im = kmu * (red[0][2 * e + 1][lane] + red[1][2 * e + 1][lane] + red[2][2 * e + 1][lane]);
+                O[(size_t)e * g.sizeh + idxh] = make_double2(re, im);
+            }
+        }
+    }
+    __syncthreads();
     red[w][0][lane] = q.c01.x;
     red[w][1][lane] = q.c01.y;
     red[w][2][lane] = q.c02.x;
@@ -425,7 +529,6 @@ __global__ __launch_bounds__(3 * kGfLinks) void k_gauge_force(GaugePtrs G, doubl
     red[w][6][lane] = q.ic00;
     red[w][7][lane] = q.ic11;
     __syncthreads();
-    const double kmu = (live && pt) ? ((const double*)(G.at(dir) + 9 * (size_t)g.sizeh))[idxh] : 1.0;
     if (w == 0 && live) {
         double t[8];
 #pragma unroll
@@ -438,26 +541,6 @@ __global__ __launch_bounds__(3 * kGfLinks) void k_gauge_force(GaugePtrs G, doubl
         o.ic11 = t[7];
         ta_store(ipdot, dir, g.sizeh, idxh, o);
         if (mom) mom_step(mom, dir, g.sizeh, idxh, o, fact);
-    }
-    if (stap) {
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                red[w][2 * (3 * r + c)][lane] = S.c[r][c].x;
-                red[w][2 * (3 * r + c) + 1][lane] = S.c[r][c].y;
-            }
-        __syncthreads();
-        if (w == 0 && live) {
-            double2* O = (double2*)((char*)stap + dir * su3_stride);
-#pragma unroll
-            for (int e = 0; e < 9; e++) {
-                const double re = kmu * (red[0][2 * e][lane] + red[1][2 * e][lane] + red[2][2 * e][lane]);
-                const double im = kmu * (red[0][2 * e + 1][lane] + red[1][2 * e + 1][lane] + red[2][2 * e + 1][lane]);
-                O[(size_t)e * g.sizeh + idxh] = make_double2(re, im);
-            }
-        }
     }
 }
 

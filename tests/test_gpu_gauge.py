@@ -127,9 +127,9 @@ def expm_h(h):
     return (v * np.exp(1j * w)) @ v.conj().T
 
 
+@pytest.mark.parametrize("shape", [(4, 4, 4, 6), (6, 6, 6, 6)])  # half-volume 192 and 648 (not a multiple of 64)
 @pytest.mark.parametrize("tlsm,par_temp", [(1, 0), (0, 0), (1, 1), (0, 1)])
-def test_staple_is_action_derivative(tlsm, par_temp):
-    shape = (4, 4, 4, 6)
+def test_staple_is_action_derivative(tlsm, par_temp, shape):
     f = Field(shape, bool(par_temp), 5)
     set_action(tlsm)
     stap, ipd = gpu_force(f)
