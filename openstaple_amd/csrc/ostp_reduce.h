@@ -258,6 +258,78 @@ __device__ __forceinline__ void block_partial_at(double v0, double v1, double* p
     }
 }
 
+// The CG-M finalizers (kFinMsAlpha / kFinMsLambda and their FP32 twins) with
+// lane k of wave 0 running the recurrence of shift k: the shifts are
+// independent given the base scalars, and the expressions are those of
+// finalize<F>, so the results are bit-identical to its serial loops (which
+// wait on device memory once per shift).  Every lane reads the old scalars
+// before lane 0 stores the new ones (one wave, program order; the per-shift
+// arrays and the scalars do not overlap).  maxiter (highest active shift + 1)
+// is a ballot.  Needs kMaxShifts <= 64.
+static_assert(kMaxShifts <= 64, "one lane per shift");
+template <int F>
+__device__ __forceinline__ void finalize_ms_lanes(SolverState* st, double tot, int k) {
+    if (F == kFinMsAlpha || F == kFinMxmAlpha) {
+        const int cur = st->flag_cur ^ 1;
+        const int maxiter = st->maxiter_next;
+        const double omega_save = st->omega;
+        const double omega = -st->delta / tot;
+        const double gammag = st->gammag;
+        if (k < maxiter && st->flags[cur][k] == 1) {
+            const double zi = st->zeta_i[k], zii = st->zeta_ii[k];
+            const double ziii =
+                (zi * zii * omega_save) / (omega * gammag * (zi - zii) + zi * omega_save * (1.0 - (st->shifts[k]) * omega));
+            st->zeta_iii[k] = ziii;
+            st->omegas[k] = omega * ziii / zii;
+        }
+        if (k == 0) {
+            st->flag_cur = cur;
+            st->maxiter = maxiter;
+            st->alpha = tot;
+            st->omega_save = omega_save;
+            st->omega = omega;
+            if (F == kFinMxmAlpha) {
+                if (st->last_max_res < st->delta) st->last_max_res = st->delta;
+                st->ru = (st->delta < st->mixed_delta * st->last_max_res) ? 1 : 0;
+            }
+        }
+    } else {
+        const double lambda = tot;
+        const int cur = st->flag_cur, nxt = cur ^ 1;
+        const double delta = st->delta, omega = st->omega;
+        const double gammag = lambda / delta;
+        const double target = st->res * st->safety;
+        const double src = st->source_norm;
+        const int order = st->order;
+        int f = 0;
+        if (k < order) {
+            f = st->flags[cur][k];
+            if (f == 1) {
+                st->gammas[k] = gammag * st->zeta_iii[k] * st->omegas[k] / (st->zeta_ii[k] * omega);
+                const double fact = sqrt(delta * st->zeta_ii[k] * st->zeta_ii[k] / src);
+                if (fact < target) f = 0;
+                st->zeta_i[k] = st->zeta_ii[k];
+                st->zeta_ii[k] = st->zeta_iii[k];
+            }
+            st->flags[nxt][k] = f;
+        }
+        const unsigned long long act = __ballot(k < order && f == 1);
+        const int maxiter = act ? 64 - __clzll(act) : 0;
+        if (k == 0) {
+            st->lambda = lambda;
+            st->gammag = gammag;
+            st->maxiter_next = maxiter;
+            st->delta = lambda;
+            st->cg += 1;
+            if (!(maxiter > 0 && st->cg < st->max_cg)) st->status = kDone;
+            if (F == kFinMxmLambda && st->ru) {
+                st->last_max_res = 0;
+                st->ru_count += 1;
+            }
+        }
+    }
+}
+
 // Sum `nparts` block partials in block order and apply finalizer F.  One
 // block; launched on the stream of the reducing kernel, with its SolverCtl
 // (the predicate reads state that only finalizers write, so it evaluates as
@@ -271,10 +343,16 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverCtl c, int nparts) {
         a0 += c.partials[2 * i];
         if (NV == 2) a1 += c.partials[2 * i + 1];
     }
-    const double t0 = block_sum<kBlock>(a0, lds[0]);
-    double t1 = 0.0;
-    if (NV == 2) t1 = block_sum<kBlock>(a1, lds[1]);
-    if (threadIdx.x == 0) finalize<F>(c.st, t0, t1);
+    constexpr bool kMs = F == kFinMsAlpha || F == kFinMxmAlpha || F == kFinMsLambda || F == kFinMxmLambda;
+    if constexpr (kMs) {
+        const double t0 = block_sum_all<kBlock>(a0, lds[0]);
+        if (threadIdx.x < 64) finalize_ms_lanes<F>(c.st, t0, (int)threadIdx.x);
+    } else {
+        const double t0 = block_sum<kBlock>(a0, lds[0]);
+        double t1 = 0.0;
+        if (NV == 2) t1 = block_sum<kBlock>(a1, lds[1]);
+        if (threadIdx.x == 0) finalize<F>(c.st, t0, t1);
+    }
 }
 
 // Host: queue k_finalize<fin> for a reduction over `nparts` blocks.
