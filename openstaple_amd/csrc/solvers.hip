@@ -207,7 +207,9 @@ __global__ __launch_bounds__(kBlock) void k_msw_p(C* __restrict__ p, const C* __
 }
 
 // x_k += a_k p_k + sum_j c_kj r_j ; p_k = B_k p_k + sum_j D_kj r_j for the
-// shifts that moved, four shifts per pass over the residuals
+// shifts that moved, kMswGroup shifts per pass over the residuals (eight: three
+// passes for 19 shifts instead of five with four, each shift's sums unchanged)
+constexpr int kMswGroup = 8;
 template <typename C>
 __global__ __launch_bounds__(kBlock) void k_msw_flush(C* __restrict__ xs, C* __restrict__ ps,
                                                       const C* __restrict__ hist, long n3, int order,
@@ -217,13 +219,15 @@ __global__ __launch_bounds__(kBlock) void k_msw_flush(C* __restrict__ xs, C* __r
     if (t == 0) return;
     const long i = (long)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n3) return;
-    for (int k0 = 0; k0 < order; k0 += 4) {
-        double2 ax[4], ap[4];
-        bool on[4];
+    for (int k0 = 0; k0 < order; k0 += kMswGroup) {
+        double2 ax[kMswGroup], ap[kMswGroup];
+        bool on[kMswGroup];
+        bool any = false;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < kMswGroup; q++) {
             const int k = k0 + q;
             on[q] = k < order && w->active[k];
+            any = any || on[q];
             ax[q] = make_double2(0.0, 0.0);
             ap[q] = make_double2(0.0, 0.0);
             if (on[q]) {
@@ -233,12 +237,12 @@ __global__ __launch_bounds__(kBlock) void k_msw_flush(C* __restrict__ xs, C* __r
                 ap[q] = make_double2(b * (double)pv.x, b * (double)pv.y);
             }
         }
-        if (!(on[0] || on[1] || on[2] || on[3])) continue;
+        if (!any) continue;
         for (int j = 0; j < t; j++) {
             const C rv = hist[(long)j * n3 + i];
             const double rx = (double)rv.x, ry = (double)rv.y;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < kMswGroup; q++) {
                 if (!on[q]) continue;
                 const double c = w->c[k0 + q][j], d = w->D[k0 + q][j];
                 ax[q].x += c * rx;
@@ -248,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_msw_flush(C* __restrict__ xs, C* __r
             }
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++)
+        for (int q = 0; q < kMswGroup; q++)
             if (on[q]) {
                 xs[(k0 + q) * n3 + i] = mkc<C>(ax[q].x, ax[q].y);
                 ps[(k0 + q) * n3 + i] = mkc<C>(ap[q].x, ap[q].y);
