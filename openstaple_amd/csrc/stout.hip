@@ -463,11 +463,26 @@ __global__ __launch_bounds__(3 * kGfLinks, 2) void k_gauge_force(GaugePtrs G, do
     __shared__ double red[3][18][kGfLinks];
     const int lane = (int)threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    // a block covers 64 half-sites of one link array: dir is uniform
-    const int nbpd = (g.sizeh + kGfLinks - 1) / kGfLinks;
+    // A block covers 64 half-sites of one link array (dir uniform).  Logical
+    // blocks run over the link array fastest, then t, then the 64-site tile
+    // of the t-slice, and each XCD takes a contiguous range of them: an XCD
+    // owns a slab of tiles for every t and every link array, and the blocks
+    // it runs together cover one tile at neighbouring t, so the neighbour
+    // links of a staple are re-read from its L2 (with each XCD sweeping the
+    // whole lattice for one link array, every XCD fetched every neighbour
+    // link from memory: 22 GB per launch at 48^3x12).  A t-slice that is not
+    // a whole number of tiles keeps the plain half-site order.
     const int blk = xcd_block_id();
-    const int dir = __builtin_amdgcn_readfirstlane(blk / nbpd);
-    const int idxh = (blk - dir * nbpd) * kGfLinks + lane;
+    const int dir = __builtin_amdgcn_readfirstlane(blk & 7);
+    const int rest = blk >> 3;
+    const int s3 = g.sizeh / g.nd[3];
+    int idxh;
+    if (s3 % kGfLinks == 0) {
+        const int tile = rest / g.nd[3], tt = rest - tile * g.nd[3];
+        idxh = tt * s3 + tile * kGfLinks + lane;
+    } else {
+        idxh = rest * kGfLinks + lane;
+    }
     const bool live = idxh < g.sizeh;
     const int mu = dir >> 1;
     double* sl = &red[w][0][lane];
