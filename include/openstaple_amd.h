@@ -26,6 +26,7 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -158,7 +159,10 @@ void fermion_matrix_multiplication_shifted_f(const su3_soa_f *u, vec3_soa_f *out
 /* fermionic_utilities.h:45-118 (single rank) */
 double l2norm2_global(const vec3_soa *in_vect1);
 double real_scal_prod_global(const vec3_soa *in_vect1, const vec3_soa *in_vect2);
-/* scal_prod_global returns d_complex; split into re/im for a plain C ABI */
+/* fermionic_utilities.c:126-145: sum over half-sites of conj(in1) . in2,
+ * returned as the reference's d_complex (C99 double complex) */
+double _Complex scal_prod_global(const vec3_soa *in_vect1, const vec3_soa *in_vect2);
+/* the same, split into re/im for callers without a complex type (ctypes) */
 void   ostp_scal_prod_global(const vec3_soa *in_vect1, const vec3_soa *in_vect2, double *re, double *im);
 void combine_in1xfactor_plus_in2(const vec3_soa *in_vect1, const double factor,
                                  const vec3_soa *in_vect2, vec3_soa *out);
@@ -495,6 +499,16 @@ int read_su3_soa_ASCII(global_su3_soa *conf, const char *nomefile, int *conf_id_
 int print_su3_soa_ildg_binary(const global_su3_soa *conf, const char *nomefile, int conf_id_iter);
 int read_su3_soa_ildg_binary(global_su3_soa *conf, const char *nomefile, int *conf_id_iter);
 /* dbgtools.c:1193-1236 */
+/* io.c:553-614 (single rank): visit the global sites in ILDG file order
+ * (t, z, y, x, x fastest; the four directions x, y, z, t of each site, or
+ * only the even sites and one call each with scalar_even_mode) and call
+ * single_element_rw(half-site index in the machine layout, parity, machine
+ * direction, datastruct, 1 if this host is little endian, fp).  Extents and
+ * direction map as for the field files. */
+void rw_iterate_on_global_sites_lx_xyzt_axis_ordering(
+    void (*single_element_rw)(const int idxh_machine, const int parity, const int dirmachine,
+                              const void *data, const int conf_machine_endianness_disagreement, FILE *fp),
+    const void *datastruct, FILE *fp, const int scalar_even_mode);
 void print_vec3_soa(const vec3_soa *fermion, const char *nomefile);
 int  read_vec3_soa(vec3_soa *fermion, const char *nomefile);
 /* save_conf_wrapper / read_conf_wrapper (io.h:51-150) on the library's
@@ -622,8 +636,19 @@ void ostp_generate_Conf_expH(su3_soa *conf, double beta, uint64_t seed);
  * timed with HIP events on the library's stream. */
 double ostp_time_dslash(const void *u, void *out, const void *in, const void *phases,
                         int eo, int single, int reps);
-/* Mean device time in microseconds of one plain copy of `bytes` bytes. */
-double ostp_time_copy(void *dst, const void *src, size_t bytes, int reps);
+/* Mean device time in microseconds of one streaming pass over `bytes` bytes:
+ * kind 0 copies src -> dst, 1 reads src, 2 writes dst; 128-bit non-temporal
+ * accesses, `unroll` (1, 2, 4, 8) loads in flight per thread, `grid`
+ * workgroups of 256 threads.  bytes must be a multiple of 4096 x unroll.
+ * The HBM ceiling the Dslash bandwidth is compared with. */
+double ostp_time_stream(int kind, void *dst, const void *src, size_t bytes, int reps, int grid,
+                        int unroll);
+/* Alternate the walk direction of consecutive Dslash launches (on by
+ * default): a launch then starts on the links the previous launch streamed
+ * last, which the memory-side cache still holds.  Results do not depend on
+ * it (reductions are summed in lattice-tile order).  Returns the previous
+ * setting. */
+int ostp_set_dslash_serpentine(int on);
 
 #ifdef __cplusplus
 }

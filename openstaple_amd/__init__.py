@@ -198,6 +198,17 @@ PTC_NPARAM = 1 + 2 * 8 * PTC_CHANNELS
 
 PT_ACTION_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_void_p)
 PT_UNIFORM_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
+# io.c:553-559 callback: (idxh machine, parity, dirmachine, data, endianness disagreement, FILE*)
+RW_SITE_FN = C.CFUNCTYPE(None, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p)
+
+
+class DComplex(C.Structure):
+    """A C99 double complex return value: on x86-64 it travels in two SSE
+    registers, exactly like this two-double struct."""
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+    def __complex__(self):
+        return complex(self.re, self.im)
 
 assert C.sizeof(RationalApprox) == 456
 assert FermParam.phases.offset == 72 and FermParam.phases_f.offset == 104
@@ -246,6 +257,8 @@ _SIGS = {
     "l2norm2_global": (_D, [_P]),
     "real_scal_prod_global": (_D, [_P, _P]),
     "ostp_scal_prod_global": (None, [_P, _P, C.POINTER(_D), C.POINTER(_D)]),
+    "scal_prod_global": (DComplex, [_P, _P]),
+    "rw_iterate_on_global_sites_lx_xyzt_axis_ordering": (None, [RW_SITE_FN, _P, _P, _I]),
     "combine_in1xfactor_plus_in2": (None, [_P, _D, _P, _P]),
     "combine_add_factor_x_in2_to_in1": (None, [_P, _P, _D]),
     "combine_in1xferm_mass_minus_in2": (None, [_P, _D, _P]),
@@ -353,7 +366,8 @@ _SIGS = {
     "ostp_ptc_loss_grad": (_D, [_P, _P, _P, _P, _P, _P, _I, _P]),
     "ostp_fcg_solve": (_I, [_P, _P, _P, _P, _P, _P, _D, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "ostp_time_dslash": (_D, [_P, _P, _P, _P, _I, _I, _I]),
-    "ostp_time_copy": (_D, [_P, _P, C.c_size_t, _I]),
+    "ostp_time_stream": (_D, [_I, _P, _P, C.c_size_t, _I, _I, _I]),
+    "ostp_set_dslash_serpentine": (_I, [_I]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

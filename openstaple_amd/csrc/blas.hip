@@ -55,6 +55,12 @@ __global__ __launch_bounds__(kBlock) void k_convert(const A* __restrict__ in, B*
     out[i] = mkc<B>((double)a.x, (double)a.y);
 }
 
+__global__ __launch_bounds__(kBlock) void k_convert_real(const double* __restrict__ in, float* __restrict__ out,
+                                                         long n) {
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
 // --------------------------------------------------------------- reductions
 // sum over sizeh half-sites of f(a(i), b(i)) over 3 colours
 template <typename C, int KIND, int F>
@@ -226,6 +232,15 @@ void ostp_scal_prod_global(const vec3_soa* in_vect1, const vec3_soa* in_vect2, d
     if (a && b) reduce_to_host(2, a, b, 0, re, im);
 }
 
+double _Complex scal_prod_global(const vec3_soa* in_vect1, const vec3_soa* in_vect2) {
+    double re = 0.0, im = 0.0;
+    ostp_scal_prod_global(in_vect1, in_vect2, &re, &im);
+    double _Complex z;
+    __real__ z = re;
+    __imag__ z = im;
+    return z;
+}
+
 static void finish(const char* what) { entry_done(what); }
 
 void combine_in1xfactor_plus_in2(const vec3_soa* in_vect1, const double factor, const vec3_soa* in_vect2,
@@ -324,8 +339,8 @@ void ostp_convert_double_to_float_gauge(const su3_soa* d_var, su3_soa_f* f_var) 
         if (L.par_temp) {  // K plane: double -> float
             const double* K = (const double*)(d + l * L.su3_bytes() + 144 * L.sizeh);
             float* Kf = (float*)(f + l * L.su3f_bytes() + 72 * L.sizeh);
-            // pairs of reals converted as (x,y)
-            if (!launch_d2f(K, Kf, L.sizeh / 2)) return;
+            hipLaunchKernelGGL(k_convert_real, dim3(nb(L.sizeh)), dim3(kBlock), 0, S(), K, Kf, L.sizeh);
+            if (!hip_ok(hipGetLastError(), "k_convert_real")) return;
         }
     }
     finish("ostp_convert_double_to_float_gauge");

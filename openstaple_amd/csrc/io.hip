@@ -167,6 +167,38 @@ void ildg_sites(const Lattice& L, F&& f) {
 
 extern "C" {
 
+void rw_iterate_on_global_sites_lx_xyzt_axis_ordering(
+    void (*single_element_rw)(const int, const int, const int, const void*, const int, FILE*), const void* datastruct,
+    FILE* fp, const int scalar_even_mode) {
+    const char* nm = "rw_iterate_on_global_sites_lx_xyzt_axis_ordering";
+    const Lattice* L = io_lattice(nm);
+    if (!L) return;
+    if (!single_element_rw) {
+        fail("%s: null callback", nm);
+        return;
+    }
+    int n[4];
+    phys_extents(*L, n);
+    int d0123[4];
+    for (int k = 0; k < 4; k++) d0123[L->map[k]] = k;
+    const int swap = little_endian() ? 1 : 0;
+    const int xlimit = scalar_even_mode ? n[0] / 2 : n[0];
+    const int dirlimit = scalar_even_mode ? 1 : 4;
+    for (int t = 0; t < n[3]; t++)
+        for (int z = 0; z < n[2]; z++)
+            for (int y = 0; y < n[1]; y++)
+                for (int xt = 0; xt < xlimit; xt++) {
+                    const int x = scalar_even_mode ? 2 * xt + ((y + z + t) & 1) : xt;
+                    const int parity = scalar_even_mode ? 0 : (x + y + z + t) % 2;
+                    const int xs[4] = {x, y, z, t};
+                    int d[4];
+                    for (int k = 0; k < 4; k++) d[k] = xs[d0123[k]];
+                    const int idxh = (int)snum_h(*L, d);
+                    for (int dir = 0; dir < dirlimit; dir++)
+                        single_element_rw(idxh, parity, L->map[dir], datastruct, swap, fp);
+                }
+}
+
 int ostp_set_lattice_host(int nd0, int nd1, int nd2, int nd3, int par_temp) {
     if (nd0 <= 0 || nd1 <= 0 || nd2 <= 0 || nd3 <= 0 || (nd0 & 1))
         return fail("ostp_set_lattice_host: invalid lattice %dx%dx%dx%d", nd0, nd1, nd2, nd3) ? 0 : 1;
