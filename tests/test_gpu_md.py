@@ -226,7 +226,7 @@ class FermionMD:
     and the fermion action <chi, r(M^+M) chi> through the library's own
     multishift solver and recombination."""
 
-    def __init__(self, shape, seed, mass=0.1, nps=2, res=1e-12):
+    def __init__(self, shape, seed, mass=0.1, nps=2, res=1e-12, bf=(0, 0, 0, 0, 0, 0), mu=0.0, charge=0.0):
         o = Oracle(*shape)
         init_lattice(shape)
         lib.ostp_set_stout(0.15, 2, 0.0, 0, 0, 1)
@@ -234,7 +234,7 @@ class FermionMD:
         self.o, self.nps, self.res = o, nps, res
         self.u = o.gen_expH(0.3, seed)
         self.lat = lat = ost.Lattice(*shape)
-        self.ph = Dev(o.phases())
+        self.ph = Dev(o.phases(bf=bf, mu=mu, charge=charge))
         self.ra = approx_md9()
         order = self.ra.approx_order
         self.chi = Dev(np.stack([o.gen_gauss(90 + i) for i in range(nps)]))
@@ -280,9 +280,13 @@ class FermionMD:
         return tot
 
 
-def test_full_md_trajectory_reversible_and_second_order():
+# no background field (phases +-1), and an e.m. field with an imaginary
+# chemical potential (complex phases in the Dslash and in the force)
+@pytest.mark.parametrize("field", [dict(), dict(bf=(0.7, -1.3, 0.4, 1.0, 3.0, -2.0), mu=0.25, charge=2.0 / 3.0)],
+                         ids=["no_field", "em_field_imag_mu"])
+def test_full_md_trajectory_reversible_and_second_order(field):
     shape = (4, 4, 4, 8)
-    F = FermionMD(shape, 13)
+    F = FermionMD(shape, 13, **field)
     sizeh = F.o.sizeh
     beta, t = 5.0, 0.5
     mom = momenta(sizeh, 6)
