@@ -31,6 +31,7 @@ struct DslashArgs {
     double m2;
     SiteGeom g;
     int tmajor;         // workgroup order: t-slices fastest (see k_dslash)
+    int rev;            // each XCD walks its slab backwards
     SolverCtl ctl;
 };
 
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> a) {
     // so the +-t neighbour slices of a tile are streamed by the same XCD at
     // nearly the same time and its L2 serves their re-reads (FP64: 81.2 us vs
     // 85.5 us per 32^4 Dslash).
-    const int blk = xcd_block_id();
+    const int blk = xcd_block_id_dir(a.rev);
     int idx;
     if (a.tmajor) {
         const int s3 = a.g.sizeh / a.g.nd3;
@@ -86,7 +87,8 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> a) {
             if (MODE == kMdagMDot) dot = re_dot(x0, s0) + re_dot(x1, s1v) + re_dot(x2, s2v);
         }
     }
-    if (MODE == kMdagMDot) block_partial<1>(dot, 0.0, a.ctl);
+    // partial slot = logical tile, so the reduction order is the same in both walk directions
+    if (MODE == kMdagMDot) block_partial_at<kBlock>(dot, 0.0, a.ctl.partials, blk);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -114,6 +116,7 @@ static bool fill_args(DslashArgs<R>& a, const void* u_dev, const void* ph_dev, c
     // t-fastest tiles need whole tiles per t-slice; used for FP64 only (no gain in FP32)
     const long s3 = L.sizeh / L.nd[3];
     a.tmajor = (sizeof(R) == 8 && s3 % kBlock == 0) ? 1 : 0;
+    a.rev = 0;
     return true;
 }
 
@@ -126,6 +129,7 @@ bool launch_dslash(int eo, int mode, int fin, const void* u, const void* ph, con
     a.x = (const typename Cplx<R>::T*)x;
     a.m2 = m2;
     a.ctl = ctl;
+    a.rev = ctx().dslash_serpentine ? (int)(ctx().dslash_launches++ & 1u) : 0;
     const int nb = blocks_for(ctx().lat.sizeh);
     hipStream_t s = ctx().stream;
 #define OSTP_L(EO, MODE) hipLaunchKernelGGL((k_dslash<R, EO, MODE>), dim3(nb), dim3(kBlock), 0, s, a)
@@ -247,21 +251,22 @@ double ostp_time_dslash(const void* u, void* out, const void* in, const void* ph
     const void* di = resolve(in, vb, "ostp_time_dslash");
     void* dout = resolve(out, vb, "ostp_time_dslash");
     if (!du || !dp || !di || !dout || reps <= 0) return -1.0;
-    hipEvent_t e0, e1;
-    if (!hip_ok(hipEventCreate(&e0), "event") || !hip_ok(hipEventCreate(&e1), "event")) return -1.0;
-    (void)hipEventRecord(e0, ctx().stream);
-    for (int r = 0; r < reps; r++) {
-        bool ok = single ? launch_dslash<float>(eo, kPlain, kFinStore, du, dp, di, dout, nullptr, 0.0, SolverCtl{})
-                         : launch_dslash<double>(eo, kPlain, kFinStore, du, dp, di, dout, nullptr, 0.0, SolverCtl{});
-        if (!ok) return -1.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double us = -1.0;
+    if (hip_ok(hipEventCreate(&e0), "event") && hip_ok(hipEventCreate(&e1), "event") &&
+        hip_ok(hipEventRecord(e0, ctx().stream), "event record")) {
+        bool ok = true;
+        for (int r = 0; r < reps && ok; r++)
+            ok = single ? launch_dslash<float>(eo, kPlain, kFinStore, du, dp, di, dout, nullptr, 0.0, SolverCtl{})
+                        : launch_dslash<double>(eo, kPlain, kFinStore, du, dp, di, dout, nullptr, 0.0, SolverCtl{});
+        float ms = 0.f;
+        if (ok && hip_ok(hipEventRecord(e1, ctx().stream), "event record") &&
+            hip_ok(hipEventSynchronize(e1), "event sync") && hip_ok(hipEventElapsedTime(&ms, e0, e1), "elapsed"))
+            us = 1000.0 * (double)ms / reps;
     }
-    (void)hipEventRecord(e1, ctx().stream);
-    (void)hipEventSynchronize(e1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return 1000.0 * (double)ms / reps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return us;
 }
 
 }  // extern "C"

@@ -266,9 +266,33 @@ __global__ __launch_bounds__(kBlock) void k_gen_momenta(void* __restrict__ mom, 
     th_store(mom, link, sizeh, i, h);
 }
 
-// plain copy (roofline reference for the HBM bandwidth of the box)
-__global__ __launch_bounds__(kBlock) void k_copy(const double2* __restrict__ a, double2* __restrict__ b, long n) {
-    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) b[i] = a[i];
+// Streaming probes: the HBM ceiling the Dslash is compared with.  KIND 0
+// copies a -> b, 1 only reads a, 2 only writes b.  Each block takes chunks of
+// U x 256 16-byte elements (U independent 128-bit loads in flight per thread,
+// non-temporal), chunks dealt grid-stride.  The read probe folds its loads
+// into one value per thread that is stored only if it equals a sentinel the
+// data never produce, so the loads cannot be dropped.  n must be a multiple
+// of U x 256 (checked by the host).
+template <int KIND, int U>
+__global__ __launch_bounds__(kBlock) void k_stream(const double2* __restrict__ a, double2* __restrict__ b, long n,
+                                                   double* __restrict__ sink) {
+    const long nchunk = n / ((long)U * kBlock);
+    double acc = 0.0;
+    for (long c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        const long base = c * U * kBlock + threadIdx.x;
+        double2 v[U];
+        if (KIND != 2) {
+#pragma unroll
+            for (int k = 0; k < U; k++) v[k] = ld_nt(a + base + (long)k * kBlock);
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            if (KIND == 0) st_nt(b + base + (long)k * kBlock, v[k]);
+            if (KIND == 1) acc += v[k].x + v[k].y;
+            if (KIND == 2) st_nt(b + base + (long)k * kBlock, make_double2((double)c, (double)k));
+        }
+    }
+    if (KIND == 1 && acc == -1.2345e300) sink[threadIdx.x] = acc;
 }
 
 static GeomArgs geom_args() {
@@ -383,25 +407,39 @@ void ostp_generate_Momenta_gauss(thmat_soa* mom8, uint64_t seed, uint32_t stream
     if (hip_ok(hipGetLastError(), nm)) hip_ok(hipStreamSynchronize(ctx().stream), nm);
 }
 
-double ostp_time_copy(void* dst, const void* src, size_t bytes, int reps) {
-    if (!require_init("ostp_time_copy")) return -1.0;
-    double2* b = (double2*)resolve(dst, bytes, "ostp_time_copy");
-    const double2* a = (const double2*)resolve(src, bytes, "ostp_time_copy");
-    if (!a || !b || reps <= 0) return -1.0;
+double ostp_time_stream(int kind, void* dst, const void* src, size_t bytes, int reps, int grid, int unroll) {
+    const char* nm = "ostp_time_stream";
+    if (!require_init(nm)) return -1.0;
+    if (kind < 0 || kind > 2 || reps <= 0 || grid <= 0) return fail("%s: bad kind/reps/grid", nm), -1.0;
+    if (unroll != 1 && unroll != 2 && unroll != 4 && unroll != 8) return fail("%s: unroll must be 1, 2, 4 or 8", nm), -1.0;
+    if (bytes % ((size_t)16 * kBlock * unroll) != 0)
+        return fail("%s: bytes must be a multiple of %d", nm, 16 * kBlock * unroll), -1.0;
+    // the read probe's never-taken store still needs a valid address: the first 256 doubles of src
+    double2* b = kind == 1 ? nullptr : (double2*)resolve(dst, bytes, nm);
+    const double2* a = kind == 2 ? nullptr : (const double2*)resolve(src, bytes, nm);
+    if ((kind != 1 && !b) || (kind != 2 && !a)) return -1.0;
+    double* sink = (double*)(kind == 1 ? (void*)a : (void*)b);
     const long n = (long)(bytes / 16);
-    const int grid = 256 * 16;
-    hipEvent_t e0, e1;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    (void)hipEventRecord(e0, ctx().stream);
-    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, ctx().stream, a, b, n);
-    (void)hipEventRecord(e1, ctx().stream);
-    (void)hipEventSynchronize(e1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return 1000.0 * (double)ms / reps;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double us = -1.0;
+    if (hip_ok(hipEventCreate(&e0), "event") && hip_ok(hipEventCreate(&e1), "event") &&
+        hip_ok(hipEventRecord(e0, ctx().stream), "event record")) {
+        for (int r = 0; r < reps; r++) {
+#define OSTP_S(K, U) hipLaunchKernelGGL((k_stream<K, U>), dim3(grid), dim3(kBlock), 0, ctx().stream, a, b, n, sink)
+#define OSTP_SK(K) if (unroll == 1) OSTP_S(K, 1); else if (unroll == 2) OSTP_S(K, 2); \
+                   else if (unroll == 4) OSTP_S(K, 4); else OSTP_S(K, 8)
+            if (kind == 0) { OSTP_SK(0); } else if (kind == 1) { OSTP_SK(1); } else { OSTP_SK(2); }
+#undef OSTP_SK
+#undef OSTP_S
+        }
+        float ms = 0.f;
+        if (hip_ok(hipGetLastError(), "k_stream") && hip_ok(hipEventRecord(e1, ctx().stream), "event record") &&
+            hip_ok(hipEventSynchronize(e1), "event sync") && hip_ok(hipEventElapsedTime(&ms, e0, e1), "elapsed"))
+            us = 1000.0 * (double)ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return us;
 }
 
 }  // extern "C"

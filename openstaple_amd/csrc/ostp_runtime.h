@@ -44,6 +44,11 @@ struct Context {
     int device = 0;
     int abort_on_error = 1;
     int sync_at_return = 1;   // entry points without a result wait for their kernels
+    // Dslash launches alternate the direction in which each XCD walks its
+    // slab, so a launch starts on the links and phases the previous one
+    // streamed last (still in the 256 MB memory-side cache)
+    int dslash_serpentine = 1;
+    unsigned dslash_launches = 0;
     // device scratch: reduction partials + solver state
     double* partials = nullptr;     // 2 * max_blocks
     SolverState* state = nullptr;   // device
@@ -94,4 +99,14 @@ __device__ __forceinline__ int xcd_block_id() {
     const int G = gridDim.x, i = blockIdx.x;
     const int q = G >> 3, r = G & 7, x = i & 7, k = i >> 3;
     return x * q + (x < r ? x : r) + k;
+}
+
+// The same with each XCD's range walked from its end when `rev` is set
+// (workgroups are dispatched in hardware-id order, so a reversed launch
+// starts where the previous, forward launch finished).
+__device__ __forceinline__ int xcd_block_id_dir(int rev) {
+    const int G = gridDim.x, i = blockIdx.x;
+    const int q = G >> 3, r = G & 7, x = i & 7, k = i >> 3;
+    const int len = q + (x < r ? 1 : 0);
+    return x * q + (x < r ? x : r) + (rev ? len - 1 - k : k);
 }

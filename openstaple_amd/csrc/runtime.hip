@@ -325,6 +325,12 @@ int ostp_set_sync_at_return(int on) {
     g_ctx.sync_at_return = on ? 1 : 0;
     return prev;
 }
+int ostp_set_dslash_serpentine(int on) {
+    const int prev = g_ctx.dslash_serpentine;
+    g_ctx.dslash_serpentine = on ? 1 : 0;
+    g_ctx.dslash_launches = 0;
+    return prev;
+}
 const char* ostp_last_error(void) { return g_last_error.c_str(); }
 void ostp_clear_error(void) { g_last_error.clear(); }
 
