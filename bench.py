@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--rhmc-only", action="store_true", help="run only the RHMC trajectory workload")
     ap.add_argument("--rhmc-lattice", type=str, default="48,48,48,12")
     ap.add_argument("--rhmc-steps", type=int, default=10, help="Omelyan steps per trajectory")
+    ap.add_argument("--rhmc-thermalise", type=int, default=12,
+                    help="gauge-only HMC trajectories before the 2+1-flavour ones")
     ap.add_argument("--dslash-only", action="store_true",
                     help="only the timed Dslash loop and the copy kernel (for counter runs)")
     return ap.parse_args()
@@ -70,6 +72,15 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
                                    "(t-slab halo exchange, see DESIGN.md)", "n_gpus": args.gpus}))
         sys.exit(2)
+    # the CPU baseline's builds run make: before this process touches the GPU
+    cpu_libs = None
+    if not args.no_cpu and not args.rhmc_only and not args.dslash_only:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import tempfile
+
+        import cpu_baseline as cpub
+
+        cpu_libs = cpub.build(tempfile.mkdtemp(prefix="ostp_cpu_baseline_"))
     import openstaple_amd as ost
     from openstaple_amd import lib
 
@@ -99,37 +110,48 @@ def main():
     lib.ostp_generate_vec3_soa_gauss(b, 1, 0)
     ost.check(0, "setup")
 
-    # ---------------- timed region: K Dslash pairs (Doe then Deo)
+    # ---------------- timed region: K Dslash pairs (Doe then Deo), HIP events on the library stream
     prev = lib.ostp_set_sync_at_return(0)
     for _ in range(args.warmup):
         lib.acc_Doe(u, h, b, ph)
         lib.acc_Deo(u, out, h, ph)
     lib.ostp_synchronize()
     t0 = time.perf_counter()
+    lib.ostp_timer_start()
     for _ in range(args.steps):
         lib.acc_Doe(u, h, b, ph)
         lib.acc_Deo(u, out, h, ph)
+    lib.ostp_timer_stop()
     lib.ostp_synchronize()
     t1 = time.perf_counter()
+    us_events = lib.ostp_timer_elapsed_us()
     lib.ostp_set_sync_at_return(prev)
     ost.check(0, "dslash loop")
     ms_per_step = 1000.0 * (t1 - t0) / args.steps
     alg_bytes_step = 2 * BYTES_PER_SITE_F64 * sizeh
     value = alg_bytes_step / (ms_per_step * 1e-3) / 1e9
-
-    # per-launch device time of the dominant kernel, HIP events on the library stream
-    reps = max(50, args.steps)
-    us_deo = lib.ostp_time_dslash(u, out, h, ph, 0, 0, reps)
-    us_doe = lib.ostp_time_dslash(u, h, b, ph, 1, 0, reps)
-    us_launch = 0.5 * (us_deo + us_doe)
+    # the dominant kernel's mean launch duration over the timed region (events bracket the 2K launches)
+    us_launch = us_events / (2 * args.steps)
     achieved = BYTES_PER_SITE_F64 * sizeh / (us_launch * 1e-6) / 1e9
-    # plain copy of the gauge field's size: what the box's HBM delivers
-    cbytes = 512 * 1024 * 1024
+    # the same kernel launched alone, back to back (Deo, then Doe)
+    reps = max(50, args.steps)
+    us_single = 0.5 * (lib.ostp_time_dslash(u, out, h, ph, 0, 0, reps) + lib.ostp_time_dslash(u, h, b, ph, 1, 0, reps))
+
+    # HBM ceilings of the box: 1 GiB streaming read / copy / write, best of a small sweep of launch shapes
+    cbytes = 1 << 30
     ca, cb = dmalloc(cbytes), dmalloc(cbytes)
-    us_copy = lib.ostp_time_copy(cb, ca, cbytes, 20)
-    copy_gbs = 2 * cbytes / (us_copy * 1e-6) / 1e9
+    ceilings = {}
+    for kind, name in ((1, "read"), (0, "copy"), (2, "write")):
+        best = (0.0, "")
+        for unroll in (2, 4, 8):
+            for grid in (4096, 8192, 16384):
+                us = lib.ostp_time_stream(kind, cb, ca, cbytes, 10, grid, unroll)
+                gbs = (2 if kind == 0 else 1) * cbytes / (us * 1e-6) / 1e9
+                best = max(best, (gbs, f"{unroll} loads/thread, {grid} blocks"))
+        ceilings[name] = {"GBs": round(best[0], 1), "shape": best[1]}
     lib.ostp_free(ca)
     lib.ostp_free(cb)
+    ost.check(0, "stream probes")
 
     # FP32 Dslash
     uf = dmalloc(8 * L.su3f_bytes)
@@ -143,6 +165,8 @@ def main():
     f32_gbs = BYTES_PER_SITE_F32 * sizeh / (us_f32 * 1e-6) / 1e9
     ost.check(0, "timing")
 
+    # HBM bytes per launch: not measured by this script (the counters need a
+    # profiler run of their own); read from the committed profile summary
     traffic = None
     prof = ROOT / "profiles" / f"dslash_fp64_{'x'.join(map(str, shape))}_traffic.json"
     if prof.exists():
@@ -179,10 +203,13 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "k_dslash<double> (Deo/Doe mean)",
+            "traffic_source": (f"{prof.relative_to(ROOT)} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy "
+                               "kernel's counters; not measured in this run)") if traffic else None,
+            "kernel": "k_dslash<double> (Doe/Deo, mean over the timed region)",
             "us_per_launch": round(us_launch, 3),
-            "copy_kernel_GBs": round(copy_gbs, 1),
-            "frac_of_copy": round(achieved / copy_gbs, 4),
+            "us_per_launch_alone": round(us_single, 3),
+            "ceilings_GBs": ceilings,
+            "frac_of_read_ceiling": round(achieved / ceilings["read"]["GBs"], 4),
             "fp32_dslash_GBs": round(f32_gbs, 1),
             "fp32_us_per_launch": round(us_f32, 3),
         },
@@ -198,8 +225,8 @@ def main():
     if not args.no_rhmc:
         result["rhmc"] = run_rhmc(ost, lib, args)
 
-    if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(shape, args.cpu_seconds)
+    if cpu_libs is not None:
+        result["cpu_baseline"] = cpu_baseline(cpu_libs, shape, bench_shifts(ost), args.cpu_seconds)
 
     print(json.dumps(result))
 
@@ -311,6 +338,24 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
                                   "max_shift_err_vs_fp64": max(rel_err(out + k * vb, out64 + k * vb)
                                                                for k in range(order))}
     res["multishift_speedup_mixed_ru_vs_fp64"] = round(res["multishift_fp64"]["seconds"] / (t1 - t0), 3)
+    # the same two at the MD force's tolerance, 1e-8
+    lib.ostp_synchronize()
+    t0 = time.perf_counter()
+    lib.ostp_multishift_solve(u, C.byref(fp), C.byref(ra), out, b, 1e-8, *d64, sf, 20000, 0, C.byref(cg), sres)
+    lib.ostp_synchronize()
+    t1 = time.perf_counter()
+    f64_8 = t1 - t0
+    lib.ostp_synchronize()
+    t0 = time.perf_counter()
+    conv8 = lib.ostp_multishift_mixed_solve(ip, C.byref(fp), C.byref(ra), out, b, 1e-8, 20000, 1, C.byref(cgm),
+                                            cgr, trs_ms)
+    lib.ostp_synchronize()
+    t1 = time.perf_counter()
+    res["multishift_at_1e-8"] = {"fp64_seconds": round(f64_8, 4), "fp64_iterations": cg.value,
+                                 "mixed_ru_seconds": round(t1 - t0, 4), "mixed_ru_cgm_iterations": cgm.value,
+                                 "mixed_ru_refine_iterations": [cgr[k] for k in range(order)],
+                                 "mixed_ru_converged_shifts": conv8,
+                                 "speedup_mixed_ru_vs_fp64": round(f64_8 / (t1 - t0), 3)}
 
     # (c) plain FP64 CG and FP32-inner CG at m = 0.005 on a 2-step stout-smeared
     # eps = 0.7 field (seed 2), the same source
@@ -608,13 +653,52 @@ def run_rhmc(ost, lib, args):
     flops = GAUGE_FORCE_FLOPS_PER_LINK * 8 * sizeh
     tflops = flops / (us_force * 1e-6) / 1e12
 
+    # thermalisation: gauge-only HMC trajectories (the same update with no
+    # fermion flavour), then one full 2+1-flavour trajectory, untimed
+    def plaquette():
+        return lib.calc_plaquette_soloopenacc(conf, None, None) / (6.0 * 2 * sizeh * 3.0)
+
     lib.ostp_set_rng_seed(2026, 0)
+    plaq = [plaquette()]
+    ai.NDiffFlavs = 0
     t0 = time.perf_counter()
+    acc_g = 0
+    for it in range(args.rhmc_thermalise):
+        # molecular dynamics with fresh momenta and no accept/reject: from the
+        # far-from-equilibrium start every Metropolis test would reject
+        acc_g = lib.UPDATE_SOLOACC_UNOSTEP_VERSATILE(conf, 1e-10, 1e-8, it, acc_g, 0, 50000)
+    ost.check(0, "gauge-only thermalisation")
+    t_therm = time.perf_counter() - t0
+    plaq.append(plaquette())
+    ai.NDiffFlavs = 2
+    warm = ost.RhmcInfo()
     acc = lib.UPDATE_SOLOACC_UNOSTEP_VERSATILE(conf, 1e-10, 1e-8, 0, 0, 1, 50000)
-    t1 = time.perf_counter()
-    ost.check(0, "UPDATE_SOLOACC_UNOSTEP_VERSATILE")
-    info = ost.RhmcInfo()
-    lib.ostp_rhmc_last_info(C.byref(info))
+    ost.check(0, "UPDATE_SOLOACC_UNOSTEP_VERSATILE (warm-up)")
+    lib.ostp_rhmc_last_info(C.byref(warm))
+    plaq.append(plaquette())
+    # the timed trajectory twice from the same links, momenta and pseudofermions:
+    # FP64 rational solves, then the reduced-precision rational solve (FP32
+    # CG-M with FP64 reliable updates, batched FP32-inner refinement) in the
+    # MD force (heat bath and acceptance stay FP64)
+    start = dm(gb)
+    ost.check(lib.ostp_memcpy_dtod(start, conf, gb), "save links")
+    runs = {}
+    for mode, name in ((0, "fp64"), (1, "mixed")):
+        ost.check(lib.ostp_memcpy_dtod(conf, start, gb), "restore links")
+        prev_mode = lib.ostp_set_multishift_precision(mode, 1)
+        lib.ostp_set_rng_seed(4242, 0)
+        t0 = time.perf_counter()
+        acc = lib.UPDATE_SOLOACC_UNOSTEP_VERSATILE(conf, 1e-10, 1e-8, 1, 0, 1, 50000)
+        t1 = time.perf_counter()
+        lib.ostp_set_multishift_precision(prev_mode, 0)
+        ost.check(0, "UPDATE_SOLOACC_UNOSTEP_VERSATILE")
+        inf = ost.RhmcInfo()
+        lib.ostp_rhmc_last_info(C.byref(inf))
+        runs[name] = (t1 - t0, acc, inf)
+    t0, t1 = 0.0, runs["fp64"][0]
+    acc, info = runs["fp64"][1], runs["fp64"][2]
+    plaq.append(plaquette())
+    lib.ostp_free(start)
     for k in bufs:
         C.c_void_p.in_dll(lib, k).value = None
     C.c_void_p.in_dll(lib, "fermions_parameters").value = None
@@ -627,7 +711,21 @@ def run_rhmc(ost, lib, args):
         "s_per_trajectory": round(t1 - t0, 3),
         "accepted": int(acc),
         "delta_S": info.delta_S,
+        "start": (f"generate_Conf_cold eps=0.5, then {args.rhmc_thermalise} gauge-only MD trajectories "
+                  f"(fresh momenta, no Metropolis test; {t_therm:.1f} s) and one 2+1-flavour RHMC trajectory "
+                  f"(dS {warm.delta_S:.4g}, accepted {warm.accepted}) before the timed one"),
+        "plaquette": {"cold": round(plaq[0], 6), "after_gauge_hmc": round(plaq[1], 6),
+                      "after_warmup": round(plaq[2], 6), "after_timed": round(plaq[3], 6)},
         "md_multishift_iterations": info.md_multishift_iterations,
+        "mixed_precision_solves": {
+            "s_per_trajectory": round(runs["mixed"][0], 3),
+            "speedup_vs_fp64": round(runs["fp64"][0] / runs["mixed"][0], 3),
+            "accepted": int(runs["mixed"][1]),
+            "delta_S": runs["mixed"][2].delta_S,
+            "md_multishift_iterations": runs["mixed"][2].md_multishift_iterations,
+            "solver": "MD force (1e-8): FP32 CG-M with FP64 reliable updates + batched FP32-inner refinement "
+                      "of the shifts above the tolerance; heat bath and acceptance (1e-10) FP64 CG-M",
+        },
         "gauge_force": {
             "kernel": "k_gauge_force (tree-level Symanzik staples + TA(U S))",
             "us_per_launch": round(us_force, 1),
@@ -640,28 +738,29 @@ def run_rhmc(ost, lib, args):
     }
 
 
-def cpu_baseline(shape, seconds):
-    """The CPU oracle's Dslash (a scalar restatement of the reference loop, one
-    core) on the same lattice: time Doe+Deo pairs for about `seconds`."""
-    sys.path.insert(0, str(ROOT / "tests"))
-    from oracle import Oracle
+def bench_shifts(ost):
+    """The 19 shifts of the bench's rational approximation (x^(-1/4) of order
+    19 rescaled to [m^2, 64], m = 0.01)."""
+    ra = ost.read_remez(str(ROOT / "tests" / "golden" / "approx_-1_over_4_order_19_mloglm_6.4.REMEZ"))
+    return [ra.RA_b[k] * 64.0 for k in range(ra.approx_order)]
 
-    o = Oracle(*shape)
-    u = o.gen_cold(0.5, 1)
-    ph = o.phases()
-    psi = o.gen_gauss(2)
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        h = o.doe(u, psi, ph)
-        o.deo(u, h, ph)
-        n += 1
-        if time.perf_counter() - t0 > seconds:
-            break
-    dt = time.perf_counter() - t0
-    gbs = 2 * n * BYTES_PER_SITE_F64 * o.sizeh / dt / 1e9
-    return {"value": round(gbs, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"{n} Doe+Deo pairs at {'x'.join(map(str, shape))} in {dt:.1f} s (gcc -O2 serial oracle)"}
+
+def cpu_baseline(libs, shape, shifts, seconds):
+    """The CPU oracle (a scalar restatement of the reference loops) timed on
+    this host: gcc -O3 -march=native, serial and OpenMP; `value` is the
+    OpenMP Dslash bandwidth at the bench lattice (928 B / half-site model)."""
+    import cpu_baseline as cpub
+
+    d = cpub.run(libs, shape, shifts, dslash_seconds=seconds / 2)
+    key = "dslash_%s" % "x".join(map(str, shape))
+    if "error" in d:
+        return {"value": None, "unit": "GB/s", "cores": 0, "kind": "port", "sample": "build failed", **d}
+    omp, ser = d["omp"], d["serial"]
+    return {"value": omp[key]["GBs"], "unit": "GB/s", "cores": omp["cores"], "kind": "port",
+            "sample": (f"{omp[key]['pairs']} Doe+Deo pairs at {'x'.join(map(str, shape))} in {omp[key]['seconds']} s "
+                       f"(OpenMP, {omp['cores']} threads); serial: {ser[key]['pairs']} pairs, {ser[key]['GBs']} GB/s; "
+                       "plus the 8^4 Dslash, the 8^4 CG (m=0.05, 1e-10) and 19-shift CG-M iterations per build"),
+            **d}
 
 
 if __name__ == "__main__":

@@ -107,6 +107,7 @@ void *ostp_malloc(size_t bytes);
 void  ostp_free(void *dev);
 int   ostp_memcpy_htod(void *dev, const void *host, size_t bytes);
 int   ostp_memcpy_dtoh(void *host, const void *dev, size_t bytes);
+int   ostp_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int   ostp_memset(void *dev, int value, size_t bytes);
 int   ostp_synchronize(void);
 /* Entry points return synchronised by default (the reference's non-async
@@ -373,6 +374,19 @@ int ostp_batched_half_solve(int nsys, const ostp_batch_system *systems, double r
 int ostp_multishift_mixed_solve(inverter_package ip, ferm_param *pars, RationalApprox *approx, vec3_soa *out,
                                 const vec3_soa *in, double res, int max_cg, int half_inner, int *cg_multishift,
                                 int *cg_refine, double *true_res);
+/* The solve inverter_multishift_wrapper performs (heat bath, MD force and
+ * acceptance of UPDATE_SOLOACC_UNOSTEP_VERSATILE, fermion_force_soloopenacc):
+ * mode 0 (default) as the reference selects it with inverter_tricks; 1 the
+ * reduced-precision rational solve of ostp_multishift_mixed_solve with FP32
+ * refinement inner solves; 2 the same with FP16 inner solves.  With
+ * force_only, modes 1 and 2 apply to the solves inside
+ * fermion_force_soloopenacc (the MD force) and every other call keeps mode 0.
+ * Modes 1 and 2 use the package's FP32 links and vectors and the flavour's
+ * FP32 phases when they are set, library buffers converted from the FP64
+ * ones when not.  The returned iteration count adds the refinement
+ * iterations of every shift.  Returns the previous mode (-1 and a recorded
+ * error for a bad one). */
+int ostp_set_multishift_precision(int mode, int force_only);
 /* Text of the autotuner's measurements so far (candidates and choice per
  * batch signature); returns its length, copies up to len-1 bytes into buf. */
 int ostp_autotune_report(char *buf, int len);
@@ -643,6 +657,11 @@ double ostp_time_dslash(const void *u, void *out, const void *in, const void *ph
  * The HBM ceiling the Dslash bandwidth is compared with. */
 double ostp_time_stream(int kind, void *dst, const void *src, size_t bytes, int reps, int grid,
                         int unroll);
+/* HIP events on the library's stream: start, then (after queueing work)
+ * stop, then the elapsed device time in microseconds (waits for stop). */
+int ostp_timer_start(void);
+int ostp_timer_stop(void);
+double ostp_timer_elapsed_us(void);
 /* Alternate the walk direction of consecutive Dslash launches (on by
  * default): a launch then starts on the links the previous launch streamed
  * last, which the memory-side cache still holds.  Results do not depend on

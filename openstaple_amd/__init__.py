@@ -237,6 +237,7 @@ _SIGS = {
     "ostp_free": (None, [_P]),
     "ostp_memcpy_htod": (_I, [_P, _P, C.c_size_t]),
     "ostp_memcpy_dtoh": (_I, [_P, _P, C.c_size_t]),
+    "ostp_memcpy_dtod": (_I, [_P, _P, C.c_size_t]),
     "ostp_memset": (_I, [_P, _I, C.c_size_t]),
     "ostp_synchronize": (_I, []),
     "ostp_set_abort_on_error": (None, [_I]),
@@ -368,6 +369,10 @@ _SIGS = {
     "ostp_time_dslash": (_D, [_P, _P, _P, _P, _I, _I, _I]),
     "ostp_time_stream": (_D, [_I, _P, _P, C.c_size_t, _I, _I, _I]),
     "ostp_set_dslash_serpentine": (_I, [_I]),
+    "ostp_set_multishift_precision": (_I, [_I, _I]),
+    "ostp_timer_start": (_I, []),
+    "ostp_timer_stop": (_I, []),
+    "ostp_timer_elapsed_us": (_D, []),
 }
 
 for _name, (_res, _args) in _SIGS.items():

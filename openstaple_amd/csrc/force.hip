@@ -270,6 +270,7 @@ void fermion_force_soloopenacc(su3_soa* tconf_acc, su3_soa* tstout_conf_acc_arr,
                                vec3_soa* tferm_shiftmulti_acc, inverter_package ipt, const int max_cg) {
     const char* nm = "fermion_force_soloopenacc";
     if (!require_init(nm)) return;
+    ForceScope in_force;  // inverter_multishift_wrapper's force-only precision mode applies
     const Lattice& L = ctx().lat;
     const size_t gb = 8 * L.su3_bytes(), tb = (size_t)8 * 64 * L.sizeh;
     const int steps = ostp_stout_steps();

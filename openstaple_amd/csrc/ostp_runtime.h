@@ -86,6 +86,14 @@ bool gauge_action_tlsm();
 // End of a void entry point: wait for the stream unless asynchronous mode is on.
 bool entry_done(const char* what);
 
+// Set while fermion_force_soloopenacc runs (force.hip): the rational solves
+// inside it are the MD force's (for a precision mode that applies there only).
+bool in_fermion_force();
+struct ForceScope {
+    ForceScope();
+    ~ForceScope();
+};
+
 // Read the device solver state into the pinned mirror (synchronises the stream).
 bool pull_state();
 bool push_state();

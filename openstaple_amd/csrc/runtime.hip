@@ -140,6 +140,17 @@ static void release_scratch() {
 
 using namespace ostp;
 
+namespace ostp {
+static int g_force_depth = 0;
+bool in_fermion_force() { return g_force_depth > 0; }
+ForceScope::ForceScope() { ++g_force_depth; }
+ForceScope::~ForceScope() { --g_force_depth; }
+}  // namespace ostp
+
+// HIP events on the library stream around a span of asynchronous calls
+// (ostp_timer_start / _stop / _elapsed_us)
+static hipEvent_t g_timer[2] = {nullptr, nullptr};
+
 extern "C" {
 
 int ostp_init(int nd0, int nd1, int nd2, int nd3, int par_temp) {
@@ -177,6 +188,10 @@ void ostp_finalize(void) {
     for (auto& kv : g_map) (void)hipFree(kv.second.dev);
     g_map.clear();
     release_scratch();
+    for (auto& e : g_timer) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
     if (g_ctx.stream) (void)hipStreamDestroy(g_ctx.stream);
     g_ctx.stream = nullptr;
     g_ctx.initialized = false;
@@ -307,6 +322,10 @@ int ostp_memcpy_dtoh(void* host, const void* dev, size_t bytes) {
     return stream_copy(host, dev, bytes, hipMemcpyDeviceToHost, "ostp_memcpy_dtoh");
 }
 
+int ostp_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    return stream_copy(dst, src, bytes, hipMemcpyDeviceToDevice, "ostp_memcpy_dtod");
+}
+
 int ostp_memset(void* dev, int value, size_t bytes) {
     hipStream_t s = lib_stream();
     if (!hip_ok(hipMemsetAsync(dev, value, bytes, s), "ostp_memset")) return 1;
@@ -325,6 +344,25 @@ int ostp_set_sync_at_return(int on) {
     g_ctx.sync_at_return = on ? 1 : 0;
     return prev;
 }
+int ostp_timer_start(void) {
+    if (!require_init("ostp_timer_start")) return 1;
+    for (auto& e : g_timer)
+        if (!e && !hip_ok(hipEventCreate(&e), "hipEventCreate")) return 1;
+    return hip_ok(hipEventRecord(g_timer[0], g_ctx.stream), "hipEventRecord") ? 0 : 1;
+}
+int ostp_timer_stop(void) {
+    if (!g_timer[1]) return fail("ostp_timer_stop without ostp_timer_start") ? 1 : 1;
+    return hip_ok(hipEventRecord(g_timer[1], g_ctx.stream), "hipEventRecord") ? 0 : 1;
+}
+double ostp_timer_elapsed_us(void) {
+    float ms = 0.f;
+    if (!g_timer[0] || !g_timer[1]) return -1.0;
+    if (!hip_ok(hipEventSynchronize(g_timer[1]), "hipEventSynchronize") ||
+        !hip_ok(hipEventElapsedTime(&ms, g_timer[0], g_timer[1]), "hipEventElapsedTime"))
+        return -1.0;
+    return 1000.0 * (double)ms;
+}
+
 int ostp_set_dslash_serpentine(int on) {
     const int prev = g_ctx.dslash_serpentine;
     g_ctx.dslash_serpentine = on ? 1 : 0;

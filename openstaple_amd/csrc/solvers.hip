@@ -173,19 +173,25 @@ __global__ __launch_bounds__(kBlock) void k_msw_r(const C* __restrict__ r_old, C
 
 // p = gammag p + r; block 0 also advances the window coefficients of every
 // active shift by this iteration's omega_k, gamma_k, zeta_k
+// With `hist`, r is also stored as the window's residual t and the position
+// is left for k_msw_tick to advance (every block reads t here: the mixed
+// CG-M, whose reliable updates restart windows at positions the host does not
+// see, keeps r in place and the window's copies in hist).
 template <typename C>
 __global__ __launch_bounds__(kBlock) void k_msw_p(C* __restrict__ p, const C* __restrict__ r, long n3, MsWin* w,
-                                                  SolverCtl ctl) {
+                                                  SolverCtl ctl, C* __restrict__ hist) {
     if (!pred_pass(ctl)) return;
     const SolverState* st = ctl.st;
     const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    const int t = w->t;
     if (i < n3) {
         const double g = st->gammag;
         const C rr = r[i], pp = p[i];
         p[i] = mkc<C>((double)pp.x * g + (double)rr.x, (double)pp.y * g + (double)rr.y);
+        if (hist) hist[(long)t * n3 + i] = rr;
     }
     if (blockIdx.x != 0) return;
-    const int t = w->t, cur = st->flag_cur, order = st->order;
+    const int cur = st->flag_cur, order = st->order;
     const int per = t + 2;  // j < t, the new column t, and the scalar pair
     for (int e = threadIdx.x; e < order * per; e += kBlock) {
         const int k = e / per, j = e - k * per;
@@ -203,7 +209,13 @@ __global__ __launch_bounds__(kBlock) void k_msw_p(C* __restrict__ p, const C* __
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) w->t = t + 1;
+    if (threadIdx.x == 0 && !hist) w->t = t + 1;
+}
+
+// the window position after a k_msw_p with hist
+__global__ void k_msw_tick(MsWin* w, SolverCtl ctl) {
+    if (!pred_pass(ctl)) return;
+    if (threadIdx.x == 0) w->t += 1;
 }
 
 // x_k += a_k p_k + sum_j c_kj r_j ; p_k = B_k p_k + sum_j D_kj r_j for the
@@ -213,10 +225,10 @@ constexpr int kMswGroup = 8;
 template <typename C>
 __global__ __launch_bounds__(kBlock) void k_msw_flush(C* __restrict__ xs, C* __restrict__ ps,
                                                       const C* __restrict__ hist, long n3, int order,
-                                                      const MsWin* __restrict__ w, SolverCtl ctl) {
+                                                      const MsWin* __restrict__ w, SolverCtl ctl, int only_at) {
     if (!pred_pass(ctl)) return;
     const int t = w->t;
-    if (t == 0) return;
+    if (t == 0 || (only_at > 0 && t != only_at)) return;  // only_at: flush a full window only
     const long i = (long)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n3) return;
     for (int k0 = 0; k0 < order; k0 += kMswGroup) {
@@ -260,9 +272,12 @@ __global__ __launch_bounds__(kBlock) void k_msw_flush(C* __restrict__ xs, C* __r
     }
 }
 
-// a fresh window: coefficients of the identity map
-__global__ __launch_bounds__(kBlock) void k_msw_reset(MsWin* w, SolverCtl ctl) {
+// a fresh window: coefficients of the identity map (with only_at > 0: only
+// when the window holds only_at residuals)
+__global__ __launch_bounds__(kBlock) void k_msw_reset(MsWin* w, SolverCtl ctl, int only_at) {
     if (!pred_pass(ctl)) return;
+    if (only_at > 0 && w->t != only_at) return;
+    __syncthreads();  // every thread has read t before thread 0 clears it
     for (int e = threadIdx.x; e < kMaxShifts * kMsWinMax; e += kBlock) {
         const int k = e / kMsWinMax, j = e - k * kMsWinMax;
         w->c[k][j] = 0.0;
@@ -745,7 +760,7 @@ int multishift_solve(const MsBufs& B, double mass, int order, const double* shif
         }
         hist = (char*)g_msw_hist;
         if (!g_msw && !hip_ok(hipMalloc(&g_msw, sizeof(MsWin)), "multishift window")) return INVERTER_FAILURE;
-        hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kAlways));
+        hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kAlways), 0);
     }
     auto slot = [&](int j) { return (C*)(hist + (size_t)((j - 1) % W) * vb); };  // residual of iteration j (1-based)
     auto iteration = [&](int j) -> bool {
@@ -764,12 +779,12 @@ int multishift_solve(const MsBufs& B, double mass, int order, const double* shif
                            ctl_of(kIfRunning));
         if (!launch_finalize(kFinMsLambda, ctl_of(kIfRunning), nbh)) return false;
         hipLaunchKernelGGL((k_msw_p<C>), dim3(nb3), dim3(kBlock), 0, S(), (C*)B.p, (const C*)r_new, n3, g_msw,
-                           ctl_of(kIfIter, j));
+                           ctl_of(kIfIter, j), (C*)nullptr);
         if (j % W == 0) {
             // only when iteration j ran: a window cut short is flushed once, at the end
             hipLaunchKernelGGL((k_msw_flush<C>), dim3(nb3), dim3(kBlock), 0, S(), (C*)B.out, (C*)B.shiftferm,
-                               (const C*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kIfIter, j));
-            hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kIfIter, j));
+                               (const C*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kIfIter, j), 0);
+            hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kIfIter, j), 0);
         }
         return check_launch("multishift iteration");
     };
@@ -779,7 +794,7 @@ int multishift_solve(const MsBufs& B, double mass, int order, const double* shif
         // the partial last window, and the final residual where the per-iteration
         // version leaves it
         hipLaunchKernelGGL((k_msw_flush<C>), dim3(nb3), dim3(kBlock), 0, S(), (C*)B.out, (C*)B.shiftferm,
-                           (const C*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kAlways));
+                           (const C*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kAlways), 0);
         if (cg > 0 && !hip_ok(hipMemcpyAsync(B.r, slot(cg), vb, hipMemcpyDeviceToDevice, S()), "final residual"))
             return INVERTER_FAILURE;
         if (!check_launch("multishift window flush")) return INVERTER_FAILURE;
@@ -842,7 +857,14 @@ static void* g_aux1_f = nullptr;
 static void* g_shiftmulti_f = nullptr;
 static size_t g_aux1_bytes = 0, g_shiftmulti_bytes = 0;
 
-static void* scratch(void*& p, size_t& have, size_t need) {
+// inverter_multishift_wrapper's reduced-precision mode (ostp_set_multishift_precision)
+// and the FP32 copies it makes when the caller's package carries none
+static int g_ms_precision = 0;
+static int g_ms_force_only = 0;  // the mode applies inside fermion_force_soloopenacc only
+static void *g_wr_conf_f = nullptr, *g_wr_ph_f = nullptr, *g_wr_vec_f = nullptr;
+static size_t g_wr_conf_f_bytes = 0, g_wr_ph_f_bytes = 0, g_wr_vec_f_bytes = 0;
+
+static void* grow_buffer(void*& p, size_t& have, size_t need) {
     if (have >= need) return p;
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -1061,14 +1083,56 @@ int inverter_multishift_wrapper(inverter_package ip, ferm_param* pars, RationalA
     int total_iterations = 0;
     int cg_return = 0;
     int st;
+    if (g_ms_precision != 0 && (!g_ms_force_only || in_fermion_force())) {
+        // the library's reduced-precision rational solve (ostp_set_multishift_precision),
+        // with FP32 links, phases and vectors from library buffers where the package has none
+        const Lattice& L = ctx().lat;
+        const int order = approx->approx_order;
+        const size_t vbf = L.vec3f_bytes();
+        inverter_package q = ip;
+        ferm_param pf = *pars;
+        if (!q.u_f) {
+            const void* du = resolve(ip.u, 8 * L.su3_bytes(), nm);
+            void* cf = grow_buffer(g_wr_conf_f, g_wr_conf_f_bytes, 8 * L.su3f_bytes());
+            if (!du || !cf) return 0;
+            ostp_convert_double_to_float_gauge((const su3_soa*)du, (su3_soa_f*)cf);
+            q.u_f = (const su3_soa_f*)cf;
+        }
+        if (!pf.phases_f) {
+            void* phf = grow_buffer(g_wr_ph_f, g_wr_ph_f_bytes, 8 * L.fsoa_bytes());
+            if (!phf) return 0;
+            ostp_phases_to_float(pars->phases, (float_soa*)phf);
+            pf.phases_f = (float_soa*)phf;
+        }
+        if (!q.ferm_shift_temp_f || !q.loc_r_f || !q.loc_h_f || !q.loc_s_f || !q.loc_p_f || !q.out_f) {
+            char* v = (char*)grow_buffer(g_wr_vec_f, g_wr_vec_f_bytes, (size_t)(order + 5) * vbf);
+            if (!v) return 0;
+            q.loc_r_f = (vec3_soa_f*)v;
+            q.loc_h_f = (vec3_soa_f*)(v + vbf);
+            q.loc_s_f = (vec3_soa_f*)(v + 2 * vbf);
+            q.loc_p_f = (vec3_soa_f*)(v + 3 * vbf);
+            q.out_f = (vec3_soa_f*)(v + 4 * vbf);
+            q.ferm_shift_temp_f = (vec3_soa_f*)(v + 5 * vbf);
+        }
+        if (ostp_last_error()[0]) return 0;
+        std::vector<int> cgr(order > 0 ? order : 1, 0);
+        std::vector<double> trs(order > 0 ? order : 1, 0.0);
+        const int conv = ostp_multishift_mixed_solve(q, &pf, approx, out, in, res, max_cg, g_ms_precision == 2 ? 1 : 0,
+                                                     &cg_return, cgr.data(), trs.data());
+        convergence_messages(convergence_importance, conv == order ? INVERTER_SUCCESS : INVERTER_FAILURE);
+        total_iterations = cg_return;
+        for (int k = 0; k < order; k++) total_iterations += cgr[k];
+        multishift_invert_iterations += cg_return;
+        return total_iterations;
+    }
     if (inverter_tricks.singlePInvAccelMultiInv) {
         // SP multishift to a loose target, then per-shift DP (or mixed) CG refinement
         // (inverter_wrappers.c:58-98)
         const Lattice& L = ctx().lat;
         const int order = approx->approx_order;
         const size_t nv = order > 0 ? (size_t)order : 1;
-        void* aux1 = scratch(g_aux1_f, g_aux1_bytes, L.vec3f_bytes());
-        void* xsf = scratch(g_shiftmulti_f, g_shiftmulti_bytes, nv * L.vec3f_bytes());
+        void* aux1 = grow_buffer(g_aux1_f, g_aux1_bytes, L.vec3f_bytes());
+        void* xsf = grow_buffer(g_shiftmulti_f, g_shiftmulti_bytes, nv * L.vec3f_bytes());
         const void* db = resolve(in, L.vec3_bytes(), nm);
         if (!aux1 || !xsf || !db) return 0;
         if (!launch_d2f(db, aux1, 3 * L.sizeh)) return 0;
@@ -1111,6 +1175,17 @@ int inverter_multishift_wrapper(inverter_package ip, ferm_param* pars, RationalA
     return total_iterations;
 }
 
+int ostp_set_multishift_precision(int mode, int force_only) {
+    if (mode < 0 || mode > 2) {
+        fail("ostp_set_multishift_precision: mode %d outside 0..2", mode);
+        return -1;
+    }
+    const int prev = g_ms_precision;
+    g_ms_precision = mode;
+    g_ms_force_only = force_only ? 1 : 0;
+    return prev;
+}
+
 // Reduced-precision rational solve.  FP32 CG-M: FP32 Dslash and vectors, the
 // reference's FP64 shift recurrences (inverter_multishift_full.c), FP32
 // accumulators for the shifted solutions and for the base system's solution
@@ -1136,8 +1211,8 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
     const long sizeh = L.sizeh, n3 = 3 * sizeh;
     const int nbh = blocks_for(sizeh), nb3 = blocks_for(n3);
     const size_t vb = L.vec3_bytes(), vbf = L.vec3f_bytes();
-    void* bf = scratch(g_aux1_f, g_aux1_bytes, vbf);
-    void* xsf = scratch(g_shiftmulti_f, g_shiftmulti_bytes, (size_t)order * vbf);
+    void* bf = grow_buffer(g_aux1_f, g_aux1_bytes, vbf);
+    void* xsf = grow_buffer(g_shiftmulti_f, g_shiftmulti_bytes, (size_t)order * vbf);
     const void* db = resolve(in, vb, nm);
     void* dout = resolve(out, (size_t)order * vb, nm);
     const void* u = resolve(ip.u, 8 * L.su3_bytes(), nm);
@@ -1198,23 +1273,57 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
         fl.x64[k + 1] = (double2*)((char*)dout + k * vb);
         fl.x32[k + 1] = (float2*)((char*)xsf + k * vbf);
     }
+    // shift updates once per window of W iterations (as multishift_solve),
+    // the window flushed early by every reliable update
+    const int W = msw_window(vbf);
+    float2* hist = nullptr;
+    if (W >= 2) {
+        if (g_msw_hist_bytes < (size_t)W * vbf) {
+            if (g_msw_hist) (void)hipFree(g_msw_hist);
+            g_msw_hist = nullptr;
+            g_msw_hist_bytes = 0;
+            if (!hip_ok(hipMalloc(&g_msw_hist, (size_t)W * vbf), "multishift residual window")) return 0;
+            g_msw_hist_bytes = (size_t)W * vbf;
+        }
+        hist = (float2*)g_msw_hist;
+        if (!g_msw && !hip_ok(hipMalloc(&g_msw, sizeof(MsWin)), "multishift window")) return 0;
+        hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kAlways), 0);
+    }
     auto iteration = [&](int j) -> bool {
         if (!launch_mdagm<float>(uf, phf, p, h, sv, m2, 2, kFinMxmAlpha, ctl_of(kIfRunning))) return false;
         hipLaunchKernelGGL(k_msx_r, dim3(nbh), dim3(kBlock), 0, S(), (float2*)r, (const float2*)sv,
                            (const float2*)p, (float2*)x0f, (int)sizeh, ctl_of(kIfRunning));
         // reliable update: accumulators into the FP64 solutions, r = b - M x0 in FP64
+        if (hist) {
+            // the shifts' pending window first, so that their accumulators are current
+            hipLaunchKernelGGL((k_msw_flush<float2>), dim3(nb3), dim3(kBlock), 0, S(), (float2*)xsf, (float2*)pk,
+                               (const float2*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kIfRunningRU), 0);
+            hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kIfRunningRU), 0);
+        }
         hipLaunchKernelGGL(k_msx_flush, dim3(nb3), dim3(kBlock), 0, S(), fl, n3, ctl_of(kIfRunningRU));
         if (!launch_mdagm<double>(u, ph, x0, h64, s64, m2, 1, kFinStore, ctl_of(kIfRunningRU))) return false;
         hipLaunchKernelGGL(k_msx_resid, dim3(nbh), dim3(kBlock), 0, S(), (const double2*)db, (const double2*)s64,
                            (float2*)r, (int)sizeh, ctl_of(kIfRunningRU));
         if (!launch_finalize(kFinMxmLambda, ctl_of(kIfRunning), nbh)) return false;
-        hipLaunchKernelGGL((k_ms_update<float2>), dim3(nb3), dim3(kBlock), 0, S(), (const float2*)r, (float2*)p,
-                           (float2*)xsf, (float2*)pk, n3, n3, ctl_of(kIfIter, j));
+        if (!hist) {
+            hipLaunchKernelGGL((k_ms_update<float2>), dim3(nb3), dim3(kBlock), 0, S(), (const float2*)r, (float2*)p,
+                               (float2*)xsf, (float2*)pk, n3, n3, ctl_of(kIfIter, j));
+            return check_launch("mixed multishift iteration");
+        }
+        hipLaunchKernelGGL((k_msw_p<float2>), dim3(nb3), dim3(kBlock), 0, S(), (float2*)p, (const float2*)r, n3,
+                           g_msw, ctl_of(kIfIter, j), hist);
+        hipLaunchKernelGGL(k_msw_tick, dim3(1), dim3(64), 0, S(), g_msw, ctl_of(kIfIter, j));
+        hipLaunchKernelGGL((k_msw_flush<float2>), dim3(nb3), dim3(kBlock), 0, S(), (float2*)xsf, (float2*)pk,
+                           (const float2*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kIfIter, j), W);
+        hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kIfIter, j), W);
         return check_launch("mixed multishift iteration");
     };
     if (!run_batches(iteration)) return 0;
     const int cg = H().cg;
     if (cg_multishift) *cg_multishift = cg;
+    if (hist)
+        hipLaunchKernelGGL((k_msw_flush<float2>), dim3(nb3), dim3(kBlock), 0, S(), (float2*)xsf, (float2*)pk,
+                           (const float2*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kAlways), 0);
     hipLaunchKernelGGL(k_msx_flush, dim3(nb3), dim3(kBlock), 0, S(), fl, n3, ctl_of(kAlways));
     if (!check_launch("flush")) return 0;
 

@@ -233,6 +233,8 @@ static void dslash(const or_geom *g, const void *u, void *out, const void *in,
     const int nd0 = g->nd[0], nd1 = g->nd[1], nd2 = g->nd[2], nd3 = g->nd[3];
     const int nd0h = nd0 >> 1;
     or_dc *o0 = vplane(g, out, 0), *o1 = vplane(g, out, 1), *o2 = vplane(g, out, 2);
+    /* OpenMP only in the CPU-baseline build (oracle/Makefile `bench`); ignored otherwise */
+#pragma omp parallel for collapse(2) schedule(static)
     for (int d3 = 0; d3 < nd3; d3++)
         for (int d2 = 0; d2 < nd2; d2++)
             for (int d1 = 0; d1 < nd1; d1++)
@@ -335,6 +337,7 @@ static void dslash_f(const or_geom *g, const void *u, void *out, const void *in,
     const int nd0 = g->nd[0], nd1 = g->nd[1], nd2 = g->nd[2], nd3 = g->nd[3];
     const int nd0h = nd0 >> 1;
     or_fc *o0 = vplane_f(g, out, 0), *o1 = vplane_f(g, out, 1), *o2 = vplane_f(g, out, 2);
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int d3 = 0; d3 < nd3; d3++)
         for (int d2 = 0; d2 < nd2; d2++)
             for (int d1 = 0; d1 < nd1; d1++)
@@ -384,6 +387,7 @@ void or_acc_Doe_f(const or_geom *g, const void *u, void *out, const void *in, co
 double or_l2norm2(const or_geom *g, const void *v)
 {
     double resR = 0.0;
+#pragma omp parallel for reduction(+ : resR) schedule(static)
     for (long t = 0; t < g->sizeh; t++) {
         /* l2norm2_loc, fermionic_utilities.h:32-40 */
         or_dc a = vplane(g, v, 0)[t], b = vplane(g, v, 1)[t], c = vplane(g, v, 2)[t];
@@ -398,6 +402,7 @@ double or_l2norm2(const or_geom *g, const void *v)
 double or_real_scal_prod(const or_geom *g, const void *x, const void *y)
 {
     double resR = 0.0;
+#pragma omp parallel for reduction(+ : resR) schedule(static)
     for (long t = 0; t < g->sizeh; t++) {
         /* scal_prod_loc_1double, fermionic_utilities.h:20-28 */
         double sum = 0;
@@ -459,6 +464,7 @@ static void axpy_to(const or_geom *g, const void *in1, double factor, const void
 {
     for (int c = 0; c < 3; c++) {
         or_dc *a = vplane(g, in1, c), *b = vplane(g, in2, c), *o = vplane(g, out, c);
+#pragma omp parallel for schedule(static)
         for (long i = 0; i < g->sizeh; i++) o[i] = (a[i] * factor) + b[i];
     }
 }
@@ -475,6 +481,7 @@ static void sub_to(const or_geom *g, const void *in1, const void *in2, void *out
 {
     for (int c = 0; c < 3; c++) {
         or_dc *a = vplane(g, in1, c), *b = vplane(g, in2, c), *o = vplane(g, out, c);
+#pragma omp parallel for schedule(static)
         for (long i = 0; i < g->sizeh; i++) o[i] = a[i] - b[i];
     }
 }
@@ -493,6 +500,7 @@ void or_fermion_matrix_multiplication_shifted(const or_geom *g, const void *u, v
     double m2 = mass * mass + shift;
     for (int c = 0; c < 3; c++) {
         or_dc *a = vplane(g, in, c), *o = vplane(g, out, c);
+#pragma omp parallel for schedule(static)
         for (long i = 0; i < g->sizeh; i++) o[i] = (a[i] * m2) - o[i];
     }
 }

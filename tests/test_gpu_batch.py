@@ -162,3 +162,71 @@ def test_batched_half_precision_inner(mu):
         r = b - f.o.mdagm(f.u, x, f.ph, m)
         assert np.linalg.norm(r) / np.linalg.norm(b) <= RES, k
         assert cg[k] <= 1.5 * cg32[k], (k, cg[k], cg32[k])
+
+
+@pytest.mark.parametrize("half", [0, 1])
+def test_config3_full_shape(half):
+    """The full batched workload: 24 Z2 sources on one 32^3x8 field plus 6
+    parallel-tempering replicas on 24^3x8 fields carrying the K plane (defect
+    coefficients 0, 0.2, ..., 1 on the t boundary), m = 0.02, to 1e-9, in one
+    batched solve (HIP graph, 16 iterations per replay; FP32 or FP16 inner).
+    Every system's FP64 true residual is recomputed afterwards with the
+    single-lattice operator fermion_matrix_multiplication (one lattice at a
+    time) and must be below 1e-9."""
+    mass, res = 0.02, 1e-9
+    A, B = (32, 32, 32, 8), (24, 24, 24, 8)
+    La = init_lattice(A)
+    fa = Dev(nbytes=8 * La.su3_bytes)
+    lib.ostp_generate_Conf_cold(fa.p, 0.4, 3)
+    faf = Dev(nbytes=8 * La.su3f_bytes)
+    lib.ostp_convert_double_to_float_gauge(fa.p, faf.p)
+    pa, paf = Dev(nbytes=8 * 8 * La.sizeh), Dev(nbytes=4 * 8 * La.sizeh)
+    lib.calc_u1_phases(pa.p, ost.BfParam(0, 0, 0, 0, 0, 0), 0.0, 0.0)
+    lib.ostp_phases_to_float(pa.p, paf.p)
+    bufs = []
+    for k in range(24):
+        b, x = Dev(nbytes=La.vec3_bytes), Dev(nbytes=La.vec3_bytes)
+        lib.ostp_generate_vec3_soa_z2noise(b.p, 5, k)
+        bufs.append((A, 0, fa, faf, pa, paf, b, x))
+    Lb = init_lattice(B, par_temp=True)
+    pb, pbf = Dev(nbytes=8 * 8 * Lb.sizeh), Dev(nbytes=4 * 8 * Lb.sizeh)
+    lib.calc_u1_phases(pb.p, ost.BfParam(0, 0, 0, 0, 0, 0), 0.0, 0.0)
+    lib.ostp_phases_to_float(pb.p, pbf.p)
+    for r in range(6):
+        fr = Dev(nbytes=8 * Lb.su3_bytes)
+        lib.ostp_generate_Conf_cold(fr.p, 0.4, 20 + r)
+        lib.init_k(fr.p, 0.2 * r, 3, (C.c_int * 3)(24, 24, 24), None, 1)
+        frf = Dev(nbytes=8 * Lb.su3f_bytes)
+        lib.ostp_convert_double_to_float_gauge(fr.p, frf.p)
+        b, x = Dev(nbytes=Lb.vec3_bytes), Dev(nbytes=Lb.vec3_bytes)
+        lib.ostp_generate_vec3_soa_z2noise(b.p, 6, r)
+        bufs.append((B, 1, fr, frf, pb, pbf, b, x))
+    ost.check(0, "setup")
+    n = len(bufs)
+    arr = (ost.BatchSystem * n)()
+    for k, (shape, pt, u, uf, ph, phf, b, x) in enumerate(bufs):
+        s = arr[k]
+        s.nd[:] = shape
+        s.par_temp = pt
+        s.u, s.u_f, s.phases, s.phases_f, s.in_, s.out, s.mass = u.p, uf.p, ph.p, phf.p, b.p, x.p, mass
+    cg, ru, tr = (C.c_int * n)(), (C.c_int * n)(), (C.c_double * n)()
+    solve = lib.ostp_batched_half_solve if half else lib.ostp_batched_mixed_solve
+    conv = solve(n, arr, res, 10000, 0.1, 16, cg, ru, tr)
+    ost.check(0, "batched solve")
+    assert conv == n, list(tr)
+    assert min(cg) > 10
+    # independent FP64 residuals, one lattice at a time
+    for shape, pt in ((A, 0), (B, 1)):
+        L = init_lattice(shape, par_temp=bool(pt))
+        vb = L.vec3_bytes
+        out, tmp, d = Dev(nbytes=vb), Dev(nbytes=vb), Dev(nbytes=vb)
+        for k, (sh, p_, u, uf, ph, phf, b, x) in enumerate(bufs):
+            if sh != shape:
+                continue
+            fp = ost.ferm_param(mass, ph.p)
+            lib.fermion_matrix_multiplication(u.p, out.p, x.p, tmp.p, C.byref(fp))
+            lib.combine_in1_minus_in2(b.p, out.p, d.p)
+            rr = (lib.l2norm2_global(d.p) / lib.l2norm2_global(b.p)) ** 0.5
+            ost.check(0, "residual")
+            assert rr <= res, (k, rr)
+            assert abs(rr - tr[k]) <= 0.05 * res, (k, rr, tr[k])
