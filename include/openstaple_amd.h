@@ -183,6 +183,19 @@ void ostp_convert_double_to_float_gauge(const su3_soa *d_var, su3_soa_f *f_var);
 int ker_invert_openacc(const su3_soa *u, ferm_param *pars, vec3_soa *solution, const vec3_soa *in,
                        double res, vec3_soa *loc_r, vec3_soa *loc_h, vec3_soa *loc_s, vec3_soa *loc_p,
                        const int max_cg, double shift, int *cg_return);
+/* No counterpart in the reference.  ker_invert_openacc for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): the same arguments, kernels and results
+ * (bit-identical), with the reduction partials and the solver state in a
+ * static device array of the library instead of the buffers ostp_init
+ * allocated; the call itself allocates nothing.  Limit: lattices of at most
+ * 2^23 half-sites (64^4); larger ones are rejected with INVERTER_FAILURE and
+ * a recorded error.  The array is one per GPU and keeps the last call's
+ * contents; calls that overlap on one GPU race (the library is not
+ * re-entrant). */
+int ker_invert_openacc_noalloc(const su3_soa *u, ferm_param *pars, vec3_soa *solution, const vec3_soa *in,
+                               double res, vec3_soa *loc_r, vec3_soa *loc_h, vec3_soa *loc_s,
+                               vec3_soa *loc_p, const int max_cg, double shift, int *cg_return);
 /* inverter_mixedp.h:15-22 */
 int inverter_mixed_precision(inverter_package ip, ferm_param *pars, vec3_soa *solution,
                              const vec3_soa *in, double res, const int max_cg, double shift,

@@ -270,6 +270,7 @@ _SIGS = {
     "convert_float_to_double_vec3_soa": (None, [_P, _P]),
     "ostp_convert_double_to_float_gauge": (None, [_P, _P]),
     "ker_invert_openacc": (_I, [_P, _P, _P, _P, _D, _P, _P, _P, _P, _I, _D, C.POINTER(_I)]),
+    "ker_invert_openacc_noalloc": (_I, [_P, _P, _P, _P, _D, _P, _P, _P, _P, _I, _D, C.POINTER(_I)]),
     "inverter_mixed_precision": (_I, [InverterPackage, _P, _P, _P, _D, _I, _D, C.POINTER(_I)]),
     "multishift_invert": (_I, [_P, _P, _P, _P, _P, _D, _P, _P, _P, _P, _P, _I, C.POINTER(_I)]),
     "recombine_shifted_vec3_to_vec3": (None, [_P, _P, _P, _P]),

@@ -94,6 +94,10 @@ struct ForceScope {
     ~ForceScope();
 };
 
+// Address of the allocation-free solver scratch on `device` (solvers.hip);
+// called by ostp_init with that device current.
+bool noalloc_setup(int device);
+
 // Read the device solver state into the pinned mirror (synchronises the stream).
 bool pull_state();
 bool push_state();

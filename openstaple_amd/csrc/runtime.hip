@@ -177,6 +177,7 @@ int ostp_init(int nd0, int nd1, int nd2, int nd3, int par_temp) {
             return 1;
         c.max_blocks = blocks;
     }
+    if (!noalloc_setup(c.device)) return 1;
     c.lat = L;
     c.initialized = true;
     c.host_lat_set = false;

@@ -874,6 +874,48 @@ static void* grow_buffer(void*& p, size_t& have, size_t need) {
     return p;
 }
 
+// ------------------------------------------------ allocation-free CG scratch
+// ker_invert_openacc_noalloc keeps the reduction partials and the solver state
+// of its kernels in this file-scope device array (one copy per GPU, held
+// until the process ends) instead of the buffers ostp_init allocates.  Host
+// code must not pass `scratch` itself to a kernel or a copy: its address on
+// each device comes from hipGetSymbolAddress (noalloc_setup, called by
+// ostp_init with the device current) and is kept in g_scratch_dev.
+constexpr long kNoAllocMaxSizeh = 1L << 23;  // half-sites: a 64^4 lattice
+constexpr size_t kNoAllocPartialBytes = sizeof(double) * 4 * (size_t)((kNoAllocMaxSizeh + 63) / 64 + 64);
+constexpr size_t kNoAllocBytes = kNoAllocPartialBytes + ((sizeof(SolverState) + 255) / 256) * 256;
+static_assert(kNoAllocPartialBytes % 256 == 0, "state offset alignment");
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+constexpr int kNoAllocMaxDevices = 64;
+static void* g_scratch_dev[kNoAllocMaxDevices] = {};
+
+bool noalloc_setup(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return fail("device %d beyond %d", device, kNoAllocMaxDevices);
+    if (g_scratch_dev[device]) return true;
+    void* p = nullptr;
+    if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return false;
+    g_scratch_dev[device] = p;
+    return true;
+}
+
+// For the lifetime of the object, the solver context uses the static scratch
+// of the current device as its partials and state buffers.
+struct ScratchSwap {
+    double* partials;
+    SolverState* state;
+    int max_blocks;
+    explicit ScratchSwap(void* base) : partials(ctx().partials), state(ctx().state), max_blocks(ctx().max_blocks) {
+        ctx().partials = (double*)base;
+        ctx().state = (SolverState*)((char*)base + kNoAllocPartialBytes);
+        ctx().max_blocks = (int)((kNoAllocMaxSizeh + 63) / 64 + 64);
+    }
+    ~ScratchSwap() {
+        ctx().partials = partials;
+        ctx().state = state;
+        ctx().max_blocks = max_blocks;
+    }
+};
+
 }  // namespace ostp
 
 using namespace ostp;
@@ -903,6 +945,25 @@ int ker_invert_openacc(const su3_soa* u, ferm_param* pars, vec3_soa* solution, c
     int st = cg_solve(B, pars->ferm_mass, shift, res, max_cg, inverter_tricks.restartingEvery, &cg, nullptr);
     *cg_return = cg;
     return st;
+}
+
+int ker_invert_openacc_noalloc(const su3_soa* u, ferm_param* pars, vec3_soa* solution, const vec3_soa* in,
+                               double res, vec3_soa* loc_r, vec3_soa* loc_h, vec3_soa* loc_s, vec3_soa* loc_p,
+                               const int max_cg, double shift, int* cg_return) {
+    const char* nm = "ker_invert_openacc_noalloc";
+    if (!require_init(nm)) return INVERTER_FAILURE;
+    if (ctx().lat.sizeh > kNoAllocMaxSizeh) {
+        fail("%s: the lattice has %ld half-sites; the static scratch holds at most %ld (a 64^4 lattice)", nm,
+             ctx().lat.sizeh, kNoAllocMaxSizeh);
+        return INVERTER_FAILURE;
+    }
+    void* base = g_scratch_dev[ctx().device];
+    if (!base) {
+        fail("%s: no scratch address for device %d (ostp_init sets it up)", nm, ctx().device);
+        return INVERTER_FAILURE;
+    }
+    ScratchSwap swap(base);
+    return ker_invert_openacc(u, pars, solution, in, res, loc_r, loc_h, loc_s, loc_p, max_cg, shift, cg_return);
 }
 
 int inverter_mixed_precision(inverter_package ip, ferm_param* pars, vec3_soa* solution, const vec3_soa* in,
