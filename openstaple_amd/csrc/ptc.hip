@@ -8,18 +8,20 @@
 // with T_nu the eight parallel transports of D_oe (fermion_matrix.c:101-157,
 // before its 1/2 and signs): N = w0 + sum_c B_c^dag B_c is Hermitian (positive
 // for w0 > 0; trained networks have w0 < 0 with N still positive on the
-// spectrum), as conjugate-gradient methods need.  Links and transports are FP32; the
-// channel mixing (h = W t, g = W^dag h) runs on the matrix cores as two
-// chained BF16 MFMAs with FP32 accumulation (32x32x8: the 32x32 result of the
-// first product has exactly the lane layout of the B operand of the second,
-// so h never leaves registers); the transported vectors reach the MFMA
-// operand layout through LDS.
+// spectrum), as conjugate-gradient methods need.  Since g = W^dag W t, the
+// channel mixing of a site and colour is one Hermitian 8 x 8 product
+// g = M t, M = W^dag W, run on the matrix cores as a 32x32x16 BF16 MFMA with
+// FP32 accumulation (the real 16 x 16 form of M, 32 columns = 32 colour
+// vectors per MFMA; the transported vectors reach the operand layout through
+// LDS).  Links are FP16 records (ostp_half.h), transports FP32 arithmetic,
+// the channel fields g BF16 records; the inputs and outputs stay FP64.
 //
 // Training kernels: the loss |e|^2 or <e, A e> (e = N b - x) over (b, x)
 // pairs, its gradient via the 8x8 Gram matrix P[nu][mu] = <T_nu a, T_mu b>
 // with a = e or A e.  The CPU
 // oracle (oracle/ptc.c) restates forward, loss and gradient in FP64.
 #include "ostp_dslash.h"
+#include "ostp_half.h"
 
 #include <cmath>
 #include <cstring>
@@ -29,159 +31,245 @@ namespace ostp {
 
 constexpr int kC = OSTP_PTC_CHANNELS;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-// FP32 -> BF16, round to nearest even (finite inputs)
-__device__ __forceinline__ short to_bf16(float x) {
-    unsigned u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-}
 
 struct PtcParams {
     float w0;
     float w[kC][8][2];  // (re, im)
 };
 
-struct PtcGeom {
-    const float2* u[8];
-    const float* ph[8];
-    SiteGeom g;
+// The channel mixing g = W^dag (W t) of every site and colour is g = M t with
+// the 8 x 8 Hermitian M = W^dag W; its real 16 x 16 form (rows 2 nu + re/im,
+// columns 2 mu + re/im) is the A operand of one 32x32x16 BF16 MFMA per 32
+// columns (rows 16..31 zero).
+struct PtcMix {
+    float m[16][16];
 };
 
-__device__ __forceinline__ float2 ldv(const double2* p) {
-    const double2 v = *p;
-    return make_float2((float)v.x, (float)v.y);
-}
-__device__ __forceinline__ float2 ldv(const float2* p) { return *p; }
+// links of the preconditioner: the FP16 records of ostp_half.h (rows 0, 1 and
+// the phase), built from the FP32 link copy and FP32 phases by k_ptc_links
+struct PtcGeom {
+    const h8* la[8];
+    const h6* lb[8];  // (cos, sin) phases; with SGN read as h4 (+-1 phases in a sign bit)
+    SiteGeom g;
+};
 
 struct V3 {
     float2 c[3];
 };
 
-// One transport (FP32): U(iu) e^{i th(iu)} in(iv)  (DAG = false)
-//                       U(iu)^dag e^{-i th(iu)} in(iv)  (DAG = true)
-template <bool DAG, typename In>
-__device__ __forceinline__ V3 xport(const float2* __restrict__ U, const float* __restrict__ P,
-                                    const In* __restrict__ in, int sizeh, int iu, int iv) {
-    const float2 a0 = U[iu], a1 = U[sizeh + iu], a2 = U[2 * sizeh + iu];
-    const float2 b0 = U[3 * sizeh + iu], b1 = U[4 * sizeh + iu], b2 = U[5 * sizeh + iu];
-    float s, c;
-    sincos_t(P[iu], &s, &c);
-    const float2 ph = DAG ? make_float2(c, -s) : make_float2(c, s);
-    const float2 v0 = cmul(ldv(in + iv), ph), v1 = cmul(ldv(in + sizeh + iv), ph), v2 = cmul(ldv(in + 2 * sizeh + iv), ph);
-    const float2 x0 = csub(cmul(a1, b2), cmul(a2, b1));
-    const float2 x1 = csub(cmul(a2, b0), cmul(a0, b2));
-    const float2 x2 = csub(cmul(a0, b1), cmul(a1, b0));
-    V3 o;
-    if (!DAG) {
-        float2 r0 = cmul(a0, v0); r0 = cfma(a1, v1, r0); o.c[0] = cfma(a2, v2, r0);
-        float2 r1 = cmul(b0, v0); r1 = cfma(b1, v1, r1); o.c[1] = cfma(b2, v2, r1);
-        float2 r2 = cmul(cconj(x0), v0); r2 = cfmac(x1, v1, r2); o.c[2] = cfmac(x2, v2, r2);
+// Channel fields g_nu of the odd sites: three complex BF16 numbers per record
+// (re in the low half of each dword).  BF16 keeps FP32's exponent range, so
+// the fields of a residual that has fallen by 10 orders of magnitude stay
+// normal numbers (FP16 would flush them to zero); their 8-bit mantissa
+// matches the MFMA operands they come from.
+struct __attribute__((aligned(4))) b6 {
+    unsigned int v[3];
+};
+__device__ __forceinline__ unsigned int bf_pack(float x, float y) {
+    return (unsigned int)__builtin_bit_cast(unsigned short, (__bf16)x) |
+           ((unsigned int)__builtin_bit_cast(unsigned short, (__bf16)y) << 16);
+}
+__device__ __forceinline__ float2 bf_unpack(unsigned int w) {
+    return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+}
+typedef __attribute__((address_space(1))) const unsigned int gcu32;
+typedef __attribute__((address_space(1))) unsigned int gu32;
+__device__ __forceinline__ b6 ldg(const b6* p) {
+    gcu32* g = (gcu32*)p;
+    b6 r;
+    r.v[0] = g[0]; r.v[1] = g[1]; r.v[2] = g[2];
+    return r;
+}
+__device__ __forceinline__ void stg(b6* p, const b6& v) {
+    gu32* g = (gu32*)p;
+    g[0] = v.v[0]; g[1] = v.v[1]; g[2] = v.v[2];
+}
+
+__device__ __forceinline__ float2 ldv(const double2* p) {
+    const double2 v = *p;
+    return make_float2((float)v.x, (float)v.y);
+}
+
+// One transport: U e^{i th} v (DAG = false) or U^dag e^{-i th} v (DAG = true)
+// with the link of array l at half-site iu
+template <bool DAG, bool SGN>
+__device__ __forceinline__ V3 xport(const PtcGeom& A, int l, int iu, float2 v0, float2 v1, float2 v2) {
+    const h8 la = ldg(A.la[l] + iu);
+    float2 a0 = h2f(la.v[0]);
+    const float2 a1 = h2f(la.v[1]), a2 = h2f(la.v[2]), b0 = h2f(la.v[3]);
+    float2 b1, b2;
+    if constexpr (SGN) {
+        const h4 lc = ldg((const h4*)A.lb[l] + iu);
+        b1 = h2f(lc.v[0]);
+        b2 = h2f(lc.v[1]);
+        const unsigned short bits = __builtin_bit_cast(unsigned short, la.v[0].x);
+        a0.x = (float)__builtin_bit_cast(_Float16, (unsigned short)(bits & 0xbfffu));
+        if (bits & 0x4000u) {
+            v0 = make_float2(-v0.x, -v0.y);
+            v1 = make_float2(-v1.x, -v1.y);
+            v2 = make_float2(-v2.x, -v2.y);
+        }
     } else {
-        float2 r0 = cmul(cconj(a0), v0); r0 = cfmac(b0, v1, r0); o.c[0] = cfma(x0, v2, r0);
-        float2 r1 = cmul(cconj(a1), v0); r1 = cfmac(b1, v1, r1); o.c[1] = cfma(x1, v2, r1);
-        float2 r2 = cmul(cconj(a2), v0); r2 = cfmac(b2, v1, r2); o.c[2] = cfma(x2, v2, r2);
+        const h6 lb = ldg(A.lb[l] + iu);
+        b1 = h2f(lb.v[0]);
+        b2 = h2f(lb.v[1]);
+        const float2 cs = h2f(lb.v[2]);
+        const float2 ph = DAG ? make_float2(cs.x, -cs.y) : cs;
+        v0 = cmul(v0, ph);
+        v1 = cmul(v1, ph);
+        v2 = cmul(v2, ph);
+    }
+    float2 o0 = make_float2(0.f, 0.f), o1 = o0, o2 = o0;
+    hop_rows<DAG>(a0, a1, a2, b0, b1, b2, v0, v1, v2, o0, o1, o2);  // DAG subtracts
+    V3 o;
+    if (DAG) {
+        o.c[0] = make_float2(-o0.x, -o0.y);
+        o.c[1] = make_float2(-o1.x, -o1.y);
+        o.c[2] = make_float2(-o2.x, -o2.y);
+    } else {
+        o.c[0] = o0;
+        o.c[1] = o1;
+        o.c[2] = o2;
     }
     return o;
 }
 
-// The eight transports T_nu in (in: even field) at odd half-site y.
-template <typename In>
-__device__ __forceinline__ void transports(const PtcGeom& A, const In* in, int y, V3 t[8]) {
+template <bool DAG, bool SGN>
+__device__ __forceinline__ V3 xport(const PtcGeom& A, int l, int iu, const double2* __restrict__ in, int sizeh,
+                                    int iv) {
+    return xport<DAG, SGN>(A, l, iu, ldv(in + iv), ldv(in + sizeh + iv), ldv(in + 2 * sizeh + iv));
+}
+
+// The eight transports T_nu in (in: even FP64 field) at odd half-site y
+// (the hops of D_oe, fermion_matrix.c:101-157, before its 1/2 and signs).
+template <bool SGN>
+__device__ __forceinline__ void transports(const PtcGeom& A, const double2* in, int y, V3 t[8]) {
     const Nbrs nb = site_neighbours<1>(A.g, y);
+    const int sizeh = A.g.sizeh;
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
-        t[mu] = xport<false>(A.u[2 * mu + 1], A.ph[2 * mu + 1], in, A.g.sizeh, y, nb.ip[mu]);
-        t[mu + 4] = xport<true>(A.u[2 * mu], A.ph[2 * mu], in, A.g.sizeh, nb.im[mu], nb.im[mu]);
+        t[mu] = xport<false, SGN>(A, 2 * mu + 1, y, in, sizeh, nb.ip[mu]);
+        t[mu + 4] = xport<true, SGN>(A, 2 * mu, nb.im[mu], in, sizeh, nb.im[mu]);
     }
 }
 
-// Layer 1 + channel mixing: g_nu(y) = sum_c conj(w[c][nu]) sum_mu w[c][mu] T_mu r (y), odd y.
+// FP16 link records of the preconditioner from the FP32 links and phases
+// (sgn: every phase is +-1, carried in a sign bit; flag: set when some phase
+// is not, or some row 0 entry 0 exceeds 1 in magnitude)
+struct PtcLinkSrc {
+    const float2* u[8];
+    const float* ph[8];
+    h8* la[8];
+    h6* lb[8];
+    int sizeh;
+};
+__global__ __launch_bounds__(kBlock) void k_ptc_links_check(PtcLinkSrc a, int* flag) {
+    const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (gid >= 8L * a.sizeh) return;
+    const int l = (int)(gid / a.sizeh), i = (int)(gid - (long)l * a.sizeh);
+    float sn, cs;
+    sincos_t(a.ph[l][i], &sn, &cs);
+    const h2 p = f2h(cs, sn);
+    const float c = (float)p.x, sv = (float)p.y;
+    const float e = (float)(_Float16)a.u[l][i].x;
+    if (!(sv == 0.f && (c == 1.f || c == -1.f)) || !(fabsf(e) <= 1.f)) *flag = 1;
+}
+__global__ __launch_bounds__(kBlock) void k_ptc_links(PtcLinkSrc a, int sgn) {
+    const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (gid >= 8L * a.sizeh) return;
+    const int l = (int)(gid / a.sizeh), i = (int)(gid - (long)l * a.sizeh);
+    h2 e[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const float2 v = a.u[l][(long)q * a.sizeh + i];
+        e[q] = f2h(v.x, v.y);
+    }
+    float sn, cs;
+    sincos_t(a.ph[l][i], &sn, &cs);
+    const h2 ph = f2h(cs, sn);
+    h8 la;
+    la.v[0] = e[0]; la.v[1] = e[1]; la.v[2] = e[2]; la.v[3] = e[3];
+    if (sgn) {
+        unsigned short bits = __builtin_bit_cast(unsigned short, la.v[0].x);
+        bits = (unsigned short)(bits | ((float)ph.x < 0.f ? 0x4000u : 0u));
+        la.v[0].x = __builtin_bit_cast(_Float16, bits);
+        h4 lc;
+        lc.v[0] = e[4]; lc.v[1] = e[5];
+        stg(a.la[l] + i, la);
+        stg((h4*)a.lb[l] + i, lc);
+    } else {
+        h6 lb;
+        lb.v[0] = e[4]; lb.v[1] = e[5]; lb.v[2] = ph;
+        stg(a.la[l] + i, la);
+        stg(a.lb[l] + i, lb);
+    }
+}
+
+// Layer 1 + channel mixing: g_nu(y) = sum_mu M[nu][mu] T_mu r (y) at odd y,
+// stored as one BF16 record (3 colours) per channel nu and half-site.
 // One wave = 64 odd sites; MFMA columns n = colour * 64 + site-in-wave.
-template <typename In>
-__global__ __launch_bounds__(256) void k_ptc_layer1(PtcGeom A, const In* __restrict__ r, float2* __restrict__ gout,
-                                                    PtcParams W, SolverCtl ctl) {
+template <bool SGN>
+__global__ __launch_bounds__(256) void k_ptc_layer1(PtcGeom A, const double2* __restrict__ r, b6* __restrict__ gout,
+                                                    PtcMix M, SolverCtl ctl) {
     if (!pred_pass(ctl)) return;
-    __shared__ bf16x4 lds[4][192 * 4];  // per wave: [n][k] with k = 2 nu + re/im (16 bf16 per column)
+    __shared__ bf16x8 lds[4][192 * 2];  // per wave: [n][k], k = 2 mu + re/im (16 bf16 per column)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int base = blockIdx.x * 256 + wave * 64;
     const int y = base + lane;
     const int sizeh = A.g.sizeh;
     V3 t[8];
     if (y < sizeh) {
-        transports(A, r, y, t);
+        transports<SGN>(A, r, y, t);
     } else {
 #pragma unroll
         for (int mu = 0; mu < 8; mu++) t[mu].c[0] = t[mu].c[1] = t[mu].c[2] = make_float2(0.f, 0.f);
     }
-    bf16x4* T = lds[wave];
+    bf16x8* T = lds[wave];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const int n = c * 64 + lane;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {  // k = 4q .. 4q+3 = (nu = 2q, re, im), (nu = 2q+1, re, im)
-            bf16x4 v;
-            v[0] = to_bf16(t[2 * q].c[c].x);
-            v[1] = to_bf16(t[2 * q].c[c].y);
-            v[2] = to_bf16(t[2 * q + 1].c[c].x);
-            v[3] = to_bf16(t[2 * q + 1].c[c].y);
-            T[n * 4 + q] = v;
+        for (int q = 0; q < 2; q++) {  // k = 8q .. 8q+7: mu = 4q .. 4q+3, (re, im) each
+            bf16x8 v;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                v[2 * e] = (__bf16)t[4 * q + e].c[c].x;
+                v[2 * e + 1] = (__bf16)t[4 * q + e].c[c].y;
+            }
+            T[n * 2 + q] = v;
         }
     }
     __syncthreads();
-    // A operands.  Lane: row m = lane % 32, k = 8 s + 4 (lane / 32) + e.
-    const int m = lane & 31, kh = 4 * (lane >> 5);
-    bf16x4 a1[2], a2[3];
+    // A operand: row m = lane % 32, k = 8 (lane / 32) + e
+    const int m = lane & 31, h = lane >> 5;
+    bf16x8 a;
 #pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            // stage 1: H = W t, rows m = 2c + ri (c < 12), cols k = 2 nu + rk
-            const int k = 8 * s + kh + e, c = m >> 1, nu = k >> 1;
-            float v = 0.f;
-            if (c < kC) {
-                const float wr = W.w[c][nu][0], wi = W.w[c][nu][1];
-                v = (m & 1) ? ((k & 1) ? wr : wi) : ((k & 1) ? -wi : wr);
-            }
-            a1[s][e] = to_bf16(v);
-        }
-#pragma unroll
-    for (int s = 0; s < 3; s++)
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            // stage 2: G = W^dag H, rows m = 2 nu + ri (nu < 8), cols k = 2c + rk (c < 12)
-            const int k = 8 * s + kh + e, nu = m >> 1, c = k >> 1;
-            float v = 0.f;
-            if (nu < 8) {
-                const float wr = W.w[c][nu][0], wi = W.w[c][nu][1];
-                v = (m & 1) ? ((k & 1) ? wr : -wi) : ((k & 1) ? wi : wr);
-            }
-            a2[s][e] = to_bf16(v);
-        }
+    for (int e = 0; e < 8; e++) a[e] = (__bf16)(m < 16 ? M.m[m][8 * h + e] : 0.f);
+    // result: lane holds column n = 32 j + m, rows 8 (v / 4) + 4 h + v % 4; rows < 16 are (nu, re/im):
+    // v = 0..7 -> nu = 2h + (v / 2) % 2 (v < 4) or 4 + 2h + (v / 2) % 2 (4 <= v < 8)
+    float2 acc[2][4][3];  // [site parity p][nu slot][colour]
 #pragma unroll
     for (int j = 0; j < 6; j++) {
         const int n = j * 32 + m;
-        f32x16 acc1 = {0.f};
+        f32x16 d = {0.f};
+        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, T[n * 2 + h], d, 0, 0, 0);
+        const int p = j & 1, c = j >> 1;  // site = m + 32 p, colour c
 #pragma unroll
-        for (int s = 0; s < 2; s++) acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1[s], T[n * 4 + 2 * s + (lane >> 5)], acc1, 0, 0, 0);
-        f32x16 acc2 = {0.f};
+        for (int q = 0; q < 4; q++) acc[p][q][c] = make_float2(d[2 * q], d[2 * q + 1]);
+    }
 #pragma unroll
-        for (int s = 0; s < 3; s++) {
-            bf16x4 b;
-#pragma unroll
-            for (int e = 0; e < 4; e++) b[e] = to_bf16(acc1[4 * s + e]);
-            acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2[s], b, acc2, 0, 0, 0);
-        }
-        // lane holds column n, rows 8 (v/4) + 4 (lane/32) + v%4; rows < 16 are (nu, re/im)
-        const int colour = n >> 6, site = base + (n & 63);
+    for (int p = 0; p < 2; p++) {
+        const int site = base + m + 32 * p;
         if (site < sizeh) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int row = 8 * (q >> 1) + kh + 2 * (q & 1);
-                const int nu = row >> 1;
-                gout[(size_t)(nu * 3 + colour) * sizeh + site] = make_float2(acc2[2 * q], acc2[2 * q + 1]);
+                const int nu = (q < 2 ? 0 : 4) + 2 * h + (q & 1);
+                b6 rec;
+#pragma unroll
+                for (int c = 0; c < 3; c++) rec.v[c] = bf_pack(acc[p][q][c].x, acc[p][q][c].y);
+                stg(gout + (size_t)nu * sizeh + site, rec);
             }
         }
     }
@@ -193,8 +281,8 @@ enum PtcMode : int { kPtcApply = 0, kPtcFcg = 1, kPtcTrain = 2 };
 //  kPtcApply: out = z
 //  kPtcFcg:   out = z, partials (Re<z,q>, Re<z,r>)        (aux = q; q == nullptr -> 0)
 //  kPtcTrain: out = e = z - x, partials (|e|^2, Re<e,r>)  (aux = x)
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_ptc_layer2(PtcGeom A, const float2* __restrict__ g,
+template <int MODE, bool SGN>
+__global__ __launch_bounds__(kBlock) void k_ptc_layer2(PtcGeom A, const b6* __restrict__ g,
                                                        const double2* __restrict__ r, double2* __restrict__ out,
                                                        const double2* __restrict__ aux, float w0, SolverCtl ctl) {
     if (!pred_pass(ctl)) return;
@@ -207,10 +295,12 @@ __global__ __launch_bounds__(kBlock) void k_ptc_layer2(PtcGeom A, const float2* 
 #pragma unroll
         for (int mu = 0; mu < 4; mu++) {
             // adjoint of the forward transport: U_mu(x - mu)^dag e^{-i th} g_mu(x - mu)
-            const V3 a = xport<true>(A.u[2 * mu + 1], A.ph[2 * mu + 1], g + (size_t)mu * 3 * sizeh, sizeh, nb.im[mu],
-                                     nb.im[mu]);
+            const b6 ga = ldg(g + (size_t)mu * sizeh + nb.im[mu]);
+            const V3 a = xport<true, SGN>(A, 2 * mu + 1, nb.im[mu], bf_unpack(ga.v[0]), bf_unpack(ga.v[1]),
+                                          bf_unpack(ga.v[2]));
             // adjoint of the backward transport: U_mu(x) e^{i th} g_{mu+4}(x + mu)
-            const V3 b = xport<false>(A.u[2 * mu], A.ph[2 * mu], g + (size_t)(mu + 4) * 3 * sizeh, sizeh, x, nb.ip[mu]);
+            const b6 gb = ldg(g + (size_t)(mu + 4) * sizeh + nb.ip[mu]);
+            const V3 b = xport<false, SGN>(A, 2 * mu, x, bf_unpack(gb.v[0]), bf_unpack(gb.v[1]), bf_unpack(gb.v[2]));
 #pragma unroll
             for (int c = 0; c < 3; c++) acc[c] = cadd(acc[c], cadd(a.c[c], b.c[c]));
         }
@@ -239,6 +329,7 @@ __global__ __launch_bounds__(kBlock) void k_ptc_layer2(PtcGeom A, const float2* 
 
 // Gradient Gram matrix: per odd site, P[nu][mu] += <T_nu e, T_mu b>; block
 // partials of the 128 doubles (re/im of 64 entries), nu-major.
+template <bool SGN>
 __global__ __launch_bounds__(kBlock) void k_ptc_gram(PtcGeom A, const double2* __restrict__ b,
                                                      const double2* __restrict__ e, double* __restrict__ partials) {
     __shared__ double lds[kBlock / 64][128];
@@ -248,16 +339,15 @@ __global__ __launch_bounds__(kBlock) void k_ptc_gram(PtcGeom A, const double2* _
     const bool valid = y < A.g.sizeh;
     Nbrs nb;
     if (valid) {
-        transports(A, b, y, t);
+        transports<SGN>(A, b, y, t);
         nb = site_neighbours<1>(A.g, y);
     }
 #pragma unroll
     for (int nu = 0; nu < 8; nu++) {
         V3 a;
         if (valid) {
-            a = (nu < 4) ? xport<false>(A.u[2 * nu + 1], A.ph[2 * nu + 1], e, A.g.sizeh, y, nb.ip[nu])
-                         : xport<true>(A.u[2 * (nu - 4)], A.ph[2 * (nu - 4)], e, A.g.sizeh, nb.im[nu - 4],
-                                       nb.im[nu - 4]);
+            a = (nu < 4) ? xport<false, SGN>(A, 2 * nu + 1, y, e, A.g.sizeh, nb.ip[nu])
+                         : xport<true, SGN>(A, 2 * (nu - 4), nb.im[nu - 4], e, A.g.sizeh, nb.im[nu - 4]);
         }
 #pragma unroll
         for (int mu = 0; mu < 8; mu++) {
@@ -307,42 +397,29 @@ static bool params_of(const float* p, PtcParams& W) {
     return true;
 }
 
-static bool geom_of(const void* u_f, const void* ph_f, PtcGeom& A, const char* nm) {
-    const Lattice& L = ctx().lat;
-    const char* u = (const char*)resolve(u_f, 8 * L.su3f_bytes(), nm);
-    const char* ph = (const char*)resolve(ph_f, 8 * L.fsoa_bytes(), nm);
-    if (!u || !ph) return false;
-    for (int l = 0; l < 8; l++) {
-        A.u[l] = (const float2*)(u + l * L.su3f_bytes());
-        A.ph[l] = (const float*)(ph + l * L.fsoa_bytes());
-    }
-    A.g.nd0 = L.nd[0];
-    A.g.nd0h = L.nd[0] / 2;
-    A.g.nd1 = L.nd[1];
-    A.g.nd2 = L.nd[2];
-    A.g.nd3 = L.nd[3];
-    A.g.sizeh = (int)L.sizeh;
-    return true;
-}
-
-// library-owned work space: g (8 odd FP32 fields), e, f, h (FP64), gram partials + sum
+// library-owned work space: g (8 odd BF16 channel fields), the FP16 link
+// records, e, f, h (FP64), gram partials + sum, the link check flag
 struct PtcScratch {
     void* p = nullptr;
     size_t bytes = 0;
-    float2* g = nullptr;
+    b6* g = nullptr;
+    h8* la = nullptr;
+    h6* lb = nullptr;
     double2* e = nullptr;
     double2* f = nullptr;
     double2* h = nullptr;
     double* part = nullptr;
     double* gram = nullptr;
+    int* flag = nullptr;
 };
 static PtcScratch g_ps;
 
 static bool ptc_scratch() {
     const long sh = ctx().lat.sizeh;
     const int nb = blocks_for(sh);
-    const size_t gb = (size_t)8 * 24 * sh, eb = (size_t)48 * sh, pb = (size_t)nb * 128 * 8;
-    const size_t need = gb + 3 * eb + pb + 128 * 8 + 6 * 256;
+    const size_t gb = (size_t)8 * 12 * sh, lab = (size_t)8 * 16 * sh, lbb = (size_t)8 * 12 * sh,
+                 eb = (size_t)48 * sh, pb = (size_t)nb * 128 * 8;
+    const size_t need = gb + lab + lbb + 3 * eb + pb + 128 * 8 + 256 + 10 * 256;
     if (g_ps.bytes < need) {
         if (g_ps.p) (void)hipFree(g_ps.p);
         g_ps = PtcScratch{};
@@ -355,30 +432,95 @@ static bool ptc_scratch() {
         c += (n + 255) & ~(size_t)255;
         return q;
     };
-    g_ps.g = (float2*)take(gb);
+    g_ps.g = (b6*)take(gb);
+    g_ps.la = (h8*)take(lab);
+    g_ps.lb = (h6*)take(lbb);
     g_ps.e = (double2*)take(eb);
     g_ps.f = (double2*)take(eb);
     g_ps.h = (double2*)take(eb);
     g_ps.part = (double*)take(pb);
     g_ps.gram = (double*)take(128 * 8);
+    g_ps.flag = (int*)take(256);
     return true;
 }
 
-// z = N r on device pointers (r, z FP64 even fields), queued on the library stream
-static bool launch_apply(const PtcGeom& A, const PtcParams& W, const double2* r, double2* z, int mode,
-                         const double2* aux, const SolverCtl& ctl) {
+// Links of the preconditioner: FP16 records built from the FP32 link copy and
+// the FP32 phases (the compact 24 B form when every phase is +-1, reported in
+// sgn); needs the scratch.  Returns false on error.
+static bool geom_of(const void* u_f, const void* ph_f, PtcGeom& A, bool& sgn, const char* nm) {
+    const Lattice& L = ctx().lat;
+    const char* u = (const char*)resolve(u_f, 8 * L.su3f_bytes(), nm);
+    const char* ph = (const char*)resolve(ph_f, 8 * L.fsoa_bytes(), nm);
+    if (!u || !ph) return false;
+    const long sh = L.sizeh;
+    PtcLinkSrc src;
+    for (int l = 0; l < 8; l++) {
+        src.u[l] = (const float2*)(u + l * L.su3f_bytes());
+        src.ph[l] = (const float*)(ph + l * L.fsoa_bytes());
+        src.la[l] = g_ps.la + (size_t)l * sh;
+        src.lb[l] = g_ps.lb + (size_t)l * sh;  // read as an h4 array when sgn (8 B records)
+        A.la[l] = src.la[l];
+        A.lb[l] = src.lb[l];
+    }
+    src.sizeh = (int)sh;
+    const dim3 grid((unsigned)((8 * sh + kBlock - 1) / kBlock));
+    int flag = 0;
+    if (!hip_ok(hipMemsetAsync(g_ps.flag, 0, sizeof(int), S()), nm)) return false;
+    hipLaunchKernelGGL(k_ptc_links_check, grid, dim3(kBlock), 0, S(), src, g_ps.flag);
+    if (!hip_ok(hipGetLastError(), nm) || ostp_memcpy_dtoh(&flag, g_ps.flag, sizeof flag) != 0) return false;
+    sgn = flag == 0;
+    hipLaunchKernelGGL(k_ptc_links, grid, dim3(kBlock), 0, S(), src, sgn ? 1 : 0);
+    A.g.nd0 = L.nd[0];
+    A.g.nd0h = L.nd[0] / 2;
+    A.g.nd1 = L.nd[1];
+    A.g.nd2 = L.nd[2];
+    A.g.nd3 = L.nd[3];
+    A.g.sizeh = (int)sh;
+    return hip_ok(hipGetLastError(), nm);
+}
+
+// M = W^dag W in the real 16 x 16 form of the layer-1 MFMA
+static PtcMix mix_of(const PtcParams& W) {
+    PtcMix M;
+    for (int nu = 0; nu < 8; nu++)
+        for (int mu = 0; mu < 8; mu++) {
+            double a = 0.0, b = 0.0;  // sum_c conj(w[c][nu]) w[c][mu]
+            for (int c = 0; c < kC; c++) {
+                const double xr = W.w[c][nu][0], xi = W.w[c][nu][1], yr = W.w[c][mu][0], yi = W.w[c][mu][1];
+                a += xr * yr + xi * yi;
+                b += xr * yi - xi * yr;
+            }
+            M.m[2 * nu][2 * mu] = (float)a;
+            M.m[2 * nu][2 * mu + 1] = (float)-b;
+            M.m[2 * nu + 1][2 * mu] = (float)b;
+            M.m[2 * nu + 1][2 * mu + 1] = (float)a;
+        }
+    return M;
+}
+
+template <bool SGN>
+static void launch_apply_t(const PtcGeom& A, const PtcMix& M, float w0, const double2* r, double2* z, int mode,
+                           const double2* aux, const SolverCtl& ctl) {
     const long sh = A.g.sizeh;
-    hipLaunchKernelGGL((k_ptc_layer1<double2>), dim3((sh + 255) / 256), dim3(256), 0, S(), A, r, g_ps.g, W, ctl);
+    hipLaunchKernelGGL((k_ptc_layer1<SGN>), dim3((unsigned)((sh + 255) / 256)), dim3(256), 0, S(), A, r, g_ps.g, M,
+                       ctl);
     const int nb = blocks_for(sh);
+    const b6* g = g_ps.g;
     if (mode == kPtcApply)
-        hipLaunchKernelGGL((k_ptc_layer2<kPtcApply>), dim3(nb), dim3(kBlock), 0, S(), A, (const float2*)g_ps.g, r, z,
-                           aux, W.w0, ctl);
+        hipLaunchKernelGGL((k_ptc_layer2<kPtcApply, SGN>), dim3(nb), dim3(kBlock), 0, S(), A, g, r, z, aux, w0, ctl);
     else if (mode == kPtcFcg)
-        hipLaunchKernelGGL((k_ptc_layer2<kPtcFcg>), dim3(nb), dim3(kBlock), 0, S(), A, (const float2*)g_ps.g, r, z,
-                           aux, W.w0, ctl);
+        hipLaunchKernelGGL((k_ptc_layer2<kPtcFcg, SGN>), dim3(nb), dim3(kBlock), 0, S(), A, g, r, z, aux, w0, ctl);
     else
-        hipLaunchKernelGGL((k_ptc_layer2<kPtcTrain>), dim3(nb), dim3(kBlock), 0, S(), A, (const float2*)g_ps.g, r, z,
-                           aux, W.w0, ctl);
+        hipLaunchKernelGGL((k_ptc_layer2<kPtcTrain, SGN>), dim3(nb), dim3(kBlock), 0, S(), A, g, r, z, aux, w0, ctl);
+}
+
+// z = N r on device pointers (r, z FP64 even fields), queued on the library stream
+static bool launch_apply(const PtcGeom& A, bool sgn, const PtcMix& M, float w0, const double2* r, double2* z,
+                         int mode, const double2* aux, const SolverCtl& ctl) {
+    if (sgn)
+        launch_apply_t<true>(A, M, w0, r, z, mode, aux, ctl);
+    else
+        launch_apply_t<false>(A, M, w0, r, z, mode, aux, ctl);
     return hip_ok(hipGetLastError(), "ptc launch");
 }
 
@@ -400,12 +542,13 @@ int ostp_ptc_apply(const float* params, const su3_soa_f* u_f, const float_soa* p
     if (!require_init(nm)) return 1;
     PtcParams W;
     PtcGeom A;
-    if (!params_of(params, W) || !geom_of(u_f, phases_f, A, nm) || !ptc_scratch()) return 1;
+    bool sgn = false;
+    if (!params_of(params, W) || !ptc_scratch() || !geom_of(u_f, phases_f, A, sgn, nm)) return 1;
     const size_t vb = ctx().lat.vec3_bytes();
     const double2* r = (const double2*)resolve(in, vb, nm);
     double2* z = (double2*)resolve(out, vb, nm);
     if (!r || !z) return 1;
-    if (!launch_apply(A, W, r, z, kPtcApply, nullptr, plain_ctl())) return 1;
+    if (!launch_apply(A, sgn, mix_of(W), W.w0, r, z, kPtcApply, nullptr, plain_ctl())) return 1;
     return entry_done(nm) ? 0 : 1;
 }
 
@@ -419,7 +562,8 @@ double ostp_ptc_loss_grad(const float* params, const su3_soa* u, const su3_soa_f
     }
     PtcParams W;
     PtcGeom A;
-    if (!params_of(params, W) || !geom_of(u_f, pars->phases_f, A, nm) || !ptc_scratch()) return -1.0;
+    bool sgn = false;
+    if (!params_of(params, W) || !ptc_scratch() || !geom_of(u_f, pars->phases_f, A, sgn, nm)) return -1.0;
     const Lattice& Lt = ctx().lat;
     const size_t vb = Lt.vec3_bytes();
     const double2* db = (const double2*)resolve(b, vb, nm);
@@ -432,7 +576,7 @@ double ostp_ptc_loss_grad(const float* params, const su3_soa* u, const su3_soa_f
     const int nb = blocks_for(sh);
     SolverState& st = *ctx().state_host;
     // e = N b - x ; |e|^2 and Re<e, b>
-    if (!launch_apply(A, W, db, g_ps.e, kPtcTrain, dx, ctl) || !launch_finalize(kFinStore2, ctl, nb) || !pull_state())
+    if (!launch_apply(A, sgn, mix_of(W), W.w0, db, g_ps.e, kPtcTrain, dx, ctl) || !launch_finalize(kFinStore2, ctl, nb) || !pull_state())
         return -1.0;
     double L = st.sum, s0 = st.sum_im;
     const double2* adj = g_ps.e;  // dL = 2 Re<adj, dz>
@@ -445,7 +589,10 @@ double ostp_ptc_loss_grad(const float* params, const su3_soa* u, const su3_soa_f
         if (!reduce_to_host(1, g_ps.f, db, 0, &s0, nullptr)) return -1.0;
         adj = g_ps.f;
     }
-    hipLaunchKernelGGL(k_ptc_gram, dim3(nb), dim3(kBlock), 0, S(), A, db, adj, g_ps.part);
+    if (sgn)
+        hipLaunchKernelGGL((k_ptc_gram<true>), dim3(nb), dim3(kBlock), 0, S(), A, db, adj, g_ps.part);
+    else
+        hipLaunchKernelGGL((k_ptc_gram<false>), dim3(nb), dim3(kBlock), 0, S(), A, db, adj, g_ps.part);
     hipLaunchKernelGGL(k_ptc_gram_sum, dim3(1), dim3(128), 0, S(), (const double*)g_ps.part, nb, g_ps.gram);
     double P[128];
     if (!hip_ok(hipGetLastError(), nm) || ostp_memcpy_dtoh(P, g_ps.gram, sizeof P) != 0) return -1.0;
@@ -481,7 +628,10 @@ int ostp_fcg_solve(const su3_soa* u, ferm_param* pars, const su3_soa_f* u_f, con
     const Lattice& L = ctx().lat;
     PtcParams W;
     PtcGeom A;
-    if (!params_of(params, W) || !geom_of(u_f, pars->phases_f, A, nm) || !ptc_scratch()) return INVERTER_FAILURE;
+    bool sgn = false;
+    if (!params_of(params, W) || !ptc_scratch() || !geom_of(u_f, pars->phases_f, A, sgn, nm))
+        return INVERTER_FAILURE;
+    const PtcMix M = mix_of(W);
     const size_t vb = L.vec3_bytes();
     const void* du = resolve(u, 8 * L.su3_bytes(), nm);
     const void* dph = resolve(pars->phases, 8 * L.dsoa_bytes(), nm);
@@ -522,7 +672,7 @@ int ostp_fcg_solve(const su3_soa* u, ferm_param* pars, const su3_soa_f* u_f, con
     // r = b - A x ; z = N r ; <z, r> ; p = z
     if (!launch_mdagm<double>(du, dph, x, h, q, m2, 1, kFinStore, ctl_pred(kAlways)) || !launch_sub(b, q, r))
         return INVERTER_FAILURE;
-    if (!launch_apply(A, W, r, z, kPtcFcg, nullptr, ctl_pred(kAlways)) ||
+    if (!launch_apply(A, sgn, M, W.w0, r, z, kPtcFcg, nullptr, ctl_pred(kAlways)) ||
         !launch_finalize(kFinFcgInit, ctl_pred(kAlways), nbh) ||
         !hip_ok(hipMemcpyAsync(p, z, vb, hipMemcpyDeviceToDevice, S()), "p = z"))
         return INVERTER_FAILURE;
@@ -542,7 +692,7 @@ int ostp_fcg_solve(const su3_soa* u, ferm_param* pars, const su3_soa_f* u_f, con
             // x += alpha p ; r -= alpha q ; |r|^2, exit test
             if (!launch_cg_xr(x, r, p, q, run) || !launch_finalize(kFinFcgRes, run, nbh)) return INVERTER_FAILURE;
             // z = N r ; beta = -alpha <z,q> / <z,r>_old ; p = z + beta p
-            if (!launch_apply(A, W, r, z, kPtcFcg, q, run) || !launch_finalize(kFinFcgBeta, run, nbh) ||
+            if (!launch_apply(A, sgn, M, W.w0, r, z, kPtcFcg, q, run) || !launch_finalize(kFinFcgBeta, run, nbh) ||
                 !launch_cg_p(p, z, run))
                 return INVERTER_FAILURE;
         }

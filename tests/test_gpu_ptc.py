@@ -1,6 +1,7 @@
 """Learned parallel-transport preconditioner on the GPU.
 
-Bar (BF16 channel mixing with FP32 accumulation, FP32 transports):
+Bar (FP16 link records, FP32 transports, channel mixing M = W^dag W as a BF16
+MFMA with FP32 accumulation, BF16 channel fields):
   * N r against the FP64 oracle (oracle/ptc.c, pinned by tests/test_oracle_ptc.py)
     within 2e-2 relative (norm-wise);
   * loss within 2e-2 and gradient within 5e-2 relative of the oracle, for the
@@ -30,11 +31,11 @@ def rel(a, b):
 
 
 class Sys:
-    def __init__(self, shape=(8, 8, 8, 8), mass=0.005, eps=0.7, seed=2):
+    def __init__(self, shape=(8, 8, 8, 8), mass=0.005, eps=0.7, seed=2, **phase_args):
         self.o = o = Oracle(*shape)
         init_lattice(shape)
         self.u = o.stout(o.stout(o.gen_cold(eps, seed), 0.15), 0.15)
-        self.ph = o.phases()
+        self.ph = o.phases(**phase_args)
         self.uf = o.to_float_gauge(self.u)
         self.du, self.duf = Dev(self.u), Dev(self.uf)
         self.dph, self.dphf = Dev(self.ph), Dev(self.ph.astype(np.float32))
@@ -83,6 +84,37 @@ def test_apply_vs_oracle(s8):
     lib.ostp_ptc_apply(p32.ctypes.data, s8.duf.p, s8.dphf.p, dr.p, dz.p)
     ref = 0.7 * r - 4.0 * o.deo(s8.u, o.doe(s8.u, r, s8.ph), s8.ph)
     assert rel(dz.get(r), ref) < 1e-2
+
+
+def test_apply_complex_phases():
+    """Phases that are not +-1 (background field, imaginary chemical
+    potential) take the 28 B link records with (cos, sin)."""
+    s = Sys(bf=(1, 0, 0, 2, 0, 0), mu=0.2, charge=-1.0 / 3.0)
+    assert np.any(np.abs(np.sin(s.ph)) > 0.05)
+    o = s.o
+    p = random_params(3)
+    r = o.gen_gauss(23)
+    dr, dz = Dev(r), Dev(nbytes=r.nbytes)
+    p32 = p.astype(np.float32)
+    assert lib.ostp_ptc_apply(p32.ctypes.data, s.duf.p, s.dphf.p, dr.p, dz.p) == 0
+    ost.check(0, "ptc apply")
+    w0, w = unpack(p)
+    assert rel(dz.get(r), o.ptc_apply(s.u, s.ph, w0, w, r)) < 2e-2
+
+
+def test_apply_tiny_residual(s8):
+    """A residual 1e-12 times smaller gives the same N r up to that factor
+    (the channel fields keep FP32's exponent range)."""
+    o = s8.o
+    p = random_params(4).astype(np.float32)
+    r = o.gen_gauss(24)
+    outs = []
+    for scale in (1.0, 1e-12):
+        dr, dz = Dev(r * scale), Dev(nbytes=r.nbytes)
+        assert lib.ostp_ptc_apply(p.ctypes.data, s8.duf.p, s8.dphf.p, dr.p, dz.p) == 0
+        ost.check(0, "ptc apply")
+        outs.append(dz.get(r) / scale)
+    assert rel(outs[1], outs[0]) < 1e-2
 
 
 def oracle_anorm_grad(s, p, b, x):
