@@ -1037,16 +1037,28 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
         return 0;
     }
 
-    // ---- x += out_f; FP64 true residual of every system
-    if (half)
-        hipLaunchKernelGGL(kb_h_flush, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kAlways);
-    else
-        hipLaunchKernelGGL(kb_add_f_to_d, gsite, dim3(kBlock), 0, S(), B.dsys);
-    launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kAlways);
-    launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kAlways);
-    hipLaunchKernelGGL(kb_resid_norm, gsite, dim3(kBlock), 0, S(), B.dsys, 1);
-    fin<kFinStore>(nsys, kAlways, kBlock);
-    if (!hip_ok(hipGetLastError(), nm) || !pull_states(nsys)) return 0;
+    // ---- x += out_f; FP64 true residual of every system.  The FP16 inner
+    // solver ends every system on a reliable update (or at its start, when
+    // that is already within the tolerance), whose FP64 residual is that of
+    // the final x: it is recomputed only if some system stopped elsewhere
+    // (at max_cg inside an inner iteration).
+    bool fresh = false;
+    if (half) {
+        if (!pull_states(nsys)) return 0;
+        fresh = true;
+        for (int k = 0; k < nsys; k++) fresh = fresh && B.st_host[k].resid_ok == 1;
+    }
+    if (!fresh) {
+        if (half)
+            hipLaunchKernelGGL(kb_h_flush, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kAlways);
+        else
+            hipLaunchKernelGGL(kb_add_f_to_d, gsite, dim3(kBlock), 0, S(), B.dsys);
+        launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kAlways);
+        launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kAlways);
+        hipLaunchKernelGGL(kb_resid_norm, gsite, dim3(kBlock), 0, S(), B.dsys, 1);
+        fin<kFinStore>(nsys, kAlways, kBlock);
+        if (!hip_ok(hipGetLastError(), nm) || !pull_states(nsys)) return 0;
+    }
     int converged = 0;
     for (int k = 0; k < nsys; k++) {
         const SolverState& s = B.st_host[k];

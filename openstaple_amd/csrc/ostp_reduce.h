@@ -172,6 +172,8 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
         st->delta = tot_im;
         st->status = kRunning;
     } else if (F == kFinHScale) {
+        // tot: |b - M x|^2 of the FP64 residual
+        st->sum = tot;
         const double n = st->nentries;
         const double sc = tot > 0.0 ? sqrt(n / tot) : 1.0;
         st->rho = st->scale > 0.0 ? sc / st->scale : 1.0;
@@ -180,6 +182,12 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
         st->delta = tot;
         st->rho = 1.0;
         st->status = kRunning;
+        // a start already within the tolerance (a refinement of a converged
+        // solution) runs no iteration; its FP64 residual is the final one
+        if (!(sqrt(st->sum / st->source_norm) > st->res * st->safety)) {
+            st->status = kDone;
+            st->resid_ok = 1;
+        }
     } else if (F == kFinHAlpha) {
         st->alpha = tot;
         st->omega = st->delta / tot;
@@ -207,7 +215,12 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
         st->last_max_res = 0;
         st->ru_count += 1;
         const double true2 = lambda / (st->scale * st->scale);
-        if (!(sqrt(true2 / st->source_norm) > st->res * st->safety) || st->cg >= st->max_cg) st->status = kDone;
+        if (!(sqrt(true2 / st->source_norm) > st->res * st->safety) || st->cg >= st->max_cg) {
+            // x was flushed by this update and no iteration follows: the
+            // FP64 residual of kFinHScale (in sum) is the final one
+            st->status = kDone;
+            st->resid_ok = 1;
+        }
     } else if (F == kFinMxAlpha) {
         st->alpha = tot;
         st->omega = st->delta / tot;

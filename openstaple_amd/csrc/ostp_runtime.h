@@ -30,6 +30,7 @@ struct SolverState {
     int xr_fresh;             // FP16 inner solver: 1 = |r|^2 partials of an x/r update not yet
                               // consumed, 2 = consumed by an iteration that stopped, 0 = none
     int ru_count;
+    int resid_ok;             // FP16 inner solver: sum holds |b - M x|^2 (FP64) of the final x
     // multi-shift CG-M (inverter_multishift_full.c:48-53)
     int order, maxiter, maxiter_next, flag_cur;
     int flags[2][kMaxShifts];

@@ -172,25 +172,24 @@ __global__ __launch_bounds__(kBlock) void k_msw_r(const C* __restrict__ r_old, C
 }
 
 // p = gammag p + r; block 0 also advances the window coefficients of every
-// active shift by this iteration's omega_k, gamma_k, zeta_k
-// With `hist`, r is also stored as the window's residual t and the position
-// is left for k_msw_tick to advance (every block reads t here: the mixed
-// CG-M, whose reliable updates restart windows at positions the host does not
-// see, keeps r in place and the window's copies in hist).
+// active shift by this iteration's omega_k, gamma_k, zeta_k and the window
+// position t.  With `slot`, r is also stored there as the window's residual
+// t (the mixed CG-M keeps r in place; the host knows the window position,
+// since windows start at fixed iterations).
 template <typename C>
 __global__ __launch_bounds__(kBlock) void k_msw_p(C* __restrict__ p, const C* __restrict__ r, long n3, MsWin* w,
-                                                  SolverCtl ctl, C* __restrict__ hist) {
+                                                  SolverCtl ctl, C* __restrict__ slot) {
     if (!pred_pass(ctl)) return;
     const SolverState* st = ctl.st;
     const long i = (long)blockIdx.x * kBlock + threadIdx.x;
-    const int t = w->t;
     if (i < n3) {
         const double g = st->gammag;
         const C rr = r[i], pp = p[i];
         p[i] = mkc<C>((double)pp.x * g + (double)rr.x, (double)pp.y * g + (double)rr.y);
-        if (hist) hist[(long)t * n3 + i] = rr;
+        if (slot) slot[i] = rr;
     }
     if (blockIdx.x != 0) return;
+    const int t = w->t;  // only block 0 reads and advances it
     const int cur = st->flag_cur, order = st->order;
     const int per = t + 2;  // j < t, the new column t, and the scalar pair
     for (int e = threadIdx.x; e < order * per; e += kBlock) {
@@ -209,26 +208,22 @@ __global__ __launch_bounds__(kBlock) void k_msw_p(C* __restrict__ p, const C* __
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0 && !hist) w->t = t + 1;
-}
-
-// the window position after a k_msw_p with hist
-__global__ void k_msw_tick(MsWin* w, SolverCtl ctl) {
-    if (!pred_pass(ctl)) return;
-    if (threadIdx.x == 0) w->t += 1;
+    if (threadIdx.x == 0) w->t = t + 1;
 }
 
 // x_k += a_k p_k + sum_j c_kj r_j ; p_k = B_k p_k + sum_j D_kj r_j for the
 // shifts that moved, kMswGroup shifts per pass over the residuals (eight: three
-// passes for 19 shifts instead of five with four, each shift's sums unchanged)
+// passes for 19 shifts instead of five with four, each shift's sums unchanged).
+// CX: the solutions' type (FP64 also when the directions and residuals C are
+// FP32: the mixed CG-M adds each window straight into its FP64 solutions).
 constexpr int kMswGroup = 8;
-template <typename C>
-__global__ __launch_bounds__(kBlock) void k_msw_flush(C* __restrict__ xs, C* __restrict__ ps,
+template <typename CX, typename C>
+__global__ __launch_bounds__(kBlock) void k_msw_flush(CX* __restrict__ xs, C* __restrict__ ps,
                                                       const C* __restrict__ hist, long n3, int order,
-                                                      const MsWin* __restrict__ w, SolverCtl ctl, int only_at) {
+                                                      const MsWin* __restrict__ w, SolverCtl ctl) {
     if (!pred_pass(ctl)) return;
     const int t = w->t;
-    if (t == 0 || (only_at > 0 && t != only_at)) return;  // only_at: flush a full window only
+    if (t == 0) return;
     const long i = (long)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n3) return;
     for (int k0 = 0; k0 < order; k0 += kMswGroup) {
@@ -243,7 +238,8 @@ __global__ __launch_bounds__(kBlock) void k_msw_flush(C* __restrict__ xs, C* __r
             ax[q] = make_double2(0.0, 0.0);
             ap[q] = make_double2(0.0, 0.0);
             if (on[q]) {
-                const C xv = xs[k * n3 + i], pv = ps[k * n3 + i];
+                const CX xv = xs[k * n3 + i];
+                const C pv = ps[k * n3 + i];
                 const double a = w->a[k], b = w->B[k];
                 ax[q] = make_double2((double)xv.x + a * (double)pv.x, (double)xv.y + a * (double)pv.y);
                 ap[q] = make_double2(b * (double)pv.x, b * (double)pv.y);
@@ -266,18 +262,15 @@ __global__ __launch_bounds__(kBlock) void k_msw_flush(C* __restrict__ xs, C* __r
 #pragma unroll
         for (int q = 0; q < kMswGroup; q++)
             if (on[q]) {
-                xs[(k0 + q) * n3 + i] = mkc<C>(ax[q].x, ax[q].y);
+                xs[(k0 + q) * n3 + i] = mkc<CX>(ax[q].x, ax[q].y);
                 ps[(k0 + q) * n3 + i] = mkc<C>(ap[q].x, ap[q].y);
             }
     }
 }
 
-// a fresh window: coefficients of the identity map (with only_at > 0: only
-// when the window holds only_at residuals)
-__global__ __launch_bounds__(kBlock) void k_msw_reset(MsWin* w, SolverCtl ctl, int only_at) {
+// a fresh window: coefficients of the identity map
+__global__ __launch_bounds__(kBlock) void k_msw_reset(MsWin* w, SolverCtl ctl) {
     if (!pred_pass(ctl)) return;
-    if (only_at > 0 && w->t != only_at) return;
-    __syncthreads();  // every thread has read t before thread 0 clears it
     for (int e = threadIdx.x; e < kMaxShifts * kMsWinMax; e += kBlock) {
         const int k = e / kMsWinMax, j = e - k * kMsWinMax;
         w->c[k][j] = 0.0;
@@ -490,11 +483,31 @@ static bool check_launch(const char* what) { return hip_ok(hipGetLastError(), wh
 
 static int batch_next(int b) { return std::min(2 * b, 64); }
 
-// Queue iterations until the device state leaves kRunning.
+// Distance of a solver from its stopping test in log units of the residual,
+// ln(residual / (res * safety)) (NaN when unknown): CG-type solvers from
+// |r|^2, CG-M from the slowest active shift, |zeta_k|^2 |r|^2.
+static double dist_cg(const SolverState& s) {
+    if (!(s.lambda > 0.0) || !(s.source_norm > 0.0) || !(s.res > 0.0)) return NAN;
+    return 0.5 * std::log(s.lambda / s.source_norm) - std::log(s.res * s.safety);
+}
+static double dist_ms(const SolverState& s) {
+    const int act = s.flag_cur ^ 1;  // the flags the next iteration makes current
+    double m = 0.0;
+    for (int k = 0; k < s.order && k < kMaxShifts; k++)
+        if (s.flags[act][k] == 1) m = std::max(m, s.delta * s.zeta_ii[k] * s.zeta_ii[k]);
+    if (!(m > 0.0) || !(s.source_norm > 0.0) || !(s.res > 0.0)) return NAN;
+    return 0.5 * std::log(m / s.source_norm) - std::log(s.res * s.safety);
+}
+
+// Queue iterations until the device state leaves kRunning.  Batches grow
+// 4, 8, ..., 64; with a distance estimate, a batch is also cut to the
+// iterations the convergence rate since the last batch predicts are left
+// (plus a margin), so that few predicated no-op iterations follow the end.
 template <typename Enq>
-static bool run_batches(Enq enqueue, int first_batch = 4) {
+static bool run_batches(Enq enqueue, int first_batch = 4, double (*dist)(const SolverState&) = nullptr) {
     int batch = first_batch;
     int last_cg = -1;
+    double prev_d = NAN;
     for (;;) {
         if (!pull_state()) return false;
         if (H().status != kRunning) return true;
@@ -502,8 +515,20 @@ static bool run_batches(Enq enqueue, int first_batch = 4) {
         // every queued iteration of a running solver advances the counter on
         // the device; a stalled counter means the scalar epilogue did not run
         if (cg0 == last_cg) return fail("solver state did not advance (iteration %d)", cg0);
+        int n = batch;
+        if (dist) {
+            const double d = dist(H());
+            if (std::isfinite(d) && std::isfinite(prev_d) && last_cg >= 0 && cg0 > last_cg) {
+                const double rate = (prev_d - d) / (double)(cg0 - last_cg);
+                if (rate > 1e-9) {
+                    const double left = d > 0.0 ? d / rate : 0.0;
+                    n = std::min(n, std::max(2, (int)std::ceil(1.1 * left) + 1));
+                }
+            }
+            prev_d = d;
+        }
         last_cg = cg0;
-        for (int j = 1; j <= batch; j++)
+        for (int j = 1; j <= n; j++)
             if (!enqueue(cg0 + j)) return false;
         batch = batch_next(batch);
     }
@@ -559,7 +584,7 @@ int cg_solve(const CgBufs& B, double mass, double shift, double res, int max_cg,
 
     if (!restart()) return INVERTER_FAILURE;
     for (;;) {
-        if (!run_batches(iteration)) return INVERTER_FAILURE;
+        if (!run_batches(iteration, 4, dist_cg)) return INVERTER_FAILURE;
         const SolverState& s = H();
         if (s.status == kInnerExit && std::sqrt(s.lambda / s.source_norm) > res && s.cg < max_cg) {
             if (!restart()) return INVERTER_FAILURE;
@@ -653,7 +678,7 @@ int mixed_solve(const MxBufs& B, double mass, double shift, double res, int max_
                            ctl_of(kIfIter, j));
         return check_launch("mixed iteration");
     };
-    if (!run_batches(iteration)) return INVERTER_FAILURE;
+    if (!run_batches(iteration, 4, dist_cg)) return INVERTER_FAILURE;
     const int cg = H().cg;
     if (ru_count) *ru_count = H().ru_count;
     hipLaunchKernelGGL(k_add_f_to_d, dim3(nb3), dim3(kBlock), 0, S(), (double2*)B.x, (const float2*)B.outf, n3);
@@ -760,7 +785,7 @@ int multishift_solve(const MsBufs& B, double mass, int order, const double* shif
         }
         hist = (char*)g_msw_hist;
         if (!g_msw && !hip_ok(hipMalloc(&g_msw, sizeof(MsWin)), "multishift window")) return INVERTER_FAILURE;
-        hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kAlways), 0);
+        hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kAlways));
     }
     auto slot = [&](int j) { return (C*)(hist + (size_t)((j - 1) % W) * vb); };  // residual of iteration j (1-based)
     auto iteration = [&](int j) -> bool {
@@ -782,19 +807,19 @@ int multishift_solve(const MsBufs& B, double mass, int order, const double* shif
                            ctl_of(kIfIter, j), (C*)nullptr);
         if (j % W == 0) {
             // only when iteration j ran: a window cut short is flushed once, at the end
-            hipLaunchKernelGGL((k_msw_flush<C>), dim3(nb3), dim3(kBlock), 0, S(), (C*)B.out, (C*)B.shiftferm,
-                               (const C*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kIfIter, j), 0);
-            hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kIfIter, j), 0);
+            hipLaunchKernelGGL((k_msw_flush<C, C>), dim3(nb3), dim3(kBlock), 0, S(), (C*)B.out, (C*)B.shiftferm,
+                               (const C*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kIfIter, j));
+            hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kIfIter, j));
         }
         return check_launch("multishift iteration");
     };
-    if (!run_batches(iteration)) return INVERTER_FAILURE;
+    if (!run_batches(iteration, 4, dist_ms)) return INVERTER_FAILURE;
     const int cg = H().cg;
     if (hist) {
         // the partial last window, and the final residual where the per-iteration
         // version leaves it
-        hipLaunchKernelGGL((k_msw_flush<C>), dim3(nb3), dim3(kBlock), 0, S(), (C*)B.out, (C*)B.shiftferm,
-                           (const C*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kAlways), 0);
+        hipLaunchKernelGGL((k_msw_flush<C, C>), dim3(nb3), dim3(kBlock), 0, S(), (C*)B.out, (C*)B.shiftferm,
+                           (const C*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kAlways));
         if (cg > 0 && !hip_ok(hipMemcpyAsync(B.r, slot(cg), vb, hipMemcpyDeviceToDevice, S()), "final residual"))
             return INVERTER_FAILURE;
         if (!check_launch("multishift window flush")) return INVERTER_FAILURE;
@@ -1273,7 +1298,12 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
     const int nbh = blocks_for(sizeh), nb3 = blocks_for(n3);
     const size_t vb = L.vec3_bytes(), vbf = L.vec3f_bytes();
     void* bf = grow_buffer(g_aux1_f, g_aux1_bytes, vbf);
-    void* xsf = grow_buffer(g_shiftmulti_f, g_shiftmulti_bytes, (size_t)order * vbf);
+    // shift updates once per window of W iterations (as multishift_solve),
+    // added straight into the FP64 solutions; without the window memory, FP32
+    // accumulators updated every iteration and moved into the FP64 solutions
+    // at each reliable update
+    const int W = msw_window(vbf);
+    void* xsf = W >= 2 ? (void*)nullptr : grow_buffer(g_shiftmulti_f, g_shiftmulti_bytes, (size_t)order * vbf);
     const void* db = resolve(in, vb, nm);
     void* dout = resolve(out, (size_t)order * vb, nm);
     const void* u = resolve(ip.u, 8 * L.su3_bytes(), nm);
@@ -1290,15 +1320,15 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
     void* h64 = resolve(ip.loc_h, vb, nm);
     void* s64 = resolve(ip.loc_s, vb, nm);
     void* r64 = resolve(ip.loc_r, vb, nm);
-    if (!bf || !xsf || !db || !dout || !u || !uf || !ph || !phf || !pk || !r || !h || !sv || !p || !x0f || !x0 ||
-        !h64 || !s64 || !r64)
+    if (!bf || (W < 2 && !xsf) || !db || !dout || !u || !uf || !ph || !phf || !pk || !r || !h || !sv || !p || !x0f ||
+        !x0 || !h64 || !s64 || !r64)
         return 0;
     const double m2 = pars->ferm_mass * pars->ferm_mass;
 
     // x_k = x_k_f = x0 = x0_f = 0 ; r = p = p_k = (float) b
     if (!launch_d2f(db, bf, n3)) return 0;
     bool ok = hip_ok(hipMemsetAsync(dout, 0, (size_t)order * vb, S()), "zero x_k") &&
-              hip_ok(hipMemsetAsync(xsf, 0, (size_t)order * vbf, S()), "zero x_k_f") &&
+              (!xsf || hip_ok(hipMemsetAsync(xsf, 0, (size_t)order * vbf, S()), "zero x_k_f")) &&
               hip_ok(hipMemsetAsync(x0, 0, vb, S()), "zero x0") && hip_ok(hipMemsetAsync(x0f, 0, vbf, S()), "zero x0_f") &&
               hip_ok(hipMemcpyAsync(r, bf, vbf, hipMemcpyDeviceToDevice, S()), "r = b") &&
               hip_ok(hipMemcpyAsync(p, bf, vbf, hipMemcpyDeviceToDevice, S()), "p = b");
@@ -1326,17 +1356,17 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
     s0.maxiter_next = order;
     if (!push_state()) return 0;
 
+    // the FP32 accumulators a reliable update moves into FP64: the base
+    // system's (whose solution the FP64 residual needs), and without the
+    // window the shifts' too
     MsxFlush fl;
-    fl.n = order + 1;
+    fl.n = xsf ? order + 1 : 1;
     fl.x64[0] = (double2*)x0;
     fl.x32[0] = (float2*)x0f;
-    for (int k = 0; k < order; k++) {
+    for (int k = 0; xsf && k < order; k++) {
         fl.x64[k + 1] = (double2*)((char*)dout + k * vb);
         fl.x32[k + 1] = (float2*)((char*)xsf + k * vbf);
     }
-    // shift updates once per window of W iterations (as multishift_solve),
-    // the window flushed early by every reliable update
-    const int W = msw_window(vbf);
     float2* hist = nullptr;
     if (W >= 2) {
         if (g_msw_hist_bytes < (size_t)W * vbf) {
@@ -1348,43 +1378,49 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
         }
         hist = (float2*)g_msw_hist;
         if (!g_msw && !hip_ok(hipMalloc(&g_msw, sizeof(MsWin)), "multishift window")) return 0;
-        hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kAlways), 0);
+        hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kAlways));
     }
+    // Reliable updates are taken at every kMsxRuEvery-th iteration (when the
+    // FP32 residual has fallen by mixed_delta since the last one): the other
+    // iterations run the plain CG-M finalizers and launch nothing for the
+    // update.  The window continues across an update: its residuals are the
+    // ones the recurrence used, replaced or not.
+    constexpr int kMsxRuEvery = 8;
     auto iteration = [&](int j) -> bool {
-        if (!launch_mdagm<float>(uf, phf, p, h, sv, m2, 2, kFinMxmAlpha, ctl_of(kIfRunning))) return false;
+        const bool ru_slot = j % kMsxRuEvery == 0;
+        if (!launch_mdagm<float>(uf, phf, p, h, sv, m2, 2, ru_slot ? kFinMxmAlpha : kFinMsAlpha, ctl_of(kIfRunning)))
+            return false;
         hipLaunchKernelGGL(k_msx_r, dim3(nbh), dim3(kBlock), 0, S(), (float2*)r, (const float2*)sv,
                            (const float2*)p, (float2*)x0f, (int)sizeh, ctl_of(kIfRunning));
-        // reliable update: accumulators into the FP64 solutions, r = b - M x0 in FP64
-        if (hist) {
-            // the shifts' pending window first, so that their accumulators are current
-            hipLaunchKernelGGL((k_msw_flush<float2>), dim3(nb3), dim3(kBlock), 0, S(), (float2*)xsf, (float2*)pk,
-                               (const float2*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kIfRunningRU), 0);
-            hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kIfRunningRU), 0);
+        if (ru_slot) {
+            // reliable update: accumulators into the FP64 solutions, r = b - M x0 in FP64
+            hipLaunchKernelGGL(k_msx_flush, dim3(nb3), dim3(kBlock), 0, S(), fl, n3, ctl_of(kIfRunningRU));
+            if (!launch_mdagm<double>(u, ph, x0, h64, s64, m2, 1, kFinStore, ctl_of(kIfRunningRU))) return false;
+            hipLaunchKernelGGL(k_msx_resid, dim3(nbh), dim3(kBlock), 0, S(), (const double2*)db,
+                               (const double2*)s64, (float2*)r, (int)sizeh, ctl_of(kIfRunningRU));
         }
-        hipLaunchKernelGGL(k_msx_flush, dim3(nb3), dim3(kBlock), 0, S(), fl, n3, ctl_of(kIfRunningRU));
-        if (!launch_mdagm<double>(u, ph, x0, h64, s64, m2, 1, kFinStore, ctl_of(kIfRunningRU))) return false;
-        hipLaunchKernelGGL(k_msx_resid, dim3(nbh), dim3(kBlock), 0, S(), (const double2*)db, (const double2*)s64,
-                           (float2*)r, (int)sizeh, ctl_of(kIfRunningRU));
-        if (!launch_finalize(kFinMxmLambda, ctl_of(kIfRunning), nbh)) return false;
+        if (!launch_finalize(ru_slot ? kFinMxmLambda : kFinMsLambda, ctl_of(kIfRunning), nbh)) return false;
         if (!hist) {
             hipLaunchKernelGGL((k_ms_update<float2>), dim3(nb3), dim3(kBlock), 0, S(), (const float2*)r, (float2*)p,
                                (float2*)xsf, (float2*)pk, n3, n3, ctl_of(kIfIter, j));
             return check_launch("mixed multishift iteration");
         }
         hipLaunchKernelGGL((k_msw_p<float2>), dim3(nb3), dim3(kBlock), 0, S(), (float2*)p, (const float2*)r, n3,
-                           g_msw, ctl_of(kIfIter, j), hist);
-        hipLaunchKernelGGL(k_msw_tick, dim3(1), dim3(64), 0, S(), g_msw, ctl_of(kIfIter, j));
-        hipLaunchKernelGGL((k_msw_flush<float2>), dim3(nb3), dim3(kBlock), 0, S(), (float2*)xsf, (float2*)pk,
-                           (const float2*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kIfIter, j), W);
-        hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kIfIter, j), W);
+                           g_msw, ctl_of(kIfIter, j), hist + (size_t)((j - 1) % W) * n3);
+        if (j % W == 0) {
+            // only when iteration j ran: a window cut short is flushed once, at the end
+            hipLaunchKernelGGL((k_msw_flush<double2, float2>), dim3(nb3), dim3(kBlock), 0, S(), (double2*)dout,
+                               (float2*)pk, (const float2*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kIfIter, j));
+            hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_msw, ctl_of(kIfIter, j));
+        }
         return check_launch("mixed multishift iteration");
     };
-    if (!run_batches(iteration)) return 0;
+    if (!run_batches(iteration, 4, dist_ms)) return 0;
     const int cg = H().cg;
     if (cg_multishift) *cg_multishift = cg;
     if (hist)
-        hipLaunchKernelGGL((k_msw_flush<float2>), dim3(nb3), dim3(kBlock), 0, S(), (float2*)xsf, (float2*)pk,
-                           (const float2*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kAlways), 0);
+        hipLaunchKernelGGL((k_msw_flush<double2, float2>), dim3(nb3), dim3(kBlock), 0, S(), (double2*)dout,
+                           (float2*)pk, (const float2*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kAlways));
     hipLaunchKernelGGL(k_msx_flush, dim3(nb3), dim3(kBlock), 0, S(), fl, n3, ctl_of(kAlways));
     if (!check_launch("flush")) return 0;
 
