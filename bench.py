@@ -40,6 +40,11 @@ BYTES_PER_SITE_F32 = 464
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(msg, file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +66,12 @@ def parse():
     ap.add_argument("--rhmc-steps", type=int, default=10, help="Omelyan steps per trajectory")
     ap.add_argument("--rhmc-thermalise", type=int, default=12,
                     help="gauge-only HMC trajectories before the 2+1-flavour ones")
+    ap.add_argument("--rhmc-tau", type=float, default=1.0, help="trajectory length")
+    ap.add_argument("--rhmc-therm-tau", type=float, default=0.25,
+                    help="trajectory length of the gauge-only thermalisation")
+    ap.add_argument("--rhmc-warmup", type=int, default=1, help="untimed 2+1-flavour trajectories before the timed one")
+    ap.add_argument("--rhmc-no-mixed", action="store_true",
+                    help="skip the second timed trajectory with reduced-precision force solves")
     ap.add_argument("--dslash-only", action="store_true",
                     help="only the timed Dslash loop and the copy kernel (for counter runs)")
     return ap.parse_args()
@@ -595,11 +606,11 @@ def run_rhmc(ost, lib, args):
     Nf = 2+1 staggered with m_l = 0.0015 (degeneracy 2) and m_s = 0.04, one
     pseudofermion each, order-19 approximations (x^(1/4), x^(-1/2) on
     [1e-7, 1] for the light quark, x^(1/8), x^(-1/4) on [4e-7, 1] for the
-    strange), args.rhmc_steps Omelyan steps of trajectory length 1 with 5 gauge
+    strange), args.rhmc_steps Omelyan steps of trajectory length args.rhmc_tau with 5 gauge
     sub-steps, FP64 multishift solves (1e-8 in the force, 1e-10 for the heat
-    bath and the acceptance), links from the generate_Conf_cold rule with
-    eps = 0.5 (seed 7), seeded momenta and pseudofermions.  Also the device
-    time of the gauge-force kernel on the same links."""
+    bath and the acceptance), links thermalised by gauge-only trajectories
+    from the generate_Conf_cold rule with eps = 0.5 (seed 7), seeded momenta
+    and pseudofermions.  Also the device time of the gauge-force kernel."""
     golden = ROOT / "tests" / "golden"
 
     def dm(n):
@@ -640,7 +651,7 @@ def run_rhmc(ost, lib, args):
     ai = ost.AllocSettings.in_dll(lib, "alloc_info")
     ai.NDiffFlavs, ai.NPS_tot, ai.maxApproxOrder, ai.maxNeededShifts = 2, 2, 19, 19
     ost.ActionParam.in_dll(lib, "act_params").beta = 3.8
-    mdp = ost.MdParam(no_md=args.rhmc_steps, gauge_scale=5, t=1.0)
+    mdp = ost.MdParam(no_md=args.rhmc_steps, gauge_scale=5, t=args.rhmc_tau)
     C.memmove(C.addressof(ost.MdParam.in_dll(lib, "md_parameters")), C.addressof(mdp),
                    C.sizeof(mdp))
     lib.initialize_md_global_variables(mdp)
@@ -653,8 +664,9 @@ def run_rhmc(ost, lib, args):
     flops = GAUGE_FORCE_FLOPS_PER_LINK * 8 * sizeh
     tflops = flops / (us_force * 1e-6) / 1e12
 
-    # thermalisation: gauge-only HMC trajectories (the same update with no
-    # fermion flavour), then one full 2+1-flavour trajectory, untimed
+    # thermalisation: gauge-only trajectories (the same update with no
+    # fermion flavour) of length --rhmc-therm-tau, then --rhmc-warmup full
+    # 2+1-flavour trajectories, untimed
     def plaquette():
         return lib.calc_plaquette_soloopenacc(conf, None, None) / (6.0 * 2 * sizeh * 3.0)
 
@@ -663,18 +675,38 @@ def run_rhmc(ost, lib, args):
     ai.NDiffFlavs = 0
     t0 = time.perf_counter()
     acc_g = 0
+    therm_plaq = []
+
+    def set_tau(tau):
+        m = ost.MdParam(no_md=args.rhmc_steps, gauge_scale=5, t=tau)
+        C.memmove(C.addressof(ost.MdParam.in_dll(lib, "md_parameters")), C.addressof(m), C.sizeof(m))
+        lib.initialize_md_global_variables(m)
+
+    set_tau(args.rhmc_therm_tau)
+    n_metro = args.rhmc_thermalise // 2
     for it in range(args.rhmc_thermalise):
-        # molecular dynamics with fresh momenta and no accept/reject: from the
-        # far-from-equilibrium start every Metropolis test would reject
-        acc_g = lib.UPDATE_SOLOACC_UNOSTEP_VERSATILE(conf, 1e-10, 1e-8, it, acc_g, 0, 50000)
+        # the first half: molecular dynamics with fresh momenta and no
+        # accept/reject (from the far-from-equilibrium start every Metropolis
+        # test would reject); the second half: HMC with the Metropolis test
+        metro = 1 if it >= args.rhmc_thermalise - n_metro else 0
+        acc_g = lib.UPDATE_SOLOACC_UNOSTEP_VERSATILE(conf, 1e-10, 1e-8, it, acc_g if metro else 0, metro, 50000)
+        therm_plaq.append(round(plaquette(), 5))
     ost.check(0, "gauge-only thermalisation")
+    set_tau(args.rhmc_tau)
     t_therm = time.perf_counter() - t0
+    progress(f"rhmc: {args.rhmc_thermalise} gauge-only trajectories in {t_therm:.1f} s, plaquette {plaquette():.5f}")
     plaq.append(plaquette())
     ai.NDiffFlavs = 2
     warm = ost.RhmcInfo()
-    acc = lib.UPDATE_SOLOACC_UNOSTEP_VERSATILE(conf, 1e-10, 1e-8, 0, 0, 1, 50000)
-    ost.check(0, "UPDATE_SOLOACC_UNOSTEP_VERSATILE (warm-up)")
-    lib.ostp_rhmc_last_info(C.byref(warm))
+    warm_info = []
+    for it in range(args.rhmc_warmup):
+        t0 = time.perf_counter()
+        lib.UPDATE_SOLOACC_UNOSTEP_VERSATILE(conf, 1e-10, 1e-8, it, 0, 1, 50000)
+        ost.check(0, "UPDATE_SOLOACC_UNOSTEP_VERSATILE (warm-up)")
+        lib.ostp_rhmc_last_info(C.byref(warm))
+        warm_info.append({"seconds": round(time.perf_counter() - t0, 2), "delta_S": warm.delta_S,
+                          "accepted": warm.accepted, "md_multishift_iterations": warm.md_multishift_iterations})
+        progress(f"rhmc: warm-up trajectory {it}: {warm_info[-1]}")
     plaq.append(plaquette())
     # the timed trajectory twice from the same links, momenta and pseudofermions:
     # FP64 rational solves, then the reduced-precision rational solve (FP32
@@ -683,9 +715,9 @@ def run_rhmc(ost, lib, args):
     start = dm(gb)
     ost.check(lib.ostp_memcpy_dtod(start, conf, gb), "save links")
     runs = {}
-    for mode, name in ((0, "fp64"), (1, "mixed")):
+    for mode, name in ((0, "fp64"), (1, "mixed"))[:1 if args.rhmc_no_mixed else 2]:
         ost.check(lib.ostp_memcpy_dtod(conf, start, gb), "restore links")
-        prev_mode = lib.ostp_set_multishift_precision(mode, 1)
+        prev_mode = lib.ostp_set_multishift_precision(2 * mode, 1)
         lib.ostp_set_rng_seed(4242, 0)
         t0 = time.perf_counter()
         acc = lib.UPDATE_SOLOACC_UNOSTEP_VERSATILE(conf, 1e-10, 1e-8, 1, 0, 1, 50000)
@@ -695,6 +727,7 @@ def run_rhmc(ost, lib, args):
         inf = ost.RhmcInfo()
         lib.ostp_rhmc_last_info(C.byref(inf))
         runs[name] = (t1 - t0, acc, inf)
+        progress(f"rhmc: timed trajectory ({name} force solves) {t1 - t0:.2f} s, dS {inf.delta_S:.4g}")
     t0, t1 = 0.0, runs["fp64"][0]
     acc, info = runs["fp64"][1], runs["fp64"][2]
     plaq.append(plaquette())
@@ -704,38 +737,45 @@ def run_rhmc(ost, lib, args):
     C.c_void_p.in_dll(lib, "fermions_parameters").value = None
     for p in list(bufs.values()) + [conf, ph]:
         lib.ostp_free(p)
-    return {
+    out = {
         "workload": "RHMC trajectory %s, TLSM beta=3.8, 2 stout rho=0.15, Nf=2+1 m_l=0.0015 m_s=0.04, order 19, "
-                    "%d Omelyan steps x 5 gauge sub-steps, tau=1, FP64 multishift (force 1e-8, "
-                    "accept 1e-10)" % ("x".join(map(str, shape)), args.rhmc_steps),
+                    "%d Omelyan steps x 5 gauge sub-steps, tau=%g, FP64 multishift (force 1e-8, "
+                    "accept 1e-10)" % ("x".join(map(str, shape)), args.rhmc_steps, args.rhmc_tau),
         "s_per_trajectory": round(t1 - t0, 3),
         "accepted": int(acc),
         "delta_S": info.delta_S,
-        "start": (f"generate_Conf_cold eps=0.5, then {args.rhmc_thermalise} gauge-only MD trajectories "
-                  f"(fresh momenta, no Metropolis test; {t_therm:.1f} s) and one 2+1-flavour RHMC trajectory "
-                  f"(dS {warm.delta_S:.4g}, accepted {warm.accepted}) before the timed one"),
+        "actions": {"gauge": [info.action_in, info.action_fin], "momenta": [info.action_mom_in, info.action_mom_fin],
+                    "fermions": [info.action_ferm_in, info.action_ferm_fin]},
+        "start": (f"generate_Conf_cold eps=0.5, then {args.rhmc_thermalise} gauge-only trajectories of length "
+                  f"{args.rhmc_therm_tau:g} ({args.rhmc_thermalise - n_metro} without and {n_metro} with the "
+                  f"Metropolis test, {acc_g} of those accepted; {t_therm:.1f} s) and {args.rhmc_warmup} "
+                  f"2+1-flavour RHMC trajectories before the timed one"),
+        "thermalisation_plaquettes": therm_plaq,
+        "warmup": warm_info,
         "plaquette": {"cold": round(plaq[0], 6), "after_gauge_hmc": round(plaq[1], 6),
                       "after_warmup": round(plaq[2], 6), "after_timed": round(plaq[3], 6)},
         "md_multishift_iterations": info.md_multishift_iterations,
-        "mixed_precision_solves": {
+    }
+    if "mixed" in runs:
+        out["mixed_precision_solves"] = {
             "s_per_trajectory": round(runs["mixed"][0], 3),
             "speedup_vs_fp64": round(runs["fp64"][0] / runs["mixed"][0], 3),
             "accepted": int(runs["mixed"][1]),
             "delta_S": runs["mixed"][2].delta_S,
             "md_multishift_iterations": runs["mixed"][2].md_multishift_iterations,
-            "solver": "MD force (1e-8): FP32 CG-M with FP64 reliable updates + batched FP32-inner refinement "
+            "solver": "MD force (1e-8): FP32 CG-M with FP64 reliable updates + batched FP16-inner refinement "
                       "of the shifts above the tolerance; heat bath and acceptance (1e-10) FP64 CG-M",
-        },
-        "gauge_force": {
-            "kernel": "k_gauge_force (tree-level Symanzik staples + TA(U S))",
-            "us_per_launch": round(us_force, 1),
-            "us_per_launch_with_momentum_step": round(us_force_mom, 1),
-            "flops_per_link": GAUGE_FORCE_FLOPS_PER_LINK,
-            "achieved_TFLOPs": round(tflops, 2),
-            "peak_TFLOPs": FP64_PEAK_TFLOPS,
-            "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
-        },
+        }
+    out["gauge_force"] = {
+        "kernel": "k_gauge_force (tree-level Symanzik staples + TA(U S))",
+        "us_per_launch": round(us_force, 1),
+        "us_per_launch_with_momentum_step": round(us_force_mom, 1),
+        "flops_per_link": GAUGE_FORCE_FLOPS_PER_LINK,
+        "achieved_TFLOPs": round(tflops, 2),
+        "peak_TFLOPs": FP64_PEAK_TFLOPS,
+        "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
     }
+    return out
 
 
 def bench_shifts(ost):

@@ -560,6 +560,12 @@ double calc_rettangolo_soloopenacc(const su3_soa *tconf_acc, su3_soa *local_plaq
  * mom_sum_mult(mom, ipdot, fact) when mom is not null), averaged over reps */
 double ostp_time_gauge_force(const su3_soa *u, su3_soa *stap, tamat_soa *ipdot, thmat_soa *mom, double fact,
                              int reps);
+/* No reference counterpart: which kernels form the gauge force when the
+ * staples are not asked for (the MD's sub-steps, ostp_time_gauge_force with
+ * stap null): 1 (default) one launch per plane with the plane's links in LDS,
+ * where every extent is a multiple of 4; 0 the per-link kernel.  Returns the
+ * previous setting. */
+int ostp_set_gauge_force_kernel(int planes);
 
 /* ===================== fermion measurements (Meas/ferm_meas.c) ========== */
 /* Z2-noise chiral condensate, ferm_meas.c:282-320 for nvec noise vectors:

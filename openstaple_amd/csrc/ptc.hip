@@ -157,7 +157,10 @@ __device__ __forceinline__ void transports(const PtcGeom& A, const double2* in, 
 
 // FP16 link records of the preconditioner from the FP32 links and phases
 // (sgn: every phase is +-1, carried in a sign bit; flag: set when some phase
-// is not, or some row 0 entry 0 exceeds 1 in magnitude)
+// is not, or some row 0 entry 0 exceeds 1 in magnitude).  A phase counts as
+// +-1 when its cosine rounds to +-1 in FP16 and |sin| <= 2^-20: the FP32
+// argument pi has sin = -8.7e-8, which the sign form drops (far below the
+// FP16 resolution of the links, 2^-11).
 struct PtcLinkSrc {
     const float2* u[8];
     const float* ph[8];
@@ -171,10 +174,9 @@ __global__ __launch_bounds__(kBlock) void k_ptc_links_check(PtcLinkSrc a, int* f
     const int l = (int)(gid / a.sizeh), i = (int)(gid - (long)l * a.sizeh);
     float sn, cs;
     sincos_t(a.ph[l][i], &sn, &cs);
-    const h2 p = f2h(cs, sn);
-    const float c = (float)p.x, sv = (float)p.y;
+    const float c = (float)(_Float16)cs;
     const float e = (float)(_Float16)a.u[l][i].x;
-    if (!(sv == 0.f && (c == 1.f || c == -1.f)) || !(fabsf(e) <= 1.f)) *flag = 1;
+    if (!(fabsf(sn) <= 0x1p-20f && (c == 1.f || c == -1.f)) || !(fabsf(e) <= 1.f)) *flag = 1;
 }
 __global__ __launch_bounds__(kBlock) void k_ptc_links(PtcLinkSrc a, int sgn) {
     const long gid = (long)blockIdx.x * kBlock + threadIdx.x;

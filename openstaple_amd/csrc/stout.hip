@@ -267,6 +267,45 @@ __device__ __forceinline__ void pc_link(const PlaneCtx& c, int which, int a, int
     rebuild3row(m);
 }
 
+// The plane kernel's link source: rows 0,1 and K of the links of a tile of
+// the plane (mu, nu) with a two-site halo, in LDS.  Region site (i, j) is
+// tile site (i - 2, j - 2) (i along mu, j along nu); slot d ns / 2 + j rw + i
+// holds its link along mu (d = 0) or nu (d = 1), so that lanes on
+// consecutive sites read consecutive 16-B entries.  A staple term of a link
+// along mu takes offsets a (along mu) and b (along nu) as in PlaneCtx; for a
+// link along nu (own = 1) the roles of the two directions swap.
+struct LdsTile {
+    const double2* e;  // 6 arrays of ns entries: row r, column c at (3 r + c) ns
+    const double* k;   // ns entries
+    int ns, rw;
+    int i0, j0;        // region coordinates of the link's site
+    int own;
+};
+__device__ __forceinline__ int lt_slot(const LdsTile& c, int which, int a, int b) {
+    const int di = c.own ? b : a, dj = c.own ? a : b;
+    return (c.own ? 1 - which : which) * (c.ns >> 1) + (c.j0 + dj) * c.rw + (c.i0 + di);
+}
+
+// rows 0,1 (src_link2) or all rows (src_link) of a staple term's link and its K
+__device__ __forceinline__ void src_link2(const PlaneCtx& c, int which, int a, int b, int pt, Su3& m, double& k) {
+    pc_link2(c, which, a, b, pt, m, k);
+}
+__device__ __forceinline__ void src_link(const PlaneCtx& c, int which, int a, int b, int pt, Su3& m, double& k) {
+    pc_link(c, which, a, b, pt, m, k);
+}
+__device__ __forceinline__ void src_link2(const LdsTile& c, int which, int a, int b, int pt, Su3& m, double& k) {
+    const int s = lt_slot(c, which, a, b);
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int col = 0; col < 3; col++) m.c[r][col] = c.e[(3 * r + col) * c.ns + s];
+    k = pt ? c.k[s] : 1.0;
+}
+__device__ __forceinline__ void src_link(const LdsTile& c, int which, int a, int b, int pt, Su3& m, double& k) {
+    src_link2(c, which, a, b, pt, m, k);
+    rebuild3row(m);
+}
+
 // Each term is a chain of SU(3) products that carries only rows 0,1 and
 // rebuilds row 2 at the end.  MMMPP is formed as its conjugate
 // m5^+ m4^+ m3 m2 m1, a chain of the same kind, and added transposed.
@@ -334,20 +373,20 @@ struct HopD {
     int which, a, b, dag;
 };
 // T = L0 L1 ... L(N-1) with all rows; *k the product of the links' K
-template <int N>
-__device__ __forceinline__ void chain(const PlaneCtx& pc, const HopD (&h)[N], int pt, Su3& T, double& k) {
+template <int N, class Src>
+__device__ __forceinline__ void chain(const Src& pc, const HopD (&h)[N], int pt, Su3& T, double& k) {
     Su3 M;
     double kj;
     if (h[0].dag) {
-        pc_link(pc, h[0].which, h[0].a, h[0].b, pt, M, kj);
+        src_link(pc, h[0].which, h[0].a, h[0].b, pt, M, kj);
         dag_r2(M, T);
     } else {
-        pc_link2(pc, h[0].which, h[0].a, h[0].b, pt, T, kj);
+        src_link2(pc, h[0].which, h[0].a, h[0].b, pt, T, kj);
     }
     k = kj;
 #pragma unroll
     for (int j = 1; j < N; j++) {
-        pc_link(pc, h[j].which, h[j].a, h[j].b, pt, M, kj);
+        src_link(pc, h[j].which, h[j].a, h[j].b, pt, M, kj);
         if (h[j].dag) mul_dag_r2(T, M, T); else mul_r2(T, M, T);
         k *= kj;
     }
@@ -401,52 +440,57 @@ __device__ __constant__ HopD c_side[2][12] = {
      {0, -1, -1, 1}, {1, -1, -1, 0}, {0, -1, 0, 0}},
 };
 
+// one side (c_side[side]) of the plane's staple terms; add(s, T) receives
+// each product to be summed with its coefficient
+template <class Src, class Add>
+__device__ __forceinline__ void plane_side(const Src& pc, const HopD* d, int pt, int tlsm, double c0, Add add) {
+    Su3 G, T;
+    double k;
+    su3_zero(G);
+    {
+        // a m: into G with c0, and with Q4 into S
+        Su3 am;
+        double k_am;
+        const HopD h[2] = {d[0], d[1]};
+        chain(pc, h, pt, am, k_am);
+        axpy3(c0 * k_am, am, G);
+        if (tlsm) {
+            const HopD h4[3] = {d[9], d[10], d[11]};
+            chain(pc, h4, pt, T, k);
+            Su3 X;
+            mul_r2(am, T, X);
+            rebuild3row(X);
+            add(kCOne * k_am * k, X);
+        }
+    }
+    if (tlsm) {
+        const HopD h0[4] = {d[0], d[3], d[4], d[5]};
+        chain(pc, h0, pt, T, k);
+        axpy3(kCOne * k, T, G);
+        const HopD h2[4] = {d[6], d[7], d[8], d[1]};
+        chain(pc, h2, pt, T, k);
+        axpy3(kCOne * k, T, G);
+    }
+    // (...) n
+    Su3 n;
+    double kn;
+    src_link(pc, d[2].which, d[2].a, d[2].b, pt, T, kn);
+    if (d[2].dag) {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) n.c[r][c] = cconj(T.c[c][r]);
+    } else {
+        n = T;
+    }
+    su3_mul(G, n, T);
+    add(kn, T);
+}
+
 __device__ __forceinline__ void plane_staple(const PlaneCtx& pc, int pt, int tlsm, double c0, double* sl) {
 #pragma unroll 1
-    for (int side = 0; side < 2; side++) {
-        const HopD* d = c_side[side];
-        Su3 G, T;
-        double k;
-        su3_zero(G);
-        {
-            // a m: into G with c0, and with Q4 into S
-            Su3 am;
-            double k_am;
-            const HopD h[2] = {d[0], d[1]};
-            chain(pc, h, pt, am, k_am);
-            axpy3(c0 * k_am, am, G);
-            if (tlsm) {
-                const HopD h4[3] = {d[9], d[10], d[11]};
-                chain(pc, h4, pt, T, k);
-                Su3 X;
-                mul_r2(am, T, X);
-                rebuild3row(X);
-                lds_axpy(kCOne * k_am * k, X, sl);
-            }
-        }
-        if (tlsm) {
-            const HopD h0[4] = {d[0], d[3], d[4], d[5]};
-            chain(pc, h0, pt, T, k);
-            axpy3(kCOne * k, T, G);
-            const HopD h2[4] = {d[6], d[7], d[8], d[1]};
-            chain(pc, h2, pt, T, k);
-            axpy3(kCOne * k, T, G);
-        }
-        // (...) n
-        Su3 n;
-        double kn;
-        pc_link(pc, d[2].which, d[2].a, d[2].b, pt, T, kn);
-        if (d[2].dag) {
-#pragma unroll
-            for (int r = 0; r < 3; r++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) n.c[r][c] = cconj(T.c[c][r]);
-        } else {
-            n = T;
-        }
-        su3_mul(G, n, T);
-        lds_axpy(kn, T, sl);
-    }
+    for (int side = 0; side < 2; side++)
+        plane_side(pc, c_side[side], pt, tlsm, c0, [&](double s, const Su3& T) { lds_axpy(s, T, sl); });
 }
 
 // Gauge force, one thread per (link, plane): a block is three waves over the
@@ -556,6 +600,152 @@ __global__ __launch_bounds__(3 * kGfLinks, 2) void k_gauge_force(GaugePtrs G, do
         o.ic11 = t[7];
         ta_store(ipdot, dir, g.sizeh, idxh, o);
         if (mom) mom_step(mom, dir, g.sizeh, idxh, o, fact);
+    }
+}
+
+// Gauge force by planes: one launch per plane (mu, nu), mu < nu, in the order
+// (0,1) (0,2) (0,3) (1,2) (1,3) (2,3).  A block takes a TM x TN tile of the
+// plane at fixed coordinates along the other two directions, loads rows 0,1
+// and K of the tile's links along mu and nu with a two-site halo into LDS
+// (every staple term of the plane stays within it), and forms the plane's
+// terms for the tile's links along mu and along nu, one side of the plane
+// per thread, with every link operand read from LDS (the per-link kernel
+// above fetches 84 links per link from L2).  TA(U S) is linear, so a plane
+// adds K(x, d) TA(U S_plane) to ipdot (the first plane of a direction
+// stores it) and the last plane of a direction applies the momentum step
+// with the sum.  Blocks run over the tiles of one (other-coordinate) slice
+// fastest, so that an XCD's consecutive tiles share their halos in its L2.
+__device__ __forceinline__ int sel4(int k, int v0, int v1, int v2, int v3) {
+    return k == 0 ? v0 : k == 1 ? v1 : k == 2 ? v2 : v3;
+}
+template <int TM, int TN>
+__global__ __launch_bounds__(4 * TM * TN, 2) void k_gf_plane(GaugePtrs G, double2* __restrict__ stap,
+                                                             size_t su3_stride, void* __restrict__ ipdot,
+                                                             void* __restrict__ mom, LinkGeom g, int mu, int nu,
+                                                             int pt, int tlsm, double c0, double fact, int flags) {
+    constexpr int RW = TM + 4, RH = TN + 4, NS = 2 * RW * RH, NT = TM * TN;
+    __shared__ double2 le[6][NS];
+    __shared__ double lk[NS];
+    __shared__ double accl[18][4 * NT];  // each thread's side sum (keeps 36 registers free)
+    const int oa = (mu != 0 && nu != 0) ? 0 : ((mu != 1 && nu != 1) ? 1 : 2);  // the other two directions
+    const int nd0 = g.nd[0], nd1 = g.nd[1], nd2 = g.nd[2], nd3 = g.nd[3];
+    const int ndm = sel4(mu, nd0, nd1, nd2, nd3), ndn = sel4(nu, nd0, nd1, nd2, nd3);
+    const int nda = sel4(oa, nd0, nd1, nd2, nd3);
+    // block -> tile.  Consecutive blocks (the same XCD) take neighbouring
+    // tiles: along mu, then nu, then the slice when the plane holds direction
+    // 0; otherwise direction 0 (which the half-site index runs along, so a
+    // 128-B line holds 16 consecutive x of one parity) fastest, so that the
+    // blocks sharing the lines of a tile's links and ipdot run together and
+    // the lines are fetched once into the XCD's L2.
+    int r = xcd_block_id();
+    const int tiles_m = ndm / TM, tiles_n = ndn / TN;
+    int tm, tn, xa, xb;
+    if (mu == 0) {
+        tm = r % tiles_m;
+        r /= tiles_m;
+        tn = r % tiles_n;
+        r /= tiles_n;
+        xa = r % nda;
+        xb = r / nda;
+    } else {
+        xa = r % nda;  // oa == 0
+        r /= nda;
+        tm = r % tiles_m;
+        r /= tiles_m;
+        tn = r % tiles_n;
+        xb = r / tiles_n;
+    }
+    int o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = k == mu ? tm * TM : k == nu ? tn * TN : k == oa ? xa : xb;
+    // links of the region into LDS
+    for (int s = threadIdx.x; s < NS; s += 4 * NT) {
+        const int d = s >= NS / 2, site = s - d * (NS / 2), i = site % RW, j = site / RW;
+        int y[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int n = sel4(k, nd0, nd1, nd2, nd3);
+            int v = o[k] + (k == mu ? i - 2 : 0) + (k == nu ? j - 2 : 0);
+            v += v < 0 ? n : 0;
+            v -= v >= n ? n : 0;
+            y[k] = v;
+        }
+        const int lex = y[0] + nd0 * (y[1] + nd1 * (y[2] + nd2 * y[3]));
+        const int arr = 2 * (d ? nu : mu) + ((y[0] + y[1] + y[2] + y[3]) & 1);
+        const double2* U = G.at(arr);
+        const size_t h = (size_t)(lex >> 1);
+#pragma unroll
+        for (int q = 0; q < 6; q++) le[q][s] = U[(size_t)q * g.sizeh + h];
+        lk[s] = pt ? ((const double*)(U + (size_t)9 * g.sizeh))[h] : 1.0;
+    }
+    __syncthreads();
+    // thread t = (2 dsel + side) NT + item: the link along mu (dsel 0) or nu
+    // (dsel 1) of tile site item, the terms on the upper (side 0) or lower
+    // (side 1) side of the plane
+    const int t = (int)threadIdx.x;
+    const int item = t % NT, w = t / NT;
+    const int dsel = w >> 1, side = w & 1;
+    const int im = item % TM, jn = item / TM;
+    LdsTile src{&le[0][0], lk, NS, RW, im + 2, jn + 2, dsel};
+#pragma unroll
+    for (int e = 0; e < 18; e++) accl[e][t] = 0.0;
+    plane_side(src, c_side[side], pt, tlsm, c0, [&](double sc, const Su3& T) {
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) {
+                const int e = 2 * (3 * rr + cc);
+                accl[e][t] = fma(sc, T.c[rr][cc].x, accl[e][t]);
+                accl[e + 1][t] = fma(sc, T.c[rr][cc].y, accl[e + 1][t]);
+            }
+    });
+    __syncthreads();
+    if (side != 0) return;
+    // the plane's staple of the link: both sides' sums (the lower side's
+    // thread is t + NT)
+    Su3 S, U, X;
+#pragma unroll
+    for (int rr = 0; rr < 3; rr++)
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+            const int e = 2 * (3 * rr + cc);
+            S.c[rr][cc] = make_double2(accl[e][t] + accl[e][t + NT], accl[e + 1][t] + accl[e + 1][t + NT]);
+        }
+    double ku;
+    src_link(src, 0, 0, 0, pt, U, ku);
+    su3_mul(U, S, X);
+    Tamat q = {};
+    ta_part(X, ku, q);
+    int y[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) y[k] = o[k] + (k == mu ? im : 0) + (k == nu ? jn : 0);
+    const int lex = y[0] + nd0 * (y[1] + nd1 * (y[2] + nd2 * y[3]));
+    const int arr = 2 * (dsel ? nu : mu) + ((y[0] + y[1] + y[2] + y[3]) & 1);
+    const int idxh = lex >> 1;
+    const int first = flags & (dsel ? 4 : 1), last = flags & (dsel ? 8 : 2);
+    if (!first) {
+        Tamat prev;
+        ta_load(ipdot, arr, g.sizeh, idxh, prev);
+        q.c01 = cadd(q.c01, prev.c01);
+        q.c02 = cadd(q.c02, prev.c02);
+        q.c12 = cadd(q.c12, prev.c12);
+        q.ic00 += prev.ic00;
+        q.ic11 += prev.ic11;
+    }
+    ta_store(ipdot, arr, g.sizeh, idxh, q);
+    if (last && mom) mom_step(mom, arr, g.sizeh, idxh, q, fact);
+    if (stap) {
+        // staples times K(x, d), summed over the planes (calc_ipdot_gauge_soloopenacc's output)
+        double2* O = (double2*)((char*)stap + (size_t)arr * su3_stride) + idxh;
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) {
+                const size_t e = (size_t)(3 * rr + cc) * g.sizeh;
+                double2 v = make_double2(ku * S.c[rr][cc].x, ku * S.c[rr][cc].y);
+                if (!first) v = cadd(v, O[e]);
+                O[e] = v;
+            }
     }
 }
 
@@ -1027,10 +1217,53 @@ bool launch_sigma_chain(void* sigma, void* lam, void* qa, const void* u, double 
     return hip_ok(hipGetLastError(), "k_sigma");
 }
 
+// tile extent of the plane kernel along a direction of extent n (0: none fits)
+static int gf_tile(int n) { return n % 8 == 0 ? 8 : (n % 4 == 0 ? 4 : 0); }
+static int g_gf_planes = 1;  // the plane kernels where the extents allow (0: the per-link kernel)
+
+template <int TM, int TN>
+static void launch_gf_plane(const void* u, void* stap, void* ipdot, void* mom, int mu, int nu, double fact,
+                            int flags) {
+    const Lattice& L = ctx().lat;
+    int oth = 1;
+    for (int k = 0; k < 4; k++)
+        if (k != mu && k != nu) oth *= L.nd[k];
+    const int nb = (L.nd[mu] / TM) * (L.nd[nu] / TN) * oth;
+    hipLaunchKernelGGL((k_gf_plane<TM, TN>), dim3(nb), dim3(4 * TM * TN), 0, ctx().stream, gauge_ptrs(u),
+                       (double2*)stap, L.su3_bytes(), ipdot, mom, link_geom(), mu, nu, L.par_temp, g_tlsm ? 1 : 0,
+                       g_c_zero, fact, flags);
+}
+
+static bool launch_gauge_force_planes(const void* u, void* stap, void* ipdot, void* mom, double fact) {
+    const Lattice& L = ctx().lat;
+    static const int planes[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    int first[4] = {-1, -1, -1, -1}, last[4] = {-1, -1, -1, -1};
+    for (int p = 0; p < 6; p++)
+        for (int e = 0; e < 2; e++) {
+            const int d = planes[p][e];
+            if (first[d] < 0) first[d] = p;
+            last[d] = p;
+        }
+    for (int p = 0; p < 6; p++) {
+        const int mu = planes[p][0], nu = planes[p][1];
+        const int flags = (first[mu] == p ? 1 : 0) | (last[mu] == p ? 2 : 0) | (first[nu] == p ? 4 : 0) |
+                          (last[nu] == p ? 8 : 0);
+        const int tm = gf_tile(L.nd[mu]), tn = gf_tile(L.nd[nu]);
+        if (tm == 8 && tn == 8) launch_gf_plane<8, 8>(u, stap, ipdot, mom, mu, nu, fact, flags);
+        else if (tm == 8) launch_gf_plane<8, 4>(u, stap, ipdot, mom, mu, nu, fact, flags);
+        else if (tn == 8) launch_gf_plane<4, 8>(u, stap, ipdot, mom, mu, nu, fact, flags);
+        else launch_gf_plane<4, 4>(u, stap, ipdot, mom, mu, nu, fact, flags);
+    }
+    return hip_ok(hipGetLastError(), "k_gf_plane");
+}
+
 bool launch_gauge_force(const void* u, void* stap, void* ipdot, void* mom, double fact) {
     const Lattice& L = ctx().lat;
     // lane offsets within a link array are 32-bit (up to 6 planes of 16 sizeh bytes)
     if ((size_t)96 * L.sizeh >= ((size_t)1 << 32)) return fail("gauge force: lattice too large (sizeh %ld)", (long)L.sizeh);
+    // the plane kernels when every extent is a multiple of 4
+    if (g_gf_planes && gf_tile(L.nd[0]) && gf_tile(L.nd[1]) && gf_tile(L.nd[2]) && gf_tile(L.nd[3]))
+        return launch_gauge_force_planes(u, stap, ipdot, mom, fact);
     // whole blocks per link array (the kernel takes the direction from the block)
     const int nb = 8 * (int)((L.sizeh + kGfLinks - 1) / kGfLinks);
     hipLaunchKernelGGL(k_gauge_force, dim3(nb), dim3(3 * kGfLinks), 0, ctx().stream, gauge_ptrs(u), (double2*)stap,
@@ -1251,6 +1484,12 @@ double calc_rettangolo_soloopenacc(const su3_soa* tconf_acc, su3_soa* local_plaq
 
 // device time per launch of the gauge-force kernel (HIP events on the
 // library stream), with the fused momentum step when mom is not null
+int ostp_set_gauge_force_kernel(int planes) {
+    const int prev = g_gf_planes;
+    g_gf_planes = planes ? 1 : 0;
+    return prev;
+}
+
 double ostp_time_gauge_force(const su3_soa* u, su3_soa* stap, tamat_soa* ipdot, thmat_soa* mom, double fact,
                              int reps) {
     const char* nm = "ostp_time_gauge_force";

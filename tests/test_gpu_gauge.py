@@ -168,3 +168,33 @@ def test_staple_is_action_derivative(tlsm, par_temp, shape):
             fd = (got[0] - got[1]) / (2 * eps)
             analytic = np.real(np.trace(1j * T[a] @ X))
             assert abs(fd - analytic) <= 1e-6 * max(1.0, abs(analytic)), (l, i, a, fd, analytic)
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8, 8), (4, 8, 4, 12), (8, 12, 4, 8)])
+@pytest.mark.parametrize("tlsm,par_temp", [(1, 0), (0, 0), (1, 1)])
+def test_plane_kernels_match_link_kernel(shape, tlsm, par_temp):
+    """The per-plane kernels (links of a tile in LDS; the force without
+    staples, as in the MD) against the per-link kernel: ipdot and the fused
+    momentum step mom -= i fact ipdot, over every tile shape (8 or 4 sites
+    along each direction) with and without the PAR_TEMP coefficients."""
+    f = Field(shape, bool(par_temp), 9)
+    set_action(tlsm)
+    init_lattice(f.shape, par_temp=f.pt)
+    du = Dev(f.u)
+    mom0 = np.random.default_rng(3).standard_normal(8 * 8 * f.sizeh)
+    outs = []
+    for planes, with_staples in ((0, True), (1, True), (1, False)):
+        prev = lib.ostp_set_gauge_force_kernel(planes)
+        dt, dm = Dev(nbytes=8 * 64 * f.sizeh), Dev(mom0)
+        ds = Dev(nbytes=8 * f.per) if with_staples else None
+        us = lib.ostp_time_gauge_force(du.p, ds.p if ds else None, dt.p, dm.p, 0.37, 1)
+        lib.ostp_set_gauge_force_kernel(prev)
+        ost.check(0, "gauge force")
+        assert us > 0
+        outs.append((dt.get(np.zeros(8 * 8 * f.sizeh)), dm.get(np.zeros(8 * 8 * f.sizeh)) - mom0,
+                     staple_planes(f, ds.get(np.zeros(8 * f.per, dtype=np.uint8))) if ds else None))
+    assert np.abs(outs[0][0]).max() > 0.1
+    for k in (1, 2):
+        assert rel(outs[k][0], outs[0][0]) < 1e-13
+        assert rel(outs[k][1], outs[0][1]) < 1e-13
+    assert rel(outs[1][2], outs[0][2]) < 1e-13
