@@ -109,20 +109,25 @@ __device__ __forceinline__ SolverCtl bctl(const BSys& S, int pred) {
 }
 
 // logical tile of this workgroup for a system with `ntiles` tiles; -1 if idle
-__device__ __forceinline__ int tile_of(int ntiles) {
-    const int blk = xcd_block_id();
+// Logical tile of a workgroup: XCD-contiguous ranges (each XCD walks one
+// slab of every system) or, with plain, the hardware order.
+__device__ __forceinline__ int tile_of(int ntiles, int plain) {
+    const int blk = plain ? (int)blockIdx.x : xcd_block_id();
     return blk < ntiles ? blk : -1;
 }
 
+// order: bit 0 t-fastest tiles (consecutive tiles at consecutive t), bit 1
+// hardware workgroup order instead of the XCD-contiguous remap
 template <int ROLE, int BLK>
-__global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, int pred, int tmajor) {
+__global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, int pred, int order) {
+    const int tmajor = order & 1;
     using R = typename std::conditional<(ROLE <= kRoleF32DeoDot || ROLE >= kRoleH16Doe), float, double>::type;
     using C = typename Cplx<R>::T;
     const BSys& S = sys[blockIdx.y];
     if (!pred_pass(bctl(S, pred))) return;
     const SiteGeom& g = S.g;
     const int ntiles = (g.sizeh + BLK - 1) / BLK;
-    const int tile = tile_of(ntiles);
+    const int tile = tile_of(ntiles, order & 2);
     if (tile < 0) return;
     const int s3 = g.sizeh / g.nd3;
     int idx;
@@ -586,13 +591,19 @@ __global__ __launch_bounds__(kBlock) void kb_finalize(const BSys* __restrict__ s
 }
 
 // ------------------------------------------------------------------ host side
+// Dslash tile of the batched kernels: threads per workgroup and order (as in
+// kb_dslash); the inner solver's hops (FP32 or FP16) and the FP64 hops of
+// the reliable updates are tuned separately
 struct Tile {
     int blk = 256;
-    int tmajor = 1;
+    int order = 1;
     float us = 0.f;
 };
+struct Tiles {
+    Tile inner, f64;
+};
 
-static std::map<std::string, Tile> g_tuned;  // batch signature -> chosen Dslash tile
+static std::map<std::string, Tiles> g_tuned;  // batch signature -> chosen Dslash tiles
 static std::string g_last_report;
 
 struct Batch {
@@ -615,11 +626,11 @@ static bool g_sgn = false;  // FP16 links of this call carry their +-1 phase as 
 // arguments depend on is unchanged (the per-system data live behind dsys and
 // are uploaded by every call; the scalars live in the device states).
 struct GraphKey {
-    int nsys = -1, max_sizeh = 0, blk = 0, tmajor = 0, half = 0, sgn = 0, iters = 0;
+    int nsys = -1, max_sizeh = 0, blk = 0, order = 0, blk64 = 0, order64 = 0, half = 0, sgn = 0, iters = 0;
     const void* dsys = nullptr;
     bool operator==(const GraphKey& o) const {
-        return nsys == o.nsys && max_sizeh == o.max_sizeh && blk == o.blk && tmajor == o.tmajor && half == o.half &&
-               sgn == o.sgn && iters == o.iters && dsys == o.dsys;
+        return nsys == o.nsys && max_sizeh == o.max_sizeh && blk == o.blk && order == o.order && blk64 == o.blk64 &&
+               order64 == o.order64 && half == o.half && sgn == o.sgn && iters == o.iters && dsys == o.dsys;
     }
 };
 static GraphKey g_graph_key;
@@ -638,10 +649,12 @@ static hipStream_t S() { return ctx().stream; }
 template <int ROLE>
 static void launch_role(const Tile& t, int nsys, int max_sizeh, int pred) {
     const int tiles = (max_sizeh + t.blk - 1) / t.blk;
-    if (t.blk == 128)
-        hipLaunchKernelGGL((kb_dslash<ROLE, 128>), dim3(tiles, nsys), dim3(128), 0, S(), g_batch.dsys, pred, t.tmajor);
+    if (t.blk == 64)
+        hipLaunchKernelGGL((kb_dslash<ROLE, 64>), dim3(tiles, nsys), dim3(64), 0, S(), g_batch.dsys, pred, t.order);
+    else if (t.blk == 128)
+        hipLaunchKernelGGL((kb_dslash<ROLE, 128>), dim3(tiles, nsys), dim3(128), 0, S(), g_batch.dsys, pred, t.order);
     else
-        hipLaunchKernelGGL((kb_dslash<ROLE, 256>), dim3(tiles, nsys), dim3(256), 0, S(), g_batch.dsys, pred, t.tmajor);
+        hipLaunchKernelGGL((kb_dslash<ROLE, 256>), dim3(tiles, nsys), dim3(256), 0, S(), g_batch.dsys, pred, t.order);
 }
 
 // odd = 1: partials stored in the odd slots (kb_h_xr)
@@ -673,42 +686,53 @@ static std::string signature(const std::vector<BSys>& v) {
     return s;
 }
 
-// Time each candidate tile on the batch's FP32 D_oe (the kernel of every
-// inner iteration), three launches each after one warm-up launch.
-static Tile autotune(const std::string& sig, int nsys, int max_sizeh, bool half) {
+// Time each candidate tile (64, 128 or 256 threads; t-fastest or contiguous
+// tiles; XCD-contiguous or hardware order) on the batch's inner D_oe (FP32,
+// or FP16 for the FP16 inner solver) and on its FP64 D_oe, three launches
+// each after one warm-up launch; keep the fastest of each.
+static Tiles autotune(const std::string& sig, int nsys, int max_sizeh, bool half) {
     auto it = g_tuned.find(sig);
     if (it != g_tuned.end()) return it->second;
-    const Tile cands[4] = {{256, 1, 0.f}, {256, 0, 0.f}, {128, 1, 0.f}, {128, 0, 0.f}};
     hipEvent_t e0, e1;
-    Tile best;
-    best.us = 1e30f;
+    Tiles best;
+    best.inner.us = best.f64.us = 1e30f;
     std::string rep = "batch " + sig + ":";
-    if (!hip_ok(hipEventCreate(&e0), "event") || !hip_ok(hipEventCreate(&e1), "event")) return Tile{};
-    for (const Tile& c0 : cands) {
-        Tile c = c0;
-        auto one = [&]() {
-            if (half && g_sgn) launch_role<kRoleS16Doe>(c, nsys, max_sizeh, kAlways);
-            else if (half) launch_role<kRoleH16Doe>(c, nsys, max_sizeh, kAlways);
-            else launch_role<kRoleF32Doe>(c, nsys, max_sizeh, kAlways);
-        };
-        one();
-        (void)hipEventRecord(e0, S());
-        for (int r = 0; r < 3; r++) one();
-        (void)hipEventRecord(e1, S());
-        (void)hipEventSynchronize(e1);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        c.us = 1000.f * ms / 3;
+    if (!hip_ok(hipEventCreate(&e0), "event") || !hip_ok(hipEventCreate(&e1), "event")) return Tiles{};
+    static const char* const kOrder[4] = {"contiguous", "t-fastest", "contiguous hw-order", "t-fastest hw-order"};
+    for (int fam = 0; fam < 2; fam++) {
+        rep += fam == 0 ? " inner D_oe:" : " fp64 D_oe:";
+        Tile& b = fam == 0 ? best.inner : best.f64;
+        for (int blk : {256, 128, 64})
+            for (int order = 0; order < 4; order++) {
+                Tile c;
+                c.blk = blk;
+                c.order = order;
+                auto one = [&]() {
+                    if (fam == 1) launch_role<kRoleF64Doe>(c, nsys, max_sizeh, kAlways);
+                    else if (half && g_sgn) launch_role<kRoleS16Doe>(c, nsys, max_sizeh, kAlways);
+                    else if (half) launch_role<kRoleH16Doe>(c, nsys, max_sizeh, kAlways);
+                    else launch_role<kRoleF32Doe>(c, nsys, max_sizeh, kAlways);
+                };
+                one();
+                (void)hipEventRecord(e0, S());
+                for (int r = 0; r < 3; r++) one();
+                (void)hipEventRecord(e1, S());
+                (void)hipEventSynchronize(e1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                c.us = 1000.f * ms / 3;
+                char buf[96];
+                snprintf(buf, sizeof buf, " [%d thr, %s] %.1f us", c.blk, kOrder[c.order], c.us);
+                rep += buf;
+                if (c.us < b.us) b = c;
+            }
         char buf[96];
-        snprintf(buf, sizeof buf, " [%d thr, %s] %.1f us", c.blk, c.tmajor ? "t-fastest" : "contiguous", c.us);
+        snprintf(buf, sizeof buf, " -> %d thr %s;", b.blk, kOrder[b.order]);
         rep += buf;
-        if (c.us < best.us) best = c;
     }
+    rep += "\n";
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    char buf[96];
-    snprintf(buf, sizeof buf, " -> %d thr %s\n", best.blk, best.tmajor ? "t-fastest" : "contiguous");
-    rep += buf;
     g_last_report += rep;
     if (verbosity_lv > 1) printf("%s", rep.c_str());
     g_tuned[sig] = best;
@@ -897,17 +921,19 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
     }
     if (!push_states(nsys)) return 0;
     const std::string sig = signature(hs) + (half ? (g_sgn ? " fp16 sign-phase" : " fp16") : " fp32");
-    const Tile t = autotune(sig, nsys, max_sizeh, half);
+    const Tiles tt = autotune(sig, nsys, max_sizeh, half);
+    const Tile& t = tt.inner;  // the inner solver's hops
+    const Tile& t64 = tt.f64;  // the FP64 hops
 
     // r = b - M x (FP64); inner r = p = r, out = 0, delta = |r_inner|^2
-    launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kAlways);
+    launch_role<kRoleF64Doe>(t64, nsys, max_sizeh, kAlways);
     if (half) {
-        launch_role<kRoleF64DeoResid>(t, nsys, max_sizeh, kAlways);
-        fin<kFinHScale>(nsys, kAlways, t.blk);
+        launch_role<kRoleF64DeoResid>(t64, nsys, max_sizeh, kAlways);
+        fin<kFinHScale>(nsys, kAlways, t64.blk);
         hipLaunchKernelGGL(kb_h_rescale, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kAlways, 1);
         fin<kFinHInit>(nsys, kAlways, kBlock);
     } else {
-        launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kAlways);
+        launch_role<kRoleF64Deo>(t64, nsys, max_sizeh, kAlways);
         hipLaunchKernelGGL(kb_mx_init, gsite, dim3(kBlock), 0, S(), B.dsys);
         fin<kFinCgRestart>(nsys, kAlways, kBlock);
     }
@@ -931,8 +957,8 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
             fin<kFinMxAlpha>(nsys, kIfRunning, t.blk);
             hipLaunchKernelGGL(kb_mx_xr, gsite, dim3(kBlock), 0, S(), B.dsys);
             fin<kFinMxLambda>(nsys, kIfRunningNoRU, kBlock);
-            launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kIfRunningRU);
-            launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kIfRunningRU);
+            launch_role<kRoleF64Doe>(t64, nsys, max_sizeh, kIfRunningRU);
+            launch_role<kRoleF64Deo>(t64, nsys, max_sizeh, kIfRunningRU);
             hipLaunchKernelGGL(kb_mx_resid, gsite, dim3(kBlock), 0, S(), B.dsys);
             fin<kFinMxLambda>(nsys, kIfRunningRU, kBlock);
             hipLaunchKernelGGL(kb_cg_p, gsite, dim3(kBlock), 0, S(), B.dsys);
@@ -947,9 +973,9 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
     auto enqueue_reliable_update = [&]() {
         if (!half) return;
         hipLaunchKernelGGL(kb_h_flush, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kIfRunningRU);
-        launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kIfRunningRU);
-        launch_role<kRoleF64DeoResid>(t, nsys, max_sizeh, kIfRunningRU);
-        fin<kFinHScale>(nsys, kIfRunningRU, t.blk);
+        launch_role<kRoleF64Doe>(t64, nsys, max_sizeh, kIfRunningRU);
+        launch_role<kRoleF64DeoResid>(t64, nsys, max_sizeh, kIfRunningRU);
+        fin<kFinHScale>(nsys, kIfRunningRU, t64.blk);
         hipLaunchKernelGGL(kb_h_rescale, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kIfRunningRU, 0);
         fin<kFinHRu>(nsys, kIfRunningRU, kBlock);
     };
@@ -959,7 +985,9 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
         key.nsys = nsys;
         key.max_sizeh = max_sizeh;
         key.blk = t.blk;
-        key.tmajor = t.tmajor;
+        key.order = t.order;
+        key.blk64 = t64.blk;
+        key.order64 = t64.order;
         key.half = half ? 1 : 0;
         key.sgn = g_sgn ? 1 : 0;
         key.iters = iters_per_graph;
@@ -1053,8 +1081,8 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
             hipLaunchKernelGGL(kb_h_flush, gsite, dim3(kBlock), 0, S(), B.dsys, (int)kAlways);
         else
             hipLaunchKernelGGL(kb_add_f_to_d, gsite, dim3(kBlock), 0, S(), B.dsys);
-        launch_role<kRoleF64Doe>(t, nsys, max_sizeh, kAlways);
-        launch_role<kRoleF64Deo>(t, nsys, max_sizeh, kAlways);
+        launch_role<kRoleF64Doe>(t64, nsys, max_sizeh, kAlways);
+        launch_role<kRoleF64Deo>(t64, nsys, max_sizeh, kAlways);
         hipLaunchKernelGGL(kb_resid_norm, gsite, dim3(kBlock), 0, S(), B.dsys, 1);
         fin<kFinStore>(nsys, kAlways, kBlock);
         if (!hip_ok(hipGetLastError(), nm) || !pull_states(nsys)) return 0;

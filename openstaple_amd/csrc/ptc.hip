@@ -381,12 +381,15 @@ __global__ __launch_bounds__(kBlock) void k_ptc_gram(PtcGeom A, const double2* _
     }
 }
 
-// sum the gram partials over blocks (one thread per entry, fixed order)
-__global__ __launch_bounds__(128) void k_ptc_gram_sum(const double* __restrict__ partials, int nblocks,
-                                                      double* __restrict__ out) {
+// sum the gram partials over blocks: one block per entry, each thread a
+// strided subset of the blocks, then the block's fixed tree (deterministic)
+__global__ __launch_bounds__(kBlock) void k_ptc_gram_sum(const double* __restrict__ partials, int nblocks,
+                                                         double* __restrict__ out) {
+    __shared__ double lds[kBlock / 64];
     double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partials[(size_t)b * 128 + threadIdx.x];
-    out[threadIdx.x] = s;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partials[(size_t)b * 128 + blockIdx.x];
+    s = block_sum<kBlock>(s, lds);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 // ------------------------------------------------------------------ host side
@@ -595,7 +598,7 @@ double ostp_ptc_loss_grad(const float* params, const su3_soa* u, const su3_soa_f
         hipLaunchKernelGGL((k_ptc_gram<true>), dim3(nb), dim3(kBlock), 0, S(), A, db, adj, g_ps.part);
     else
         hipLaunchKernelGGL((k_ptc_gram<false>), dim3(nb), dim3(kBlock), 0, S(), A, db, adj, g_ps.part);
-    hipLaunchKernelGGL(k_ptc_gram_sum, dim3(1), dim3(128), 0, S(), (const double*)g_ps.part, nb, g_ps.gram);
+    hipLaunchKernelGGL(k_ptc_gram_sum, dim3(128), dim3(kBlock), 0, S(), (const double*)g_ps.part, nb, g_ps.gram);
     double P[128];
     if (!hip_ok(hipGetLastError(), nm) || ostp_memcpy_dtoh(P, g_ps.gram, sizeof P) != 0) return -1.0;
     if (grad) {
