@@ -194,6 +194,11 @@ __device__ __forceinline__ void finalize(SolverState* st, double tot, double tot
         st->rho = 1.0;
         if (st->last_max_res < st->delta) st->last_max_res = st->delta;
         if (st->delta < st->mixed_delta * st->last_max_res) st->ru = 1;
+        // FP16 range: once |r|^2 has fallen by 1e-6 since the last update
+        // (entries ~1000x below the scale they were stored at, still normal
+        // FP16 numbers), the inner iterations stop until the reliable update
+        // at the end of the queued block, however many iterations it holds
+        if (st->delta < 1e-6 * st->last_max_res) st->hold = 1;
     } else if (F == kFinHRu) {
         // r was replaced by the (rescaled) FP64 residual and p multiplied by
         // rho: the last gammag uses the new |r|^2 over the previous |r|^2 in

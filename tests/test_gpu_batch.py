@@ -127,6 +127,32 @@ def test_batched_max_cg_reports_unconverged():
     assert conv == 0 and cg[0] == 20 and tr[0] > 1e-12
 
 
+@pytest.mark.parametrize("iters_per_graph", [64, 512])
+def test_batched_half_long_graphs(iters_per_graph):
+    """FP16 inner solver with more iterations per graph than a reliable
+    update interval: the inner iterations stop once |r|^2 has fallen by 1e-6
+    since the last update (FP16 entries stay normal numbers) and wait for the
+    update at the end of the graph, so the solve still converges (a graph of
+    the whole solve had driven the FP16 residual into underflow and NaN).
+    Updates that late cost iterations (FP16 rounding over a 1e-6 drop): the
+    bound is 8x the count with 8 iterations per graph."""
+    init_lattice((8, 8, 8, 8))
+    f = Field((8, 8, 8, 8), False, "expH", 1)
+    b = f.o.gen_gauss(7)
+    s, db, dx = system(f, b, 0.02)
+    arr = (ost.BatchSystem * 1)(s)
+    cg, ru, tr = (C.c_int * 1)(), (C.c_int * 1)(), (C.c_double * 1)()
+    conv = lib.ostp_batched_half_solve(1, arr, 1e-10, 20000, 0.1, iters_per_graph, cg, ru, tr)
+    ost.check(0, "batched half solve")
+    x = dx.get(b)
+    r = b - f.o.mdagm(f.u, x, f.ph, 0.02)
+    assert conv == 1 and tr[0] <= 1e-10 and np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+    ref = (C.c_int * 1)()
+    dx.put(np.zeros_like(b))
+    assert lib.ostp_batched_half_solve(1, arr, 1e-10, 20000, 0.1, 8, ref, None, tr) == 1
+    assert cg[0] <= 8 * ref[0], (cg[0], ref[0])
+
+
 @pytest.mark.parametrize("mu", [0.0, 0.2])
 def test_batched_half_precision_inner(mu):
     """FP16 storage in the inner solver: every system still reaches its FP64
