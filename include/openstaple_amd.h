@@ -566,6 +566,13 @@ double ostp_time_gauge_force(const su3_soa *u, su3_soa *stap, tamat_soa *ipdot, 
  * where every extent is a multiple of 4; 0 the per-link kernel.  Returns the
  * previous setting. */
 int ostp_set_gauge_force_kernel(int planes);
+/* No reference counterpart: the kernel that accumulates the fermion force's
+ * outer products (ker_openacc_compute_fermion_force,
+ * direct_product_of_fermions_into_auxmat): nmu = 4 (default) or 2 links of a
+ * site per thread, each vector of the thread's own site read once per shift;
+ * 0 one thread per link.  Results are bit-identical.  Returns the previous
+ * setting (-1 and a recorded error for another value). */
+int ostp_set_direct_product_kernel(int nmu);
 
 /* ===================== fermion measurements (Meas/ferm_meas.c) ========== */
 /* Z2-noise chiral condensate, ferm_meas.c:282-320 for nvec noise vectors:

@@ -282,6 +282,7 @@ _SIGS = {
     "calc_ipdot_gauge_soloopenacc": (None, [_P, _P, _P]),
     "ostp_time_gauge_force": (_D, [_P, _P, _P, _P, _D, _I]),
     "ostp_set_gauge_force_kernel": (_I, [_I]),
+    "ostp_set_direct_product_kernel": (_I, [_I]),
     "ostp_set_lattice_host": (_I, [_I, _I, _I, _I, _I]),
     "ostp_set_multishift_window": (_I, [_I]),
     **{f"{n}{k}": (None, [_P, _P, _P, _P]) for n in ("dM_dmu_eo", "dM_dmu_oe", "d2M_dmu2_eo", "d2M_dmu2_oe")

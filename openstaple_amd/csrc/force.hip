@@ -82,6 +82,61 @@ __global__ __launch_bounds__(kBlock) void k_direct_product(double2* __restrict__
         for (int b = 0; b < 3; b++) A[(3 * a + b) * sh + idxh] = cadd(A[(3 * a + b) * sh + idxh], acc.c[a][b]);
 }
 
+// The same sums with one thread per half-site and NMU directions: the
+// thread's own vector (r) is read once per shift for all NMU links of its
+// site instead of once per link, and the neighbours' vectors come from the
+// L2 of the XCD that owns the slab (per-link threads read all 2 x order
+// vectors 8 times over: 9.7 GB per launch at 48^3x12 and 19 shifts).  Each
+// link's sum runs over the shifts in the same order as k_direct_product, so
+// the results are bit-identical.  Thread = (parity, direction group, half-site).
+template <int NMU>
+__global__ __launch_bounds__(kBlock) void k_direct_product_sites(double2* __restrict__ aux, size_t su3_stride,
+                                                                const double2* __restrict__ xs,
+                                                                const double2* __restrict__ hs, long vstride,
+                                                                ShiftCoeffs co, FGeom g) {
+    const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
+    const int sh = g.sizeh;
+    constexpr int NG = 4 / NMU;
+    if (gid >= 2 * NG * sh) return;
+    const int pg = gid / sh, idxh = gid - pg * sh;
+    const int parity = pg / NG, mu0 = (pg - parity * NG) * NMU;
+    int ip[NMU];
+#pragma unroll
+    for (int m = 0; m < NMU; m++) ip[m] = nnp_index(idxh, parity, mu0 + m, g);
+    Su3 acc[NMU];
+#pragma unroll
+    for (int m = 0; m < NMU; m++) su3_zero(acc[m]);
+    for (int k = 0; k < co.order; k++) {
+        const double2* X = xs + k * vstride;
+        const double2* H = hs + k * vstride;
+        const double2* Lf = parity == 0 ? H : X;
+        const double2* Rf = parity == 0 ? X : H;
+        const double c = parity == 0 ? co.a[k] : -co.a[k];
+        double2 r[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) r[a] = cscale(cconj(Rf[a * sh + idxh]), c);
+#pragma unroll
+        for (int m = 0; m < NMU; m++) {
+            double2 l[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) l[a] = Lf[a * sh + ip[m]];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) acc[m].c[a][b] = cfma(l[a], r[b], acc[m].c[a][b]);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NMU; m++) {
+        double2* A = (double2*)((char*)aux + (2 * (mu0 + m) + parity) * su3_stride);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+                A[(3 * a + b) * sh + idxh] = cadd(A[(3 * a + b) * sh + idxh], acc[m].c[a][b]);
+    }
+}
+
 // out[dir] += aux[dir] * e^{i phase[dir]}
 __global__ __launch_bounds__(kBlock) void k_backfield_times_force(const double2* __restrict__ aux,
                                                                  double2* __restrict__ out, size_t su3_stride,
@@ -139,12 +194,23 @@ static FGeom fgeom() {
 }
 static int nblk8() { return blocks_for(8 * ctx().lat.sizeh); }
 
+static int g_dp_kernel = 4;  // links of a site per thread in the outer-product kernel (0: one thread per link)
+
 bool launch_direct_product(void* aux, const void* xs, const void* hs, const double* a, int order) {
     ShiftCoeffs co;
     co.order = order;
     for (int k = 0; k < kMaxShifts; k++) co.a[k] = k < order ? a[k] : 0.0;
-    hipLaunchKernelGGL(k_direct_product, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, (double2*)aux,
-                       ctx().lat.su3_bytes(), (const double2*)xs, (const double2*)hs, 3 * ctx().lat.sizeh, co, fgeom());
+    const long sh = ctx().lat.sizeh;
+    const size_t st = ctx().lat.su3_bytes();
+    if (g_dp_kernel == 4)
+        hipLaunchKernelGGL((k_direct_product_sites<4>), dim3(blocks_for(2 * sh)), dim3(kBlock), 0, ctx().stream,
+                           (double2*)aux, st, (const double2*)xs, (const double2*)hs, 3 * sh, co, fgeom());
+    else if (g_dp_kernel == 2)
+        hipLaunchKernelGGL((k_direct_product_sites<2>), dim3(blocks_for(4 * sh)), dim3(kBlock), 0, ctx().stream,
+                           (double2*)aux, st, (const double2*)xs, (const double2*)hs, 3 * sh, co, fgeom());
+    else
+        hipLaunchKernelGGL(k_direct_product, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, (double2*)aux, st,
+                           (const double2*)xs, (const double2*)hs, 3 * sh, co, fgeom());
     return hip_ok(hipGetLastError(), "k_direct_product");
 }
 
@@ -203,6 +269,16 @@ extern "C" {
 
 extern double gl_stout_rho;
 int ostp_stout_steps(void);
+
+int ostp_set_direct_product_kernel(int nmu) {
+    if (nmu != 0 && nmu != 2 && nmu != 4) {
+        fail("ostp_set_direct_product_kernel: %d directions per thread (0, 2 or 4)", nmu);
+        return -1;
+    }
+    const int prev = g_dp_kernel;
+    g_dp_kernel = nmu;
+    return prev;
+}
 
 void direct_product_of_fermions_into_auxmat(const vec3_soa* loc_s, const vec3_soa* loc_h, su3_soa* aux_u,
                                             const RationalApprox* approx, int iter) {
