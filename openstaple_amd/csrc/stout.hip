@@ -130,6 +130,19 @@ __device__ __forceinline__ void link_of(int gid, int sizeh, int& dir, int& idxh)
     dir = gid / sizeh;
     idxh = gid - dir * sizeh;
 }
+// The link of this thread in the kernels that read neighbour links (staples):
+// blocks run over the 8 link arrays fastest, then over tiles of kBlock
+// half-sites, and each XCD takes a contiguous range of blocks, so an XCD
+// owns a slab of the lattice for every link array and re-reads the slab's
+// neighbour links from its own L2 (with the link array slowest, every XCD
+// swept the whole lattice for one array and fetched every neighbour link).
+// Grid: 8 blocks per tile (nblk_tiles); false past the last half-site.
+__device__ __forceinline__ bool link_of_tile(int sizeh, int& dir, int& idxh) {
+    const int blk = xcd_block_id();
+    dir = blk & 7;
+    idxh = (blk >> 3) * kBlock + (int)threadIdx.x;
+    return idxh < sizeh;
+}
 
 
 // ============================================================ gauge force
@@ -809,10 +822,8 @@ __global__ __launch_bounds__(kBlock) void k_action_sum(GaugePtrs G, LinkGeom g, 
 __global__ __launch_bounds__(kBlock) void k_stout(GaugePtrs G, double2* __restrict__ uout, size_t su3_stride,
                                                  LinkGeom g, double rho, double c0, double2* __restrict__ stap_out,
                                                  void* __restrict__ qa_out, double2* __restrict__ exp_out) {
-    const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
-    if (gid >= 8 * g.sizeh) return;
     int dir, idxh;
-    link_of(gid, g.sizeh, dir, idxh);
+    if (!link_of_tile(g.sizeh, dir, idxh)) return;
     const int mu = dir >> 1, parity = dir & 1;
     int d[4];
     decode(idxh, parity, g, d);
@@ -843,10 +854,8 @@ __global__ __launch_bounds__(kBlock) void k_stout(GaugePtrs G, double2* __restri
 // staples accumulated into `stap` (calc_loc_staples_nnptrick_all_onlyferms adds)
 __global__ __launch_bounds__(kBlock) void k_staples(GaugePtrs G, double2* __restrict__ stap, size_t su3_stride,
                                                    LinkGeom g, double c0) {
-    const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
-    if (gid >= 8 * g.sizeh) return;
     int dir, idxh;
-    link_of(gid, g.sizeh, dir, idxh);
+    if (!link_of_tile(g.sizeh, dir, idxh)) return;
     int d[4];
     decode(idxh, dir & 1, g, d);
     Su3 S, old;
@@ -1043,10 +1052,8 @@ __device__ void lambda_local(const Tamat& q, const Su3& SP, const Su3& U, Thmat&
 __global__ __launch_bounds__(kBlock) void k_qa_lambda(GaugePtrs G, const double2* __restrict__ SP, size_t su3_stride,
                                                      void* __restrict__ qa, void* __restrict__ lam, LinkGeom g,
                                                      double rho, double c0, int qa_given) {
-    const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
-    if (gid >= 8 * g.sizeh) return;
     int dir, idxh;
-    link_of(gid, g.sizeh, dir, idxh);
+    if (!link_of_tile(g.sizeh, dir, idxh)) return;
     Su3 U, S;
     load_link(G.at(dir), g.sizeh, idxh, U);
     Tamat q;
@@ -1071,10 +1078,8 @@ __global__ __launch_bounds__(kBlock) void k_qa_lambda(GaugePtrs G, const double2
 __global__ __launch_bounds__(kBlock) void k_sigma(GaugePtrs G, const void* __restrict__ lam,
                                                  const void* __restrict__ qa, double2* __restrict__ S,
                                                  size_t su3_stride, LinkGeom g, double rho) {
-    const int gid = xcd_block_id() * kBlock + (int)threadIdx.x;
-    if (gid >= 8 * g.sizeh) return;
     int dir, idxh;
-    link_of(gid, g.sizeh, dir, idxh);
+    if (!link_of_tile(g.sizeh, dir, idxh)) return;
     const int mu = dir >> 1, parity = dir & 1, sh = g.sizeh;
     int d[4];
     decode(idxh, parity, g, d);
@@ -1200,19 +1205,20 @@ static GaugePtrs gauge_ptrs(const void* u) {
     return G;
 }
 static int nblk8() { return blocks_for(8 * ctx().lat.sizeh); }
+static int nblk_tiles() { return 8 * blocks_for(ctx().lat.sizeh); }  // link_of_tile kernels
 
 bool launch_stout(const void* u, void* uout, double rho, double c0, void* stap, void* qa, void* expo) {
-    hipLaunchKernelGGL(k_stout, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u), (double2*)uout,
+    hipLaunchKernelGGL(k_stout, dim3(nblk_tiles()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u), (double2*)uout,
                        ctx().lat.su3_bytes(), link_geom(), rho, c0, (double2*)stap, qa, (double2*)expo);
     return hip_ok(hipGetLastError(), "k_stout");
 }
 
 bool launch_sigma_chain(void* sigma, void* lam, void* qa, const void* u, double rho, double c0) {
     const size_t st = ctx().lat.su3_bytes();
-    hipLaunchKernelGGL(k_qa_lambda, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u),
+    hipLaunchKernelGGL(k_qa_lambda, dim3(nblk_tiles()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u),
                        (const double2*)sigma, st, qa, lam, link_geom(), rho, c0, 0);
     if (!hip_ok(hipGetLastError(), "k_qa_lambda")) return false;
-    hipLaunchKernelGGL(k_sigma, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u), (const void*)lam,
+    hipLaunchKernelGGL(k_sigma, dim3(nblk_tiles()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(u), (const void*)lam,
                        (const void*)qa, (double2*)sigma, st, link_geom(), rho);
     return hip_ok(hipGetLastError(), "k_sigma");
 }
@@ -1367,7 +1373,7 @@ void calc_loc_staples_nnptrick_all_onlyferms(const su3_soa* u, su3_soa* loc_stap
     const void* du = resolve(u, 8 * L.su3_bytes(), nm);
     void* ds = resolve(loc_stap, 8 * L.su3_bytes(), nm);
     if (!du || !ds) return;
-    hipLaunchKernelGGL(k_staples, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(du), (double2*)ds,
+    hipLaunchKernelGGL(k_staples, dim3(nblk_tiles()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(du), (double2*)ds,
                        L.su3_bytes(), link_geom(), g_c_zero);
     if (hip_ok(hipGetLastError(), nm)) sync(nm);
 }
@@ -1396,7 +1402,7 @@ void compute_lambda(thmat_soa* Lm, const su3_soa* SP, const su3_soa* U, const ta
     const void* du = resolve(U, gb, nm);
     const void* dq = resolve(QA, tb, nm);
     if (!dl || !ds || !du || !dq) return;
-    hipLaunchKernelGGL(k_qa_lambda, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(du), (const double2*)ds,
+    hipLaunchKernelGGL(k_qa_lambda, dim3(nblk_tiles()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(du), (const double2*)ds,
                        L.su3_bytes(), const_cast<void*>(dq), dl, link_geom(), 0.0, g_c_zero, 1);
     if (hip_ok(hipGetLastError(), nm)) sync(nm);
 }
@@ -1413,7 +1419,7 @@ void compute_sigma(const thmat_soa* Lm, const su3_soa* U, su3_soa* S, const tama
     void* ds = resolve(S, gb, nm);
     const void* dq = resolve(QA, tb, nm);
     if (!dl || !ds || !du || !dq) return;
-    hipLaunchKernelGGL(k_sigma, dim3(nblk8()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(du), dl, dq, (double2*)ds,
+    hipLaunchKernelGGL(k_sigma, dim3(nblk_tiles()), dim3(kBlock), 0, ctx().stream, gauge_ptrs(du), dl, dq, (double2*)ds,
                        L.su3_bytes(), link_geom(), rho_of(istopo));
     if (hip_ok(hipGetLastError(), nm)) sync(nm);
 }
