@@ -60,6 +60,9 @@ struct BSys {
 
 // half-sites per thread of the FP16 x/r update (kb_h_xr)
 constexpr int kXrSites = 4;
+// FP16 inner solver: iterations between the predicated reliable-update tails
+// of a captured graph
+constexpr int kRuBlock = 16;
 
 enum BRole : int {
     kRoleF32Doe = 0,   // hf = D_oe pf                          (FP32)
@@ -997,7 +1000,13 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
             drop_graph();
             hipGraph_t graph = nullptr;
             if (!hip_ok(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal), "begin capture")) return 0;
-            for (int j = 0; j < iters_per_graph; j++) enqueue_iteration();
+            // FP16 inner solver: a reliable-update tail after every kRuBlock
+            // iterations and at the end, so a long graph (a whole solve in
+            // one launch) keeps the FP16 recurrence near its stored scale
+            for (int j = 0; j < iters_per_graph; j++) {
+                enqueue_iteration();
+                if ((j + 1) % kRuBlock == 0 && j + 1 < iters_per_graph) enqueue_reliable_update();
+            }
             enqueue_reliable_update();
             const hipError_t ce = hipStreamEndCapture(S(), &graph);
             if (!hip_ok(ce, "end capture")) return 0;

@@ -129,13 +129,11 @@ def test_batched_max_cg_reports_unconverged():
 
 @pytest.mark.parametrize("iters_per_graph", [64, 512])
 def test_batched_half_long_graphs(iters_per_graph):
-    """FP16 inner solver with more iterations per graph than a reliable
-    update interval: the inner iterations stop once |r|^2 has fallen by 1e-6
-    since the last update (FP16 entries stay normal numbers) and wait for the
-    update at the end of the graph, so the solve still converges (a graph of
-    the whole solve had driven the FP16 residual into underflow and NaN).
-    Updates that late cost iterations (FP16 rounding over a 1e-6 drop): the
-    bound is 8x the count with 8 iterations per graph."""
+    """FP16 inner solver with long graphs: a graph carries a reliable-update
+    tail after every 16 iterations (and the inner iterations hold once |r|^2
+    has fallen by 1e-6 since the last update), so a graph of the whole solve
+    converges like short ones (before, it drove the FP16 residual into
+    underflow and NaN)."""
     init_lattice((8, 8, 8, 8))
     f = Field((8, 8, 8, 8), False, "expH", 1)
     b = f.o.gen_gauss(7)
@@ -150,7 +148,7 @@ def test_batched_half_long_graphs(iters_per_graph):
     ref = (C.c_int * 1)()
     dx.put(np.zeros_like(b))
     assert lib.ostp_batched_half_solve(1, arr, 1e-10, 20000, 0.1, 8, ref, None, tr) == 1
-    assert cg[0] <= 8 * ref[0], (cg[0], ref[0])
+    assert cg[0] <= 1.2 * ref[0], (cg[0], ref[0])
 
 
 @pytest.mark.parametrize("mu", [0.0, 0.2])
