@@ -460,6 +460,19 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
                                               "loss": "A-norm of N b - x over <x, b>"}}
         res["cg_speedup_learned_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t4 - t3), 3)
         res["cg_iteration_ratio_learned_vs_fp64"] = round(res["cg_fp64"]["iterations"] / max(1, its + fb), 3)
+        # the same network inside the FP32 flexible CG, corrected in FP64
+        lib.ostp_memset(x, 0, vb)
+        learned.solve_mixed(op, params, x, b, 1e-10)  # warm-up (scratch allocation)
+        lib.ostp_memset(x, 0, vb)
+        lib.ostp_synchronize()
+        t3 = time.perf_counter()
+        st, its, outer, fb, trr = learned.solve_mixed(op, params, x, b, 1e-10)
+        lib.ostp_synchronize()
+        t4 = time.perf_counter()
+        res["cg_learned_fcg_fp32"] = {"seconds": round(t4 - t3, 4), "iterations": its, "reliable_updates": outer,
+                                      "update_every": 0.1, "fallback_iterations": fb, "true_residual": trr,
+                                      "status": st, "err_vs_fp64": rel_err(x, x64)}
+        res["cg_speedup_learned_fp32_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t4 - t3), 3)
     ost.check(0, "solves")
     for p in [*d64, *d32, out, sf, sf32, x, x64, out64, diff, u4f]:
         lib.ostp_free(p)

@@ -654,6 +654,20 @@ double ostp_ptc_loss_grad(const float *params, const su3_soa *u, const su3_soa_f
 int ostp_fcg_solve(const su3_soa *u, ferm_param *pars, const su3_soa_f *u_f, const float *params,
                    vec3_soa *solution, const vec3_soa *in, double res, int max_cg, int *cg_return,
                    int *fallback_cg, double *true_res);
+/* The same flexible CG run in FP32 (FP32 links, phases and vectors, N
+ * applied to FP32 fields) with FP64 reliable updates: whenever the FP32
+ * residual has dropped by inner_res since the last update, the accumulated
+ * FP32 update is added to x in FP64 and the FP64 true residual b - A x
+ * replaces the FP32 one, keeping the search direction (0.1: an update at
+ * every 10x drop).  The exit test is the FP64 true residual
+ * ||b - A x|| / ||b|| <= res.  An update that leaves more than
+ * (1 + inner_res) / 2 of the previous FP64 residual, or an exhausted budget,
+ * hands over to plain CG as in ostp_fcg_solve.  *cg_return = FP32 iterations, *outer_return = reliable
+ * updates (may be NULL).  It has no counterpart in the reference (whose mixed
+ * solver, OpenAcc/inverter_mixed_precision.c, is unpreconditioned). */
+int ostp_fcg_mixed_solve(const su3_soa *u, ferm_param *pars, const su3_soa_f *u_f, const float *params,
+                         vec3_soa *solution, const vec3_soa *in, double res, int max_cg, double inner_res,
+                         int *cg_return, int *outer_return, int *fallback_cg, double *true_res);
 
 /* ======================================================== generators ===== */
 /* Reference field rules (OpenAcc/random_assignement.c:67-92,106-142,188-250)

@@ -369,6 +369,8 @@ _SIGS = {
     "ostp_ptc_apply": (_I, [_P, _P, _P, _P, _P]),
     "ostp_ptc_loss_grad": (_D, [_P, _P, _P, _P, _P, _P, _I, _P]),
     "ostp_fcg_solve": (_I, [_P, _P, _P, _P, _P, _P, _D, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
+    "ostp_fcg_mixed_solve": (_I, [_P, _P, _P, _P, _P, _P, _D, _I, _D, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I),
+                                  C.POINTER(_D)]),
     "ostp_time_dslash": (_D, [_P, _P, _P, _P, _I, _I, _I]),
     "ostp_time_stream": (_D, [_I, _P, _P, C.c_size_t, _I, _I, _I]),
     "ostp_set_dslash_serpentine": (_I, [_I]),

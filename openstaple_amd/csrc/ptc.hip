@@ -23,6 +23,7 @@
 #include "ostp_dslash.h"
 #include "ostp_half.h"
 
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -91,6 +92,13 @@ __device__ __forceinline__ float2 ldv(const double2* p) {
     const double2 v = *p;
     return make_float2((float)v.x, (float)v.y);
 }
+__device__ __forceinline__ float2 ldv(const float2* p) { return *p; }
+// a vector entry (FP64 or FP32 field) in double precision and back
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+template <typename TR> __device__ __forceinline__ TR from_d2(double2 v);
+template <> __device__ __forceinline__ double2 from_d2<double2>(double2 v) { return v; }
+template <> __device__ __forceinline__ float2 from_d2<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
 
 // One transport: U e^{i th} v (DAG = false) or U^dag e^{-i th} v (DAG = true)
 // with the link of array l at half-site iu
@@ -136,16 +144,17 @@ __device__ __forceinline__ V3 xport(const PtcGeom& A, int l, int iu, float2 v0, 
     return o;
 }
 
-template <bool DAG, bool SGN>
-__device__ __forceinline__ V3 xport(const PtcGeom& A, int l, int iu, const double2* __restrict__ in, int sizeh,
+template <bool DAG, bool SGN, typename TR>
+__device__ __forceinline__ V3 xport(const PtcGeom& A, int l, int iu, const TR* __restrict__ in, int sizeh,
                                     int iv) {
     return xport<DAG, SGN>(A, l, iu, ldv(in + iv), ldv(in + sizeh + iv), ldv(in + 2 * sizeh + iv));
 }
 
-// The eight transports T_nu in (in: even FP64 field) at odd half-site y
-// (the hops of D_oe, fermion_matrix.c:101-157, before its 1/2 and signs).
-template <bool SGN>
-__device__ __forceinline__ void transports(const PtcGeom& A, const double2* in, int y, V3 t[8]) {
+// The eight transports T_nu in (in: even FP64 or FP32 field) at odd
+// half-site y (the hops of D_oe, fermion_matrix.c:101-157, before its 1/2
+// and signs).
+template <bool SGN, typename TR>
+__device__ __forceinline__ void transports(const PtcGeom& A, const TR* in, int y, V3 t[8]) {
     const Nbrs nb = site_neighbours<1>(A.g, y);
     const int sizeh = A.g.sizeh;
 #pragma unroll
@@ -212,8 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_ptc_links(PtcLinkSrc a, int sgn) {
 // Layer 1 + channel mixing: g_nu(y) = sum_mu M[nu][mu] T_mu r (y) at odd y,
 // stored as one BF16 record (3 colours) per channel nu and half-site.
 // One wave = 64 odd sites; MFMA columns n = colour * 64 + site-in-wave.
-template <bool SGN>
-__global__ __launch_bounds__(256) void k_ptc_layer1(PtcGeom A, const double2* __restrict__ r, b6* __restrict__ gout,
+template <bool SGN, typename TR>
+__global__ __launch_bounds__(256) void k_ptc_layer1(PtcGeom A, const TR* __restrict__ r, b6* __restrict__ gout,
                                                     PtcMix M, SolverCtl ctl) {
     if (!pred_pass(ctl)) return;
     __shared__ bf16x8 lds[4][192 * 2];  // per wave: [n][k], k = 2 mu + re/im (16 bf16 per column)
@@ -283,10 +292,10 @@ enum PtcMode : int { kPtcApply = 0, kPtcFcg = 1, kPtcTrain = 2 };
 //  kPtcApply: out = z
 //  kPtcFcg:   out = z, partials (Re<z,q>, Re<z,r>)        (aux = q; q == nullptr -> 0)
 //  kPtcTrain: out = e = z - x, partials (|e|^2, Re<e,r>)  (aux = x)
-template <int MODE, bool SGN>
+template <int MODE, bool SGN, typename TR>
 __global__ __launch_bounds__(kBlock) void k_ptc_layer2(PtcGeom A, const b6* __restrict__ g,
-                                                       const double2* __restrict__ r, double2* __restrict__ out,
-                                                       const double2* __restrict__ aux, float w0, SolverCtl ctl) {
+                                                       const TR* __restrict__ r, TR* __restrict__ out,
+                                                       const TR* __restrict__ aux, float w0, SolverCtl ctl) {
     if (!pred_pass(ctl)) return;
     const int x = blockIdx.x * kBlock + threadIdx.x;
     const int sizeh = A.g.sizeh;
@@ -309,21 +318,23 @@ __global__ __launch_bounds__(kBlock) void k_ptc_layer2(PtcGeom A, const b6* __re
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const long k = (long)c * sizeh + x;
-            const double2 rr = r[k];
+            const double2 rr = to_d2(r[k]);
             double2 z = make_double2((double)w0 * rr.x + (double)acc[c].x, (double)w0 * rr.y + (double)acc[c].y);
             if (MODE == kPtcTrain) {
-                const double2 xx = aux[k];
+                const double2 xx = to_d2(aux[k]);
                 z = make_double2(z.x - xx.x, z.y - xx.y);
                 v0 += z.x * z.x + z.y * z.y;
                 v1 += z.x * rr.x + z.y * rr.y;
             } else if (MODE == kPtcFcg) {
+                // the dots of the stored (rounded) z, as the next kernels see it
+                z = to_d2(from_d2<TR>(z));
                 if (aux) {
-                    const double2 q = aux[k];
+                    const double2 q = to_d2(aux[k]);
                     v0 += z.x * q.x + z.y * q.y;
                 }
                 v1 += z.x * rr.x + z.y * rr.y;
             }
-            out[k] = z;
+            out[k] = from_d2<TR>(z);
         }
     }
     if (MODE != kPtcApply) block_partial<2>(v0, v1, ctl);
@@ -503,35 +514,42 @@ static PtcMix mix_of(const PtcParams& W) {
     return M;
 }
 
-template <bool SGN>
-static void launch_apply_t(const PtcGeom& A, const PtcMix& M, float w0, const double2* r, double2* z, int mode,
-                           const double2* aux, const SolverCtl& ctl) {
+template <bool SGN, typename TR>
+static void launch_apply_t(const PtcGeom& A, const PtcMix& M, float w0, const TR* r, TR* z, int mode,
+                           const TR* aux, const SolverCtl& ctl) {
     const long sh = A.g.sizeh;
-    hipLaunchKernelGGL((k_ptc_layer1<SGN>), dim3((unsigned)((sh + 255) / 256)), dim3(256), 0, S(), A, r, g_ps.g, M,
-                       ctl);
+    hipLaunchKernelGGL((k_ptc_layer1<SGN, TR>), dim3((unsigned)((sh + 255) / 256)), dim3(256), 0, S(), A, r, g_ps.g,
+                       M, ctl);
     const int nb = blocks_for(sh);
     const b6* g = g_ps.g;
     if (mode == kPtcApply)
-        hipLaunchKernelGGL((k_ptc_layer2<kPtcApply, SGN>), dim3(nb), dim3(kBlock), 0, S(), A, g, r, z, aux, w0, ctl);
+        hipLaunchKernelGGL((k_ptc_layer2<kPtcApply, SGN, TR>), dim3(nb), dim3(kBlock), 0, S(), A, g, r, z, aux, w0,
+                           ctl);
     else if (mode == kPtcFcg)
-        hipLaunchKernelGGL((k_ptc_layer2<kPtcFcg, SGN>), dim3(nb), dim3(kBlock), 0, S(), A, g, r, z, aux, w0, ctl);
+        hipLaunchKernelGGL((k_ptc_layer2<kPtcFcg, SGN, TR>), dim3(nb), dim3(kBlock), 0, S(), A, g, r, z, aux, w0, ctl);
     else
-        hipLaunchKernelGGL((k_ptc_layer2<kPtcTrain, SGN>), dim3(nb), dim3(kBlock), 0, S(), A, g, r, z, aux, w0, ctl);
+        hipLaunchKernelGGL((k_ptc_layer2<kPtcTrain, SGN, TR>), dim3(nb), dim3(kBlock), 0, S(), A, g, r, z, aux, w0,
+                           ctl);
 }
 
-// z = N r on device pointers (r, z FP64 even fields), queued on the library stream
-static bool launch_apply(const PtcGeom& A, bool sgn, const PtcMix& M, float w0, const double2* r, double2* z,
-                         int mode, const double2* aux, const SolverCtl& ctl) {
+// z = N r on device pointers (r, z even fields, FP64 or FP32), queued on the library stream
+template <typename TR>
+static bool launch_apply(const PtcGeom& A, bool sgn, const PtcMix& M, float w0, const TR* r, TR* z, int mode,
+                         const TR* aux, const SolverCtl& ctl) {
     if (sgn)
-        launch_apply_t<true>(A, M, w0, r, z, mode, aux, ctl);
+        launch_apply_t<true, TR>(A, M, w0, r, z, mode, aux, ctl);
     else
-        launch_apply_t<false>(A, M, w0, r, z, mode, aux, ctl);
+        launch_apply_t<false, TR>(A, M, w0, r, z, mode, aux, ctl);
     return hip_ok(hipGetLastError(), "ptc launch");
 }
 
 // CG pieces shared with solvers.hip
 bool launch_cg_xr(void* x, void* r, const void* p, const void* s, const SolverCtl& ctl);
 bool launch_cg_p(void* p, const void* r, const SolverCtl& ctl);
+bool launch_cg_xr_f(void* x, void* r, const void* p, const void* s, const SolverCtl& ctl);
+bool launch_cg_p_f(void* p, const void* r, const SolverCtl& ctl);
+bool launch_add_f_to_d(void* x, const void* e);
+bool run_cg_batches(const std::function<bool(int)>& enqueue, int first_batch);
 int cg_solve_from(const void* u, const void* ph, const void* b, void* x, void* r, void* h, void* s, void* p,
                   double mass, double res, int max_cg, int* cg_out, double* true_res);
 
@@ -553,7 +571,7 @@ int ostp_ptc_apply(const float* params, const su3_soa_f* u_f, const float_soa* p
     const double2* r = (const double2*)resolve(in, vb, nm);
     double2* z = (double2*)resolve(out, vb, nm);
     if (!r || !z) return 1;
-    if (!launch_apply(A, sgn, mix_of(W), W.w0, r, z, kPtcApply, nullptr, plain_ctl())) return 1;
+    if (!launch_apply<double2>(A, sgn, mix_of(W), W.w0, r, z, kPtcApply, nullptr, plain_ctl())) return 1;
     return entry_done(nm) ? 0 : 1;
 }
 
@@ -677,32 +695,19 @@ int ostp_fcg_solve(const su3_soa* u, ferm_param* pars, const su3_soa_f* u_f, con
     // r = b - A x ; z = N r ; <z, r> ; p = z
     if (!launch_mdagm<double>(du, dph, x, h, q, m2, 1, kFinStore, ctl_pred(kAlways)) || !launch_sub(b, q, r))
         return INVERTER_FAILURE;
-    if (!launch_apply(A, sgn, M, W.w0, r, z, kPtcFcg, nullptr, ctl_pred(kAlways)) ||
+    if (!launch_apply<double2>(A, sgn, M, W.w0, r, z, kPtcFcg, nullptr, ctl_pred(kAlways)) ||
         !launch_finalize(kFinFcgInit, ctl_pred(kAlways), nbh) ||
         !hip_ok(hipMemcpyAsync(p, z, vb, hipMemcpyDeviceToDevice, S()), "p = z"))
         return INVERTER_FAILURE;
-    int last_cg = -1, batch = 4;
-    for (;;) {
-        if (!pull_state()) return INVERTER_FAILURE;
-        if (hs.status != kRunning) break;
-        if (hs.cg == last_cg) {
-            fail("%s: solver state did not advance", nm);
-            return INVERTER_FAILURE;
-        }
-        last_cg = hs.cg;
-        for (int j = 0; j < batch; j++) {
-            const SolverCtl run = ctl_pred(kIfRunning);
-            // q = A p, alpha = <z,r> / <p,q>
-            if (!launch_mdagm<double>(du, dph, p, h, q, m2, 2, kFinFcgAlpha, run)) return INVERTER_FAILURE;
-            // x += alpha p ; r -= alpha q ; |r|^2, exit test
-            if (!launch_cg_xr(x, r, p, q, run) || !launch_finalize(kFinFcgRes, run, nbh)) return INVERTER_FAILURE;
-            // z = N r ; beta = -alpha <z,q> / <z,r>_old ; p = z + beta p
-            if (!launch_apply(A, sgn, M, W.w0, r, z, kPtcFcg, q, run) || !launch_finalize(kFinFcgBeta, run, nbh) ||
-                !launch_cg_p(p, z, run))
-                return INVERTER_FAILURE;
-        }
-        batch = batch < 64 ? 2 * batch : 64;
-    }
+    auto iteration = [&](int) {
+        const SolverCtl run = ctl_pred(kIfRunning);
+        // q = A p, alpha = <z,r> / <p,q> ; x += alpha p ; r -= alpha q ; |r|^2, exit test ;
+        // z = N r ; beta = -alpha <z,q> / <z,r>_old ; p = z + beta p
+        return launch_mdagm<double>(du, dph, p, h, q, m2, 2, kFinFcgAlpha, run) && launch_cg_xr(x, r, p, q, run) &&
+               launch_finalize(kFinFcgRes, run, nbh) && launch_apply(A, sgn, M, W.w0, r, z, kPtcFcg, q, run) &&
+               launch_finalize(kFinFcgBeta, run, nbh) && launch_cg_p(p, z, run);
+    };
+    if (!run_cg_batches(iteration, 4)) return INVERTER_FAILURE;
     int fcg_its = hs.cg;
     // FP64 true residual
     if (!launch_mdagm<double>(du, dph, x, h, q, m2, 1, kFinStore, ctl_pred(kAlways)) || !launch_sub(b, q, r))
@@ -722,6 +727,133 @@ int ostp_fcg_solve(const su3_soa* u, ferm_param* pars, const su3_soa_f* u_f, con
     if (fallback_cg) *fallback_cg = extra;
     if (true_res) *true_res = tr;
     (void)n3;
+    return status;
+}
+
+// Flexible CG with the network in FP32 (links, phases, vectors and the
+// network's input/output), with FP64 reliable updates: the FP32 iteration
+// accumulates its update e; whenever its residual has dropped by inner_res
+// since the last update, x += e in FP64, the FP64 true residual r = b - A x
+// replaces the FP32 one, and the iteration continues along the same search
+// direction (z = N r, beta from the kept q and <z,r>).  The exit test is the
+// FP64 true residual |b - A x| / |b| <= res.  An update that leaves more than
+// (1 + inner_res) / 2 of the previous FP64 residual (or an exhausted
+// iteration budget) hands the iterate to the FP64 plain CG, as
+// ostp_fcg_solve does.
+int ostp_fcg_mixed_solve(const su3_soa* u, ferm_param* pars, const su3_soa_f* u_f, const float* params,
+                         vec3_soa* solution, const vec3_soa* in, double res, int max_cg, double inner_res,
+                         int* cg_return, int* outer_return, int* fallback_cg, double* true_res) {
+    const char* nm = "ostp_fcg_mixed_solve";
+    if (!require_init(nm)) return INVERTER_FAILURE;
+    if (!pars || !cg_return) {
+        fail("%s: null argument", nm);
+        return INVERTER_FAILURE;
+    }
+    if (!(inner_res > 0.0 && inner_res < 1.0)) {
+        fail("%s: inner tolerance %g outside (0, 1)", nm, inner_res);
+        return INVERTER_FAILURE;
+    }
+    const Lattice& L = ctx().lat;
+    PtcParams W;
+    PtcGeom A;
+    bool sgn = false;
+    if (!params_of(params, W) || !ptc_scratch() || !geom_of(u_f, pars->phases_f, A, sgn, nm))
+        return INVERTER_FAILURE;
+    const PtcMix M = mix_of(W);
+    const size_t vb = L.vec3_bytes(), vbf = L.vec3f_bytes();
+    const void* du = resolve(u, 8 * L.su3_bytes(), nm);
+    const void* dph = resolve(pars->phases, 8 * L.dsoa_bytes(), nm);
+    const void* duf = resolve(u_f, 8 * L.su3f_bytes(), nm);
+    const void* dphf = resolve(pars->phases_f, 8 * L.fsoa_bytes(), nm);
+    const double2* b = (const double2*)resolve(in, vb, nm);
+    double2* x = (double2*)resolve(solution, vb, nm);
+    if (!du || !dph || !duf || !dphf || !b || !x) return INVERTER_FAILURE;
+    // work vectors: FP64 r, h, q, p (p only for the fallback CG); FP32 r, h, q, p, z, e
+    static void* work = nullptr;
+    static size_t work_bytes = 0;
+    const size_t need = 4 * vb + 6 * vbf;
+    if (work_bytes < need) {
+        if (work) (void)hipFree(work);
+        work = nullptr;
+        work_bytes = 0;
+        if (!hip_ok(hipMalloc(&work, need), "fcg work")) return INVERTER_FAILURE;
+        work_bytes = need;
+    }
+    char* wv = (char*)work;
+    double2 *r = (double2*)wv, *h = (double2*)(wv + vb), *q = (double2*)(wv + 2 * vb), *p = (double2*)(wv + 3 * vb);
+    char* wf = wv + 4 * vb;
+    float2 *rf = (float2*)wf, *hf = (float2*)(wf + vbf), *qf = (float2*)(wf + 2 * vbf), *pf = (float2*)(wf + 3 * vbf),
+           *zf = (float2*)(wf + 4 * vbf), *ef = (float2*)(wf + 5 * vbf);
+    const double mass = pars->ferm_mass, m2 = mass * mass;
+    const int nbh = blocks_for(L.sizeh);
+    const long n3 = 3 * L.sizeh;
+    double sn = 0.0;
+    if (!reduce_to_host(0, b, nullptr, 0, &sn, nullptr)) return INVERTER_FAILURE;
+    SolverState& hs = *ctx().state_host;
+    auto ctl_pred = [](int pred) {
+        SolverCtl c = plain_ctl();
+        c.pred = pred;
+        return c;
+    };
+    // FP64 true residual r = b - A x, |r|^2
+    auto true_residual = [&](double* rr) {
+        return launch_mdagm<double>(du, dph, x, h, q, m2, 1, kFinStore, ctl_pred(kAlways)) && launch_sub(b, q, r) &&
+               reduce_to_host(0, r, nullptr, 0, rr, nullptr);
+    };
+    auto iteration = [&](int) {
+        const SolverCtl run = ctl_pred(kIfRunning);
+        return launch_mdagm<float>(duf, dphf, pf, hf, qf, m2, 2, kFinFcgAlpha, run) &&
+               launch_cg_xr_f(ef, rf, pf, qf, run) && launch_finalize(kFinFcgRes, run, nbh) &&
+               launch_apply<float2>(A, sgn, M, W.w0, rf, zf, kPtcFcg, qf, run) &&
+               launch_finalize(kFinFcgBeta, run, nbh) && launch_cg_p_f(pf, zf, run);
+    };
+    double rr = 0.0;
+    if (!true_residual(&rr)) return INVERTER_FAILURE;
+    int outer = 0;
+    bool stalled = false;
+    std::memset(&hs, 0, sizeof hs);
+    hs.res = inner_res;
+    hs.safety = 1.0;
+    hs.max_cg = max_cg;
+    while (std::sqrt(rr / sn) > res && hs.cg < max_cg && !stalled) {
+        // the FP32 iteration restarts from the FP64 residual: e = 0, r_f = r;
+        // the first time z = N r, p = z; afterwards z = N r, beta, p = z + beta p
+        hs.source_norm = rr;
+        hs.status = kRunning;
+        if (!push_state() || !launch_d2f(r, rf, n3) || !hip_ok(hipMemsetAsync(ef, 0, vbf, S()), "e = 0"))
+            return INVERTER_FAILURE;
+        if (outer == 0) {
+            if (!launch_apply<float2>(A, sgn, M, W.w0, rf, zf, kPtcFcg, nullptr, ctl_pred(kAlways)) ||
+                !launch_finalize(kFinFcgInit, ctl_pred(kAlways), nbh) ||
+                !hip_ok(hipMemcpyAsync(pf, zf, vbf, hipMemcpyDeviceToDevice, S()), "p = z"))
+                return INVERTER_FAILURE;
+        } else if (!launch_apply<float2>(A, sgn, M, W.w0, rf, zf, kPtcFcg, qf, ctl_pred(kAlways)) ||
+                   !launch_finalize(kFinFcgBeta, ctl_pred(kAlways), nbh) || !launch_cg_p_f(pf, zf, ctl_pred(kAlways)))
+            return INVERTER_FAILURE;
+        if (!run_cg_batches(iteration, 4)) return INVERTER_FAILURE;  // leaves hs pulled
+        outer++;
+        const double rr_old = rr;
+        if (!launch_add_f_to_d(x, ef) || !true_residual(&rr)) return INVERTER_FAILURE;
+        const double keep = 0.5 * (1.0 + inner_res);  // what a working update leaves at most
+        stalled = !(rr < keep * keep * rr_old);
+        if (verbosity_lv > 1)
+            printf("%s: update %d after %d FP32 iterations, FP64 residual %e\n", nm, outer, hs.cg,
+                   std::sqrt(rr / sn));
+    }
+    const int total = hs.cg;
+    double tr = std::sqrt(rr / sn);
+    int extra = 0;
+    int status = tr <= res ? INVERTER_SUCCESS : INVERTER_FAILURE;
+    if (status != INVERTER_SUCCESS) {
+        if (verbosity_lv > 0)
+            printf("%s: stopped at %e after %d iterations and %d updates, plain CG takes over\n", nm, tr, total,
+                   outer);
+        status = cg_solve_from(du, dph, b, x, r, h, q, p, mass, res, max_cg, &extra, &tr);
+    }
+    *cg_return = total;
+    if (outer_return) *outer_return = outer;
+    if (fallback_cg) *fallback_cg = extra;
+    if (true_res) *true_res = tr;
     return status;
 }
 

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
+#include <functional>
 #include <vector>
 
 namespace ostp {
@@ -621,6 +622,29 @@ bool launch_cg_p(void* p, const void* r, const SolverCtl& ctl) {
     hipLaunchKernelGGL((k_cg_p<double2>), dim3(blocks_for(n3)), dim3(kBlock), 0, S(), (double2*)p, (const double2*)r,
                        n3, ctl);
     return check_launch("k_cg_p");
+}
+// The batch driver for solvers outside this file (flexible CG): iterations
+// queued through `enqueue`, batches cut by the |r|^2 convergence rate.
+bool run_cg_batches(const std::function<bool(int)>& enqueue, int first_batch) {
+    return run_batches([&](int j) { return enqueue(j); }, first_batch, dist_cg);
+}
+// FP32 forms for the FP32-inner flexible CG; x += (double) e
+bool launch_cg_xr_f(void* x, void* r, const void* p, const void* s, const SolverCtl& ctl) {
+    const long sizeh = ctx().lat.sizeh;
+    hipLaunchKernelGGL((k_cg_xr<float2>), dim3(blocks_for(sizeh)), dim3(kBlock), 0, S(), (float2*)x, (float2*)r,
+                       (const float2*)p, (const float2*)s, (int)sizeh, ctl);
+    return check_launch("k_cg_xr<float2>");
+}
+bool launch_cg_p_f(void* p, const void* r, const SolverCtl& ctl) {
+    const long n3 = 3 * ctx().lat.sizeh;
+    hipLaunchKernelGGL((k_cg_p<float2>), dim3(blocks_for(n3)), dim3(kBlock), 0, S(), (float2*)p, (const float2*)r, n3,
+                       ctl);
+    return check_launch("k_cg_p<float2>");
+}
+bool launch_add_f_to_d(void* x, const void* e) {
+    const long n3 = 3 * ctx().lat.sizeh;
+    hipLaunchKernelGGL(k_add_f_to_d, dim3(blocks_for(n3)), dim3(kBlock), 0, S(), (double2*)x, (const float2*)e, n3);
+    return check_launch("k_add_f_to_d");
 }
 int cg_solve_from(const void* u, const void* ph, const void* b, void* x, void* r, void* h, void* s, void* p,
                   double mass, double res, int max_cg, int* cg_out, double* true_res) {

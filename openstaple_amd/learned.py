@@ -1,7 +1,8 @@
 """Training and use of the learned parallel-transport preconditioner.
 
 The network and its forward/backward kernels live in the HIP library
-(`ostp_ptc_apply`, `ostp_ptc_loss_grad`, `ostp_fcg_solve`); this module holds
+(`ostp_ptc_apply`, `ostp_ptc_loss_grad`, `ostp_fcg_solve`,
+`ostp_fcg_mixed_solve`); this module holds
 the host side: generating (b, x) pairs with the library's plain FP64 CG,
 the Adam optimiser over the OSTP_PTC_NPARAM parameters (w0, then Re/Im of
 w[c][nu]), and the flexible-CG solve with plain-CG fallback.
@@ -133,4 +134,18 @@ def solve(op: Operator, params: np.ndarray, x: int, b: int, res: float = 1e-10, 
     return st, cg.value, fb.value, tr.value
 
 
-__all__ = ["init_params", "Operator", "Pairs", "loss_grad", "norms", "train", "solve", "PTC_CHANNELS"]
+def solve_mixed(op: Operator, params: np.ndarray, x: int, b: int, res: float = 1e-10, max_cg: int = 40000,
+                inner_res: float = 0.1):
+    """Flexible CG with the network in FP32 and FP64 reliable updates at
+    every `inner_res` drop of the residual; returns (status, FP32 iterations,
+    reliable updates, plain-CG fallback iterations, FP64 true residual)."""
+    p32 = np.ascontiguousarray(params, dtype=np.float32)
+    cg, outer, fb, tr = C.c_int(0), C.c_int(0), C.c_int(0), C.c_double(0)
+    st = lib.ostp_fcg_mixed_solve(op.u, C.byref(op.fp), op.u_f, p32.ctypes.data, x, b, res, max_cg, inner_res,
+                                  C.byref(cg), C.byref(outer), C.byref(fb), C.byref(tr))
+    check(0, "ostp_fcg_mixed_solve")
+    return st, cg.value, outer.value, fb.value, tr.value
+
+
+__all__ = ["init_params", "Operator", "Pairs", "loss_grad", "norms", "train", "solve", "solve_mixed",
+           "PTC_CHANNELS"]

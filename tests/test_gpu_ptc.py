@@ -8,6 +8,9 @@ MFMA with FP32 accumulation, BF16 channel fields):
     Euclidean loss and for the A-norm loss (oracle composition);
   * flexible CG reaches the FP64 true residual; when it is stopped early the
     plain-CG fallback finishes the solve;
+  * the FP32 flexible CG with FP64 reliable updates reaches the same FP64
+    true residual (checked against the oracle's A), in at most 10% more
+    iterations than the FP64 flexible CG when it updates at every 10x drop;
   * training with the library kernels lowers the loss and the trained network
     cuts the flexible-CG iteration count below plain CG's (the gain is printed).
 """
@@ -176,6 +179,38 @@ def test_fcg_converges_and_falls_back(s8):
     assert st == 1 and its == 5 and fb > 0 and tr <= 1e-10
 
 
+def test_fcg_mixed_reaches_fp64_residual(s8):
+    o = s8.o
+    b = o.gen_gauss(51)
+    db, x = Dev(b), Dev(nbytes=b.nbytes)
+    p = learned.init_params()
+    x64 = Dev(nbytes=b.nbytes)
+    st64, its64, fb64, _ = learned.solve(s8.op, p, x64.p, db.p, 1e-10)
+    assert st64 == 1
+    for inner, bound in ((1e-1, 1.1), (1e-2, 1.15), (1e-5, 2.0)):
+        x.put(np.zeros_like(b))
+        st, its, outer, fb, tr = learned.solve_mixed(s8.op, p, x.p, db.p, 1e-10, inner_res=inner)
+        print(f"\n8^4: inner tolerance {inner:.0e}: {its} FP32 iterations, {outer} FP64 updates "
+              f"(FP64 flexible CG {its64})")
+        r = b - o.mdagm(s8.u, x.get(b), s8.ph, s8.mass)
+        assert st == 1 and fb == 0 and tr <= 1e-10
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+        assert outer >= 2
+        # the updates keep the search direction: with an update at every 10x
+        # (100x) drop of the residual, at most 10% (15%) more iterations than
+        # the FP64 flexible CG; updates 1e5 apart let the FP32 recursion drift
+        assert its <= bound * its64
+    # a start already within the tolerance runs no pass
+    st, its2, outer2, fb2, tr2 = learned.solve_mixed(s8.op, p, x.p, db.p, 1e-10)
+    assert st == 1 and its2 == 0 and outer2 == 0 and tr2 <= 1e-10
+    # an inner tolerance outside (0, 1) is an error
+    assert lib.ostp_fcg_mixed_solve(s8.op.u, C.byref(s8.op.fp), s8.op.u_f, np.zeros(ost.PTC_NPARAM,
+                                    np.float32).ctypes.data, x.p, db.p, 1e-10, 100, 1.0, C.byref(C.c_int()),
+                                    None, None, None) == 0
+    assert lib.ostp_last_error()
+    lib.ostp_clear_error()
+
+
 def test_training_gains_over_plain_cg(s8):
     """Train on 16 pairs with the library kernels; report and check the gain."""
     pairs = learned.Pairs(s8.op, s8.o.vec3().nbytes, 16, seed=77)
@@ -232,3 +267,8 @@ def test_config4_full_shape():
     st, its, fb, tr = learned.solve(op, params, x.p, b.p, 1e-10)
     print(f"\n32^4 m=0.005: plain CG {cg.value} iterations, flexible CG {its} (+{fb} fallback), true residual {tr:.2e}")
     assert st == 1 and fb == 0 and tr <= 1e-10 and its < cg.value
+    lib.ostp_memset(x.p, 0, vb)
+    st, its32, outer, fb, tr = learned.solve_mixed(op, params, x.p, b.p, 1e-10)
+    print(f"32^4 m=0.005: FP32 flexible CG {its32} iterations, {outer} FP64 reliable updates (+{fb} fallback), "
+          f"true residual {tr:.2e}")
+    assert st == 1 and fb == 0 and tr <= 1e-10 and its32 < 1.4 * its
