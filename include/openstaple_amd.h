@@ -403,6 +403,15 @@ int ostp_set_multishift_precision(int mode, int force_only);
 /* Text of the autotuner's measurements so far (candidates and choice per
  * batch signature); returns its length, copies up to len-1 bytes into buf. */
 int ostp_autotune_report(char *buf, int len);
+/* Fix the batched Dslash tile instead of autotuning it: blk = 64, 128 or 256
+ * threads, order bit 0 t-fastest tiles, bit 1 hardware workgroup order, bit 2
+ * systems interleaved (the workgroups of one tile for every system follow
+ * each other, so systems on one gauge field share its links through L2);
+ * blk = 0 returns to the autotuner.  Results do not depend on the choice
+ * (reductions are per tile in a fixed order).  Returns the previous setting
+ * (blk * 8 + order, 0 for the autotuner), -1 and a recorded error for a bad
+ * one. */
+int ostp_set_batch_tile(int blk, int order);
 
 /* ============ molecular dynamics (su3_utilities.h, su3_measurements.h, md_integrator.h) */
 /* su3_utilities.h:53-57: mom[l] += fact * ipdot[l] in the sense

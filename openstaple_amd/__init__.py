@@ -365,6 +365,7 @@ _SIGS = {
                                       C.POINTER(_D)]),
     "ostp_batched_half_solve": (_I, [_I, C.POINTER(BatchSystem), _D, _I, _D, _I, C.POINTER(_I), C.POINTER(_I),
                                      C.POINTER(_D)]),
+    "ostp_set_batch_tile": (_I, [_I, _I]),
     "ostp_autotune_report": (_I, [C.c_char_p, _I]),
     "ostp_ptc_apply": (_I, [_P, _P, _P, _P, _P]),
     "ostp_ptc_loss_grad": (_D, [_P, _P, _P, _P, _P, _P, _I, _P]),

@@ -17,9 +17,11 @@
 //  * `iters_per_graph` iterations are captured once into a HIP graph, which
 //    is replayed until every system has left the running state; the host
 //    reads the S states once per replay;
-//  * the Dslash tile (workgroup size, t-fastest or contiguous tile order) is
-//    chosen per batch signature by timing the candidates on the batch's own
-//    data the first time that signature is seen (autotune table).
+//  * the Dslash tile (workgroup size, t-fastest or contiguous tile order,
+//    systems one after the other or interleaved so that systems on one gauge
+//    field share its links through L2) is chosen per batch signature by
+//    timing the candidates on the batch's own data the first time that
+//    signature is seen (autotune table).
 #include "ostp_dslash.h"
 #include "ostp_half.h"
 
@@ -119,18 +121,42 @@ __device__ __forceinline__ int tile_of(int ntiles, int plain) {
     return blk < ntiles ? blk : -1;
 }
 
+// (system, logical tile) of a workgroup of the (tiles, systems) grid with the
+// systems fastest: the workgroups of one tile for every system follow each
+// other (on one XCD, with the XCD-contiguous remap of the whole grid), so
+// systems that share a gauge field read its links of that tile once from
+// memory and from the XCD's L2 after that.
+__device__ __forceinline__ void sys_tile_interleaved(int plain, int& sysi, int& tile) {
+    const int G = (int)(gridDim.x * gridDim.y), i = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+    int L = i;
+    if (!plain) {
+        const int q = G >> 3, r = G & 7, x = i & 7, k = i >> 3;
+        L = x * q + (x < r ? x : r) + k;
+    }
+    const int ns = (int)gridDim.y;
+    sysi = L % ns;
+    tile = L / ns;
+}
+
 // order: bit 0 t-fastest tiles (consecutive tiles at consecutive t), bit 1
-// hardware workgroup order instead of the XCD-contiguous remap
+// hardware workgroup order instead of the XCD-contiguous remap, bit 2 the
+// systems interleaved (sys_tile_interleaved) instead of one after the other
 template <int ROLE, int BLK>
 __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, int pred, int order) {
     const int tmajor = order & 1;
     using R = typename std::conditional<(ROLE <= kRoleF32DeoDot || ROLE >= kRoleH16Doe), float, double>::type;
     using C = typename Cplx<R>::T;
-    const BSys& S = sys[blockIdx.y];
+    int sysi = (int)blockIdx.y, tile = -1;
+    if (order & 4) sys_tile_interleaved(order & 2, sysi, tile);
+    const BSys& S = sys[sysi];
     if (!pred_pass(bctl(S, pred))) return;
     const SiteGeom& g = S.g;
     const int ntiles = (g.sizeh + BLK - 1) / BLK;
-    const int tile = tile_of(ntiles, order & 2);
+    if (order & 4) {
+        if (tile >= ntiles) tile = -1;
+    } else {
+        tile = tile_of(ntiles, order & 2);
+    }
     if (tile < 0) return;
     const int s3 = g.sizeh / g.nd3;
     int idx;
@@ -166,7 +192,7 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
                 const bool at_max = cg1 >= st->max_cg;
                 const bool conv = !(sqrt(true2 / st->source_norm) > st->res * st->safety);
                 gm = (float)(lambda / delta);
-                if (blockIdx.x == 0 && threadIdx.x == 0) {
+                if (tile == 0 && threadIdx.x == 0) {
                     st->gammag = lambda / delta;
                     st->lambda = lambda;
                     if (at_max || conv) {
@@ -188,7 +214,7 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
             }
         } else {
             gm = (float)S.st->gammag;
-            if (blockIdx.x == 0 && threadIdx.x == 0 && S.st->xr_fresh) {
+            if (tile == 0 && threadIdx.x == 0 && S.st->xr_fresh) {
                 SolverState* st = S.st;
                 st->delta_old = st->delta;
                 st->delta = st->lambda;
@@ -679,21 +705,37 @@ static bool push_states(int n) {
     return hip_ok(hipStreamSynchronize(S()), "stream synchronize");
 }
 
+// lattice extents of every system and which systems share a gauge field
+// (the index of the first system with the same FP64 links)
 static std::string signature(const std::vector<BSys>& v) {
     std::string s;
-    for (const auto& b : v) {
+    for (size_t i = 0; i < v.size(); i++) {
+        size_t first = i;
+        for (size_t j = 0; j < i; j++)
+            if (v[j].u64[0] == v[i].u64[0]) {
+                first = j;
+                break;
+            }
         char buf[64];
-        snprintf(buf, sizeof buf, "%dx%dx%dx%d;", b.g.nd0, b.g.nd1, b.g.nd2, b.g.nd3);
+        snprintf(buf, sizeof buf, "%dx%dx%dx%d@%zu;", v[i].g.nd0, v[i].g.nd1, v[i].g.nd2, v[i].g.nd3, first);
         s += buf;
     }
     return s;
 }
 
 // Time each candidate tile (64, 128 or 256 threads; t-fastest or contiguous
-// tiles; XCD-contiguous or hardware order) on the batch's inner D_oe (FP32,
+// tiles; XCD-contiguous or hardware order; systems one after the other or
+// interleaved) on the batch's inner D_oe (FP32,
 // or FP16 for the FP16 inner solver) and on its FP64 D_oe, three launches
 // each after one warm-up launch; keep the fastest of each.
+static Tile g_forced_tile{0, 0, 0.f};  // blk 0: autotune
+
 static Tiles autotune(const std::string& sig, int nsys, int max_sizeh, bool half) {
+    if (g_forced_tile.blk) {
+        Tiles t;
+        t.inner = t.f64 = g_forced_tile;
+        return t;
+    }
     auto it = g_tuned.find(sig);
     if (it != g_tuned.end()) return it->second;
     hipEvent_t e0, e1;
@@ -701,12 +743,14 @@ static Tiles autotune(const std::string& sig, int nsys, int max_sizeh, bool half
     best.inner.us = best.f64.us = 1e30f;
     std::string rep = "batch " + sig + ":";
     if (!hip_ok(hipEventCreate(&e0), "event") || !hip_ok(hipEventCreate(&e1), "event")) return Tiles{};
-    static const char* const kOrder[4] = {"contiguous", "t-fastest", "contiguous hw-order", "t-fastest hw-order"};
+    static const char* const kOrder[8] = {"contiguous", "t-fastest", "contiguous hw-order", "t-fastest hw-order",
+                                          "contiguous interleaved", "t-fastest interleaved",
+                                          "contiguous hw-order interleaved", "t-fastest hw-order interleaved"};
     for (int fam = 0; fam < 2; fam++) {
         rep += fam == 0 ? " inner D_oe:" : " fp64 D_oe:";
         Tile& b = fam == 0 ? best.inner : best.f64;
         for (int blk : {256, 128, 64})
-            for (int order = 0; order < 4; order++) {
+            for (int order = 0; order < 8; order++) {
                 Tile c;
                 c.blk = blk;
                 c.order = order;
@@ -1127,6 +1171,18 @@ int ostp_batched_mixed_solve(int nsys, const ostp_batch_system* sys, double res,
 int ostp_batched_half_solve(int nsys, const ostp_batch_system* sys, double res, int max_cg, double mixed_delta,
                             int iters_per_graph, int* cg_return, int* ru_return, double* true_res) {
     return batched_solve(nsys, sys, res, max_cg, mixed_delta, iters_per_graph, cg_return, ru_return, true_res, true);
+}
+
+int ostp_set_batch_tile(int blk, int order) {
+    if (blk != 0 && ((blk != 64 && blk != 128 && blk != 256) || order < 0 || order > 7)) {
+        fail("ostp_set_batch_tile: workgroup %d / order %d (64, 128 or 256 and 0..7, or 0 for the autotuner)", blk,
+             order);
+        return -1;
+    }
+    const int prev = g_forced_tile.blk ? g_forced_tile.blk * 8 + g_forced_tile.order : 0;
+    g_forced_tile.blk = blk;
+    g_forced_tile.order = blk ? order : 0;
+    return prev;
 }
 
 int ostp_autotune_report(char* buf, int len) {

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
+#include <chrono>
 #include <functional>
 #include <vector>
 
@@ -1447,6 +1448,8 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
                            (float2*)pk, (const float2*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kAlways));
     hipLaunchKernelGGL(k_msx_flush, dim3(nb3), dim3(kBlock), 0, S(), fl, n3, ctl_of(kAlways));
     if (!check_launch("flush")) return 0;
+    const auto t_cgm = std::chrono::steady_clock::now();
+    if (verbosity_lv > 1 && !hip_ok(hipStreamSynchronize(S()), "timing")) return 0;
 
     // every shift goes to the batched solver, which starts from the CG-M
     // solution with the FP64 residual b - (M + shift_k) x_k (one that is already
@@ -1466,6 +1469,10 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
     }
     const int converged = batched_solve_internal(order, sys.data(), res, max_cg, 0.1, 16, cg_refine, nullptr,
                                                  true_res, half_inner != 0);
+    if (verbosity_lv > 1 && hip_ok(hipStreamSynchronize(S()), "timing")) {
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_cgm).count();
+        printf("mixed multishift: refinement of %d shifts %.4f s after the CG-M\n", order, sec);
+    }
     if (verbosity_lv > 0)
         printf("Terminated mixed multishift: %d CG-M iterations, %d of %d shifts converged\n", cg, converged, order);
     return converged;

@@ -110,6 +110,63 @@ def test_batched_sources_and_replicas():
     assert b"->" in buf.value
 
 
+@pytest.mark.parametrize("half", [False, True])
+def test_tile_choices_agree(half):
+    """Every fixed Dslash tile (workgroup 64/128/256; t-fastest, hardware
+    order and system-interleaved orders) solves the batch: three sources on
+    one gauge field (which the interleaved orders share through L2) and a
+    replica on a second lattice.  The workgroup size and the t-fastest bit
+    fix which sites a reduction partial sums; the order in which workgroups
+    run (hardware order, systems interleaved) does not, so those variants
+    give bit-identical solutions and iteration counts; the others converge
+    to the FP64 true residual within 2% of the autotuned iteration counts."""
+    init_lattice((8, 8, 8, 8))
+    main = Field((8, 8, 8, 8), False, "expH", 1)
+    rep = Field((4, 6, 8, 10), True, "cold", 2)
+    cases = [(main, main.o.gen_z2(21, k), 0.05 + 0.01 * k) for k in range(3)]
+    cases.append((rep, rep.o.gen_z2(22, 0), 0.1))
+    keep, arr = [], (ost.BatchSystem * len(cases))()
+    for k, (f, b, m) in enumerate(cases):
+        sy, db, dx = system(f, b, m)
+        arr[k] = sy
+        keep.append((db, dx))
+    n = len(cases)
+    solve = lib.ostp_batched_half_solve if half else lib.ostp_batched_mixed_solve
+
+    def run():
+        for k in range(n):
+            keep[k][1].put(np.zeros_like(cases[k][1]))
+        cg, ru, tr = (C.c_int * n)(), (C.c_int * n)(), (C.c_double * n)()
+        conv = solve(n, arr, RES, 10000, 0.1, 8, cg, ru, tr)
+        ost.check(0, "batched solve")
+        assert conv == n, list(tr)
+        return list(cg), [keep[k][1].get(cases[k][1]) for k in range(n)]
+
+    cg0, _ = run()
+    try:
+        for blk in (64, 128, 256):
+            for tfast in (0, 1):
+                first = None
+                for order in (tfast, tfast | 2, tfast | 4, tfast | 6):
+                    assert lib.ostp_set_batch_tile(blk, order) >= 0
+                    cg1, x1 = run()
+                    for k, (f, b, m) in enumerate(cases):
+                        assert abs(cg1[k] - cg0[k]) <= max(1, 0.02 * cg0[k]), (blk, order, cg1, cg0)
+                        r = b - f.o.mdagm(f.u, x1[k], f.ph, m)
+                        assert np.linalg.norm(r) / np.linalg.norm(b) <= RES, (blk, order, k)
+                    if first is None:
+                        first = (cg1, x1)
+                    else:
+                        assert cg1 == first[0], (blk, order)
+                        for k in range(n):
+                            assert np.array_equal(x1[k], first[1][k]), (blk, order, k)
+    finally:
+        lib.ostp_set_batch_tile(0, 0)
+    assert lib.ostp_set_batch_tile(96, 0) == -1
+    assert lib.ostp_last_error()
+    lib.ostp_clear_error()
+
+
 def test_batched_max_cg_reports_unconverged():
     init_lattice((8, 8, 8, 8))
     f = Field((8, 8, 8, 8), False, "expH", 1)
