@@ -59,7 +59,8 @@ def parse():
     ap.add_argument("--learned-steps", type=int, default=300)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve workload")
     ap.add_argument("--iters-per-graph", type=int, default=16,
-                    help="iterations captured per HIP graph in the batched driver (0: direct launches, for profilers)")
+                    help="iterations captured per HIP graph in the batched driver and the library's own batched "
+                         "solves (0: direct launches, for profilers)")
     ap.add_argument("--no-rhmc", action="store_true", help="skip the RHMC trajectory workload")
     ap.add_argument("--rhmc-only", action="store_true", help="run only the RHMC trajectory workload")
     ap.add_argument("--rhmc-lattice", type=str, default="48,48,48,12")
@@ -101,6 +102,8 @@ def main():
     shape = tuple(int(x) for x in args.lattice.split(","))
     L = ost.Lattice(*shape)
     L.init()
+    # the library's own batched solves (rational-solve refinement) follow the same setting
+    lib.ostp_set_batched_graph_iterations(args.iters_per_graph)
     sizeh = L.sizeh
     vb, gb, pb = L.vec3_bytes, 8 * L.su3_bytes, 8 * 8 * sizeh
 
@@ -636,6 +639,7 @@ def run_rhmc(ost, lib, args):
     shape = tuple(int(x) for x in args.rhmc_lattice.split(","))
     L = ost.Lattice(*shape)
     L.init()
+    lib.ostp_set_batched_graph_iterations(args.iters_per_graph)
     lib.ostp_set_stout(0.15, 2, 0.0, 0, 0, 1)
     lib.ostp_set_inverter_tricks(0, 0, 0.1, 10000)
     sizeh, gb, vb, tb = L.sizeh, 8 * L.su3_bytes, L.vec3_bytes, 8 * 64 * L.sizeh

@@ -400,16 +400,24 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param *pars, RationalA
  * iterations of every shift.  Returns the previous mode (-1 and a recorded
  * error for a bad one). */
 int ostp_set_multishift_precision(int mode, int force_only);
+/* Iterations per HIP graph of the batched solves the library runs itself
+ * (the refinement of ostp_multishift_mixed_solve, ostp_chiral_condensate);
+ * default 16, 0 launches the kernels directly (for tracers that do not
+ * follow graph launches).  Returns the previous value, -1 and a recorded
+ * error outside 0..4096. */
+int ostp_set_batched_graph_iterations(int n);
 /* Text of the autotuner's measurements so far (candidates and choice per
  * batch signature); returns its length, copies up to len-1 bytes into buf. */
 int ostp_autotune_report(char *buf, int len);
 /* Fix the batched Dslash tile instead of autotuning it: blk = 64, 128 or 256
  * threads, order bit 0 t-fastest tiles, bit 1 hardware workgroup order, bit 2
  * systems interleaved (the workgroups of one tile for every system follow
- * each other, so systems on one gauge field share its links through L2);
- * blk = 0 returns to the autotuner.  Results do not depend on the choice
- * (reductions are per tile in a fixed order).  Returns the previous setting
- * (blk * 8 + order, 0 for the autotuner), -1 and a recorded error for a bad
+ * each other, so systems on one gauge field share its links through L2),
+ * bit 3 chunked hops (one thread applies each link to up to four systems on
+ * one gauge field: the FP64 hops and the FP16 inner solver's); blk = 0
+ * returns to the autotuner.  Bits 1 and 2 do not change results (reductions
+ * are per tile in a fixed order).  Returns the previous setting
+ * (blk * 16 + order, 0 for the autotuner), -1 and a recorded error for a bad
  * one. */
 int ostp_set_batch_tile(int blk, int order);
 

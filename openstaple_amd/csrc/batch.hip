@@ -314,6 +314,342 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
     }
 }
 
+// ------------------------------------------------ FP64 hops, several systems
+// Systems of a batch that share a gauge field (noise sources on one
+// configuration, the shifts of a rational solve) form chunks of up to NR; a
+// chunk's workgroup computes the FP64 hops of all of them: each link (two
+// rows, the rebuilt third, its phase) is loaded and prepared once per
+// half-site and applied to the NR systems' spinors, with the single-system
+// hop's arithmetic for each (so a system's result does not depend on its
+// chunk).  Links are 768 of the 928 B a half-site's FP64 hop moves, so the
+// link traffic of the chunk's systems drops NR-fold.
+template <bool DAG, int NR>
+__device__ __forceinline__ void hop_mr(const double2* __restrict__ U, const double* __restrict__ P,
+                                       const double2* const (&in)[NR], const bool (&act)[NR], int sizeh, int iu,
+                                       int iv, double2 (&o)[NR][3]) {
+    const double2 a0 = ld_link(U + iu), a1 = ld_link(U + sizeh + iu), a2 = ld_link(U + 2 * sizeh + iu);
+    const double2 b0 = ld_link(U + 3 * sizeh + iu), b1 = ld_link(U + 4 * sizeh + iu), b2 = ld_link(U + 5 * sizeh + iu);
+    const double arg = ld_link(P + iu);
+    double s, c;
+    sincos_t(arg, &s, &c);
+    const double2 phase = DAG ? make_double2(c, -s) : make_double2(c, s);
+    const double2 x0 = csub(cmul(a1, b2), cmul(a2, b1));
+    const double2 x1 = csub(cmul(a2, b0), cmul(a0, b2));
+    const double2 x2 = csub(cmul(a0, b1), cmul(a1, b0));
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        if (!act[r]) continue;
+        const double2* v = in[r];
+        const double2 v0 = cmul(v[iv], phase), v1 = cmul(v[sizeh + iv], phase), v2 = cmul(v[2 * sizeh + iv], phase);
+        if (!DAG) {
+            double2 r0 = cmul(a0, v0); r0 = cfma(a1, v1, r0); r0 = cfma(a2, v2, r0);
+            double2 r1 = cmul(b0, v0); r1 = cfma(b1, v1, r1); r1 = cfma(b2, v2, r1);
+            double2 r2 = cmul(cconj(x0), v0); r2 = cfmac(x1, v1, r2); r2 = cfmac(x2, v2, r2);
+            o[r][0] = cadd(o[r][0], r0); o[r][1] = cadd(o[r][1], r1); o[r][2] = cadd(o[r][2], r2);
+        } else {
+            double2 r0 = cmul(cconj(a0), v0); r0 = cfmac(b0, v1, r0); r0 = cfma(x0, v2, r0);
+            double2 r1 = cmul(cconj(a1), v0); r1 = cfmac(b1, v1, r1); r1 = cfma(x1, v2, r1);
+            double2 r2 = cmul(cconj(a2), v0); r2 = cfmac(b2, v1, r2); r2 = cfma(x2, v2, r2);
+            o[r][0] = csub(o[r][0], r0); o[r][1] = csub(o[r][1], r1); o[r][2] = csub(o[r][2], r2);
+        }
+    }
+}
+
+template <int ROLE, int BLK, int NR>
+__global__ __launch_bounds__(BLK) void kb_dslash_mr(const BSys* __restrict__ sys, const int* __restrict__ chunks,
+                                                    int pred, int order) {
+    static_assert(ROLE == kRoleF64Doe || ROLE == kRoleF64Deo || ROLE == kRoleF64DeoResid, "FP64 roles");
+    constexpr int EO = ROLE == kRoleF64Doe ? 1 : 0;
+    int ci = (int)blockIdx.y, tile = -1;
+    if (order & 4) sys_tile_interleaved(order & 2, ci, tile);
+    int sidx[NR];
+    bool act[NR];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        sidx[r] = chunks[NR * ci + r];
+        act[r] = sidx[r] >= 0 && pred_pass(bctl(sys[sidx[r]], pred));
+        any = any || act[r];
+    }
+    if (!any) return;
+    const BSys& S0 = sys[sidx[0]];
+    const SiteGeom& g = S0.g;
+    const int ntiles = (g.sizeh + BLK - 1) / BLK;
+    if (order & 4) {
+        if (tile >= ntiles) tile = -1;
+    } else {
+        tile = tile_of(ntiles, order & 2);
+    }
+    if (tile < 0) return;
+    const int s3 = g.sizeh / g.nd3;
+    int idx;
+    if ((order & 1) && s3 % BLK == 0) {
+        const int q = tile / g.nd3, tt = tile - q * g.nd3;
+        idx = tt * s3 + q * BLK + (int)threadIdx.x;
+    } else {
+        idx = tile * BLK + (int)threadIdx.x;
+    }
+    double dot[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) dot[r] = 0.0;
+    if (idx < g.sizeh) {
+        const double2* in[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const BSys& S = sys[act[r] ? sidx[r] : sidx[0]];
+            in[r] = EO == 1 ? S.x : S.h64;
+        }
+        double2 o[NR][3];
+#pragma unroll
+        for (int r = 0; r < NR; r++) o[r][0] = o[r][1] = o[r][2] = make_double2(0.0, 0.0);
+        const Nbrs nb = site_neighbours<EO>(g, idx);
+        constexpr int F = EO == 0 ? 0 : 1, B = 1 - F;
+#pragma unroll
+        for (int mu = 0; mu < 4; mu++)
+            hop_mr<true, NR>(S0.u64[2 * mu + B], S0.ph64[2 * mu + B], in, act, g.sizeh, nb.im[mu], nb.im[mu], o);
+#pragma unroll
+        for (int mu = 0; mu < 4; mu++)
+            hop_mr<false, NR>(S0.u64[2 * mu + F], S0.ph64[2 * mu + F], in, act, g.sizeh, idx, nb.ip[mu], o);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (!act[r]) continue;
+            const BSys& S = sys[sidx[r]];
+            double2 o0 = o[r][0], o1 = o[r][1], o2 = o[r][2];
+            o0.x *= 0.5; o0.y *= 0.5; o1.x *= 0.5; o1.y *= 0.5; o2.x *= 0.5; o2.y *= 0.5;
+            if constexpr (EO == 1) {
+                S.h64[idx] = o0;
+                S.h64[g.sizeh + idx] = o1;
+                S.h64[2 * g.sizeh + idx] = o2;
+            } else {
+                const double m2 = S.m2;
+                const double2 x0 = S.x[idx], x1 = S.x[g.sizeh + idx], x2 = S.x[2 * g.sizeh + idx];
+                const double2 sv[3] = {make_double2(x0.x * m2 - o0.x, x0.y * m2 - o0.y),
+                                       make_double2(x1.x * m2 - o1.x, x1.y * m2 - o1.y),
+                                       make_double2(x2.x * m2 - o2.x, x2.y * m2 - o2.y)};
+#pragma unroll
+                for (int c = 0; c < 3; c++) S.s64[(long)c * g.sizeh + idx] = sv[c];
+                if constexpr (ROLE == kRoleF64DeoResid) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        const long k = (long)c * g.sizeh + idx;
+                        const double2 bb = S.b[k];
+                        const double2 rr = make_double2(bb.x - sv[c].x, bb.y - sv[c].y);
+                        S.r64[k] = rr;
+                        dot[r] += rr.x * rr.x + rr.y * rr.y;
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (ROLE == kRoleF64DeoResid) {
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+            if (act[r]) block_partial_at<BLK, 1>(dot[r], 0.0, sys[sidx[r]].partials, tile);
+    }
+}
+
+// ------------------------------------------- FP16 hops, several systems
+// The inner FP16 hops (kRoleH16*/kRoleS16*) by chunks of systems on one gauge
+// field, as kb_dslash_mr does for the FP64 hops: one thread decodes each FP16
+// link record once per half-site and applies it to the chunk's systems, each
+// with its own direction update p' = p gammag + r on the fly, its own scalar
+// epilogue and its own partials, exactly as kb_dslash does for one system.
+template <bool DAG, bool SGN, int NR>
+__device__ __forceinline__ void hop_hv_mr(const h8* __restrict__ LA, const h6* __restrict__ LB,
+                                          const h4* __restrict__ LC, int iu, const h6 (&vin)[NR],
+                                          const bool (&act)[NR], float2 (&o)[NR][3]) {
+    const h8 la = ldg(LA + iu);
+    float2 a0 = h2f(la.v[0]);
+    const float2 a1 = h2f(la.v[1]), a2 = h2f(la.v[2]), b0 = h2f(la.v[3]);
+    float2 b1, b2, ph = make_float2(1.f, 0.f);
+    bool neg = false;
+    if constexpr (SGN) {
+        const h4 lc = ldg(LC + iu);
+        b1 = h2f(lc.v[0]);
+        b2 = h2f(lc.v[1]);
+        const unsigned short bits = __builtin_bit_cast(unsigned short, la.v[0].x);
+        neg = (bits & 0x4000u) != 0;
+        a0.x = (float)__builtin_bit_cast(_Float16, (unsigned short)(bits & 0xbfffu));
+    } else {
+        const h6 lb = ldg(LB + iu);
+        b1 = h2f(lb.v[0]);
+        b2 = h2f(lb.v[1]);
+        const float2 cs = h2f(lb.v[2]);
+        ph = DAG ? make_float2(cs.x, -cs.y) : cs;
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        if (!act[r]) continue;
+        float2 v0, v1, v2;
+        if constexpr (SGN) {
+            const float2 i0 = h2f(vin[r].v[0]), i1 = h2f(vin[r].v[1]), i2 = h2f(vin[r].v[2]);
+            v0 = neg ? make_float2(-i0.x, -i0.y) : i0;
+            v1 = neg ? make_float2(-i1.x, -i1.y) : i1;
+            v2 = neg ? make_float2(-i2.x, -i2.y) : i2;
+        } else {
+            v0 = cmul(h2f(vin[r].v[0]), ph);
+            v1 = cmul(h2f(vin[r].v[1]), ph);
+            v2 = cmul(h2f(vin[r].v[2]), ph);
+        }
+        hop_rows<DAG>(a0, a1, a2, b0, b1, b2, v0, v1, v2, o[r][0], o[r][1], o[r][2]);
+    }
+}
+
+template <int ROLE, int BLK, int NR>
+__global__ __launch_bounds__(BLK) void kb_dslash_h_mr(const BSys* __restrict__ sys, const int* __restrict__ chunks,
+                                                      int pred, int order) {
+    static_assert(ROLE >= kRoleH16Doe, "FP16 roles");
+    constexpr int EOH = (ROLE == kRoleH16Doe || ROLE == kRoleS16Doe) ? 1 : 0;
+    constexpr bool SGN = ROLE >= kRoleS16Doe;
+    int ci = (int)blockIdx.y, tile = -1;
+    if (order & 4) sys_tile_interleaved(order & 2, ci, tile);
+    int sidx[NR];
+    bool act[NR];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        sidx[r] = chunks[NR * ci + r];
+        act[r] = sidx[r] >= 0 && pred_pass(bctl(sys[sidx[r]], pred));
+        any = any || act[r];
+    }
+    if (!any) return;
+    const BSys& S0 = sys[sidx[0]];
+    const SiteGeom& g = S0.g;
+    const int ntiles = (g.sizeh + BLK - 1) / BLK;
+    if (order & 4) {
+        if (tile >= ntiles) tile = -1;
+    } else {
+        tile = tile_of(ntiles, order & 2);
+    }
+    if (tile < 0) return;
+    const int s3 = g.sizeh / g.nd3;
+    int idx;
+    if ((order & 1) && s3 % BLK == 0) {
+        const int q = tile / g.nd3, tt = tile - q * g.nd3;
+        idx = tt * s3 + q * BLK + (int)threadIdx.x;
+    } else {
+        idx = tile * BLK + (int)threadIdx.x;
+    }
+    // each system's end of the previous iteration (D_oe) or scalar commit
+    // (D_eo), the single-system kernel's code with "stop" clearing act[r]
+    __shared__ double lds[NR][BLK / 64];
+    float gm[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        gm[r] = 0.f;
+        if (!act[r]) continue;
+        const BSys& S = sys[sidx[r]];
+        SolverState* st = S.st;
+        if constexpr (EOH == 1) {
+            if (st->xr_fresh) {
+                const int np = (g.sizeh + kBlock * kXrSites - 1) / (kBlock * kXrSites);
+                double a = 0.0;
+                for (int j = threadIdx.x; j < np; j += BLK) a += S.partials[2 * j + 1];
+                const double lambda = block_sum_all<BLK>(a, lds[r]);
+                const double delta = st->delta;
+                const int cg1 = st->cg + 1;
+                const double true2 = lambda / (st->scale * st->scale);
+                const bool at_max = cg1 >= st->max_cg;
+                const bool conv = !(sqrt(true2 / st->source_norm) > st->res * st->safety);
+                gm[r] = (float)(lambda / delta);
+                if (tile == 0 && threadIdx.x == 0) {
+                    st->gammag = lambda / delta;
+                    st->lambda = lambda;
+                    if (at_max || conv) {
+                        st->delta_old = delta;
+                        st->delta = lambda;
+                        st->cg = cg1;
+                        st->xr_fresh = 2;
+                        if (at_max) {
+                            st->status = kDone;
+                        } else {
+                            st->ru = 1;
+                            st->hold = 1;
+                        }
+                    }
+                }
+                if (at_max || conv) act[r] = false;
+            } else {
+                gm[r] = (float)st->gammag;
+            }
+        } else {
+            gm[r] = (float)st->gammag;
+            if (tile == 0 && threadIdx.x == 0 && st->xr_fresh) {
+                st->delta_old = st->delta;
+                st->delta = st->lambda;
+                st->cg += 1;
+                st->xr_fresh = 0;
+            }
+        }
+    }
+    any = false;
+#pragma unroll
+    for (int r = 0; r < NR; r++) any = any || act[r];
+    if (!any) return;
+    double dot[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) dot[r] = 0.0;
+    if (idx < g.sizeh) {
+        const BSys* Sr[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) Sr[r] = &sys[act[r] ? sidx[r] : sidx[0]];
+        float2 o[NR][3];
+#pragma unroll
+        for (int r = 0; r < NR; r++) o[r][0] = o[r][1] = o[r][2] = make_float2(0.f, 0.f);
+        const Nbrs nb = site_neighbours<EOH>(g, idx);
+        constexpr int F = EOH == 0 ? 0 : 1, B = 1 - F;
+        h6 vin[NR];
+#pragma unroll
+        for (int mu = 0; mu < 4; mu++) {
+            const int j = nb.im[mu];
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+                if (act[r])
+                    vin[r] = EOH == 1 ? pnext(ldg(Sr[r]->pph + j), ldg(Sr[r]->rh + j), gm[r]) : ldg(Sr[r]->hh + j);
+            hop_hv_mr<true, SGN, NR>(S0.ula[2 * mu + B], S0.ulb[2 * mu + B], S0.ulc[2 * mu + B], j, vin, act, o);
+        }
+#pragma unroll
+        for (int mu = 0; mu < 4; mu++) {
+            const int j = nb.ip[mu];
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+                if (act[r])
+                    vin[r] = EOH == 1 ? pnext(ldg(Sr[r]->pph + j), ldg(Sr[r]->rh + j), gm[r]) : ldg(Sr[r]->hh + j);
+            hop_hv_mr<false, SGN, NR>(S0.ula[2 * mu + F], S0.ulb[2 * mu + F], S0.ulc[2 * mu + F], idx, vin, act, o);
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (!act[r]) continue;
+            const BSys& S = *Sr[r];
+            float2 o0 = o[r][0], o1 = o[r][1], o2 = o[r][2];
+            o0.x *= 0.5f; o0.y *= 0.5f; o1.x *= 0.5f; o1.y *= 0.5f; o2.x *= 0.5f; o2.y *= 0.5f;
+            if constexpr (EOH == 1) {
+                h6 hv;
+                hv.v[0] = f2h(o0.x, o0.y);
+                hv.v[1] = f2h(o1.x, o1.y);
+                hv.v[2] = f2h(o2.x, o2.y);
+                stg(S.hh + idx, hv);
+            } else {
+                const float m2 = (float)S.m2f;
+                const h6 q = pnext(ldg(S.pph + idx), ldg(S.rh + idx), gm[r]);
+                stg(S.pph + idx, q);
+                const float2 x0 = h2f(q.v[0]), x1 = h2f(q.v[1]), x2 = h2f(q.v[2]);
+                h6 sv;
+                sv.v[0] = f2h(x0.x * m2 - o0.x, x0.y * m2 - o0.y);
+                sv.v[1] = f2h(x1.x * m2 - o1.x, x1.y * m2 - o1.y);
+                sv.v[2] = f2h(x2.x * m2 - o2.x, x2.y * m2 - o2.y);
+                stg(S.sh + idx, sv);
+                dot[r] = re_dot(x0, h2f(sv.v[0])) + re_dot(x1, h2f(sv.v[1])) + re_dot(x2, h2f(sv.v[2]));
+            }
+        }
+    }
+    if constexpr (EOH == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+            if (act[r]) block_partial_at<BLK, 1>(dot[r], 0.0, sys[sidx[r]].partials, tile);
+    }
+}
+
 // Elementwise kernels: one thread per half-site, 3 colours each, kBlock threads.
 #define BATCH_SITE_PROLOGUE(pred)                                     \
     const BSys& S = sys[blockIdx.y];                                  \
@@ -625,9 +961,10 @@ __global__ __launch_bounds__(kBlock) void kb_finalize(const BSys* __restrict__ s
 // the reliable updates are tuned separately
 struct Tile {
     int blk = 256;
-    int order = 1;
+    int order = 1;   // bits 0-2 as in kb_dslash; bit 3: FP64 hops by chunks (kb_dslash_mr)
     float us = 0.f;
 };
+constexpr int kMrSystems = 4;  // systems per chunk of kb_dslash_mr
 struct Tiles {
     Tile inner, f64;
 };
@@ -647,6 +984,9 @@ struct Batch {
     void* arena = nullptr;
     size_t arena_bytes = 0;
     size_t st_cap = 0;
+    int* chunks = nullptr;  // kMrSystems system indices per chunk (-1 padded), in the arena
+    int nchunks = 0;
+    int shared = 0;         // some chunk holds more than one system
 };
 static Batch g_batch;
 static bool g_sgn = false;  // FP16 links of this call carry their +-1 phase as a sign bit
@@ -656,10 +996,13 @@ static bool g_sgn = false;  // FP16 links of this call carry their +-1 phase as 
 // are uploaded by every call; the scalars live in the device states).
 struct GraphKey {
     int nsys = -1, max_sizeh = 0, blk = 0, order = 0, blk64 = 0, order64 = 0, half = 0, sgn = 0, iters = 0;
+    int nchunks = 0;
     const void* dsys = nullptr;
+    const void* chunks = nullptr;
     bool operator==(const GraphKey& o) const {
         return nsys == o.nsys && max_sizeh == o.max_sizeh && blk == o.blk && order == o.order && blk64 == o.blk64 &&
-               order64 == o.order64 && half == o.half && sgn == o.sgn && iters == o.iters && dsys == o.dsys;
+               order64 == o.order64 && half == o.half && sgn == o.sgn && iters == o.iters && dsys == o.dsys &&
+               nchunks == o.nchunks && chunks == o.chunks;
     }
 };
 static GraphKey g_graph_key;
@@ -678,6 +1021,39 @@ static hipStream_t S() { return ctx().stream; }
 template <int ROLE>
 static void launch_role(const Tile& t, int nsys, int max_sizeh, int pred) {
     const int tiles = (max_sizeh + t.blk - 1) / t.blk;
+    if constexpr (ROLE >= kRoleH16Doe) {
+        if (t.order & 8) {
+            const dim3 grid(tiles, g_batch.nchunks);
+            const int o = t.order & 7;
+            const int* ch = g_batch.chunks;
+            if (t.blk == 64)
+                hipLaunchKernelGGL((kb_dslash_h_mr<ROLE, 64, kMrSystems>), grid, dim3(64), 0, S(), g_batch.dsys, ch, pred,
+                                   o);
+            else if (t.blk == 128)
+                hipLaunchKernelGGL((kb_dslash_h_mr<ROLE, 128, kMrSystems>), grid, dim3(128), 0, S(), g_batch.dsys, ch,
+                                   pred, o);
+            else
+                hipLaunchKernelGGL((kb_dslash_h_mr<ROLE, 256, kMrSystems>), grid, dim3(256), 0, S(), g_batch.dsys, ch,
+                                   pred, o);
+            return;
+        }
+    }
+    if constexpr (ROLE == kRoleF64Doe || ROLE == kRoleF64Deo || ROLE == kRoleF64DeoResid) {
+        if (t.order & 8) {
+            const dim3 grid(tiles, g_batch.nchunks);
+            const int o = t.order & 7;
+            const int* ch = g_batch.chunks;
+            if (t.blk == 64)
+                hipLaunchKernelGGL((kb_dslash_mr<ROLE, 64, kMrSystems>), grid, dim3(64), 0, S(), g_batch.dsys, ch, pred, o);
+            else if (t.blk == 128)
+                hipLaunchKernelGGL((kb_dslash_mr<ROLE, 128, kMrSystems>), grid, dim3(128), 0, S(), g_batch.dsys, ch, pred,
+                                   o);
+            else
+                hipLaunchKernelGGL((kb_dslash_mr<ROLE, 256, kMrSystems>), grid, dim3(256), 0, S(), g_batch.dsys, ch, pred,
+                                   o);
+            return;
+        }
+    }
     if (t.blk == 64)
         hipLaunchKernelGGL((kb_dslash<ROLE, 64>), dim3(tiles, nsys), dim3(64), 0, S(), g_batch.dsys, pred, t.order);
     else if (t.blk == 128)
@@ -734,6 +1110,7 @@ static Tiles autotune(const std::string& sig, int nsys, int max_sizeh, bool half
     if (g_forced_tile.blk) {
         Tiles t;
         t.inner = t.f64 = g_forced_tile;
+        if (!half) t.inner.order &= 7;  // chunked inner hops: FP16 only
         return t;
     }
     auto it = g_tuned.find(sig);
@@ -743,14 +1120,19 @@ static Tiles autotune(const std::string& sig, int nsys, int max_sizeh, bool half
     best.inner.us = best.f64.us = 1e30f;
     std::string rep = "batch " + sig + ":";
     if (!hip_ok(hipEventCreate(&e0), "event") || !hip_ok(hipEventCreate(&e1), "event")) return Tiles{};
-    static const char* const kOrder[8] = {"contiguous", "t-fastest", "contiguous hw-order", "t-fastest hw-order",
-                                          "contiguous interleaved", "t-fastest interleaved",
-                                          "contiguous hw-order interleaved", "t-fastest hw-order interleaved"};
+    static const char* const kOrder[16] = {
+        "contiguous", "t-fastest", "contiguous hw-order", "t-fastest hw-order",
+        "contiguous interleaved", "t-fastest interleaved", "contiguous hw-order interleaved",
+        "t-fastest hw-order interleaved", "chunks contiguous", "chunks t-fastest", "chunks contiguous hw-order",
+        "chunks t-fastest hw-order", "chunks contiguous interleaved", "chunks t-fastest interleaved",
+        "chunks contiguous hw-order interleaved", "chunks t-fastest hw-order interleaved"};
     for (int fam = 0; fam < 2; fam++) {
         rep += fam == 0 ? " inner D_oe:" : " fp64 D_oe:";
         Tile& b = fam == 0 ? best.inner : best.f64;
+        // chunked hops: FP64, and the FP16 inner solver's (not the FP32 one's)
+        const int norders = (g_batch.shared && (fam == 1 || half)) ? 16 : 8;
         for (int blk : {256, 128, 64})
-            for (int order = 0; order < 8; order++) {
+            for (int order = 0; order < norders; order++) {
                 Tile c;
                 c.blk = blk;
                 c.order = order;
@@ -824,6 +1206,7 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
     }
     const size_t part_per = (size_t)2 * max_t64;
     arena += (sizeof(double) * part_per + 256) * nsys;
+    arena += sizeof(int) * kMrSystems * (size_t)nsys + 256;  // chunk table
     Batch& B = g_batch;
     if (arena > B.arena_bytes) {
         if (B.arena) (void)hipFree(B.arena);
@@ -895,6 +1278,35 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
         b.partials = nullptr;
     }
     for (int k = 0; k < nsys; k++) hs[k].partials = (double*)take(sizeof(double) * part_per);
+    {
+        // chunks of up to kMrSystems systems on one gauge field (same FP64 links,
+        // phases and extents), in system order
+        std::vector<int> tab;
+        std::vector<char> done(nsys, 0);
+        int shared = 0;
+        for (int k = 0; k < nsys; k++) {
+            if (done[k]) continue;
+            std::vector<int> grp;
+            for (int j = k; j < nsys; j++)
+                if (!done[j] && hs[j].u64[0] == hs[k].u64[0] && hs[j].ph64[0] == hs[k].ph64[0] &&
+                    hs[j].g.nd0 == hs[k].g.nd0 && hs[j].g.nd1 == hs[k].g.nd1 && hs[j].g.nd2 == hs[k].g.nd2 &&
+                    hs[j].g.nd3 == hs[k].g.nd3) {
+                    grp.push_back(j);
+                    done[j] = 1;
+                }
+            for (size_t a = 0; a < grp.size(); a += kMrSystems) {
+                for (int r = 0; r < kMrSystems; r++)
+                    tab.push_back(a + r < grp.size() ? grp[a + r] : -1);
+                if (grp.size() - a > 1) shared = 1;
+            }
+        }
+        B.nchunks = (int)(tab.size() / kMrSystems);
+        B.shared = shared;
+        B.chunks = (int*)take(sizeof(int) * tab.size());
+        if (!hip_ok(hipMemcpyAsync(B.chunks, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice, S()),
+                    "chunk table"))
+            return 0;
+    }
     if (half) {
         // half vectors, and one half copy of each distinct gauge field (systems sharing links share it)
         std::map<std::pair<const void*, const void*>, int> seen;
@@ -1039,6 +1451,8 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
         key.sgn = g_sgn ? 1 : 0;
         key.iters = iters_per_graph;
         key.dsys = B.dsys;
+        key.nchunks = B.nchunks;
+        key.chunks = B.chunks;
         if (!(g_graph_exec && key == g_graph_key)) {
             // the stream is idle here (the states were pulled), so no replay of the old graph runs
             drop_graph();
@@ -1174,12 +1588,12 @@ int ostp_batched_half_solve(int nsys, const ostp_batch_system* sys, double res, 
 }
 
 int ostp_set_batch_tile(int blk, int order) {
-    if (blk != 0 && ((blk != 64 && blk != 128 && blk != 256) || order < 0 || order > 7)) {
-        fail("ostp_set_batch_tile: workgroup %d / order %d (64, 128 or 256 and 0..7, or 0 for the autotuner)", blk,
+    if (blk != 0 && ((blk != 64 && blk != 128 && blk != 256) || order < 0 || order > 15)) {
+        fail("ostp_set_batch_tile: workgroup %d / order %d (64, 128 or 256 and 0..15, or 0 for the autotuner)", blk,
              order);
         return -1;
     }
-    const int prev = g_forced_tile.blk ? g_forced_tile.blk * 8 + g_forced_tile.order : 0;
+    const int prev = g_forced_tile.blk ? g_forced_tile.blk * 16 + g_forced_tile.order : 0;
     g_forced_tile.blk = blk;
     g_forced_tile.order = blk ? order : 0;
     return prev;

@@ -472,6 +472,9 @@ __global__ __launch_bounds__(kBlock) void k_add_f_to_d(double2* __restrict__ x, 
 
 // -------------------------------------------------------------- host driver
 static inline hipStream_t S() { return ctx().stream; }
+// iterations per HIP graph of the batched solves run inside the library
+// (rational-solve refinement, chiral condensate); 0: direct launches
+static int g_internal_ipg = 16;
 static inline SolverState& H() { return *ctx().state_host; }
 
 static SolverCtl ctl_of(int pred, int iter = 0) {
@@ -1170,7 +1173,7 @@ int ostp_chiral_condensate(const su3_soa* u, const su3_soa_f* u_f, ferm_param* p
         q.out = xe;
         q.mass = m;
     }
-    const int conv = batched_solve_internal(nvec, sys.data(), res, max_cg, 0.1, 16, cg_return, nullptr, nullptr,
+    const int conv = batched_solve_internal(nvec, sys.data(), res, max_cg, 0.1, g_internal_ipg, cg_return, nullptr, nullptr,
                                             half_inner != 0);
     const double factor = degeneracy * 0.25 / ((double)L.nd[0] * L.nd[1] * L.nd[2] * L.nd[3]);
     bool ok = true;
@@ -1284,6 +1287,16 @@ int inverter_multishift_wrapper(inverter_package ip, ferm_param* pars, RationalA
         total_iterations += cg_return;
     }
     return total_iterations;
+}
+
+int ostp_set_batched_graph_iterations(int n) {
+    if (n < 0 || n > 4096) {
+        fail("ostp_set_batched_graph_iterations: %d outside 0..4096", n);
+        return -1;
+    }
+    const int prev = g_internal_ipg;
+    g_internal_ipg = n;
+    return prev;
 }
 
 int ostp_set_multishift_precision(int mode, int force_only) {
@@ -1467,7 +1480,7 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
         q.out = (vec3_soa*)((char*)out + k * vb);
         q.mass = std::sqrt(m2 + approx->RA_b[k]);
     }
-    const int converged = batched_solve_internal(order, sys.data(), res, max_cg, 0.1, 16, cg_refine, nullptr,
+    const int converged = batched_solve_internal(order, sys.data(), res, max_cg, 0.1, g_internal_ipg, cg_refine, nullptr,
                                                  true_res, half_inner != 0);
     if (verbosity_lv > 1 && hip_ok(hipStreamSynchronize(S()), "timing")) {
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_cgm).count();

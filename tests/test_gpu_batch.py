@@ -113,9 +113,12 @@ def test_batched_sources_and_replicas():
 @pytest.mark.parametrize("half", [False, True])
 def test_tile_choices_agree(half):
     """Every fixed Dslash tile (workgroup 64/128/256; t-fastest, hardware
-    order and system-interleaved orders) solves the batch: three sources on
-    one gauge field (which the interleaved orders share through L2) and a
-    replica on a second lattice.  The workgroup size and the t-fastest bit
+    order and system-interleaved orders; hops of one system or of a chunk of
+    systems on one gauge field per thread, FP64 and FP16) solves the batch:
+    three sources on one gauge field (which the interleaved orders share
+    through L2 and the chunked hops through registers) and a replica on a
+    second lattice.
+    The workgroup size, the t-fastest bit and the chunking
     fix which sites a reduction partial sums; the order in which workgroups
     run (hardware order, systems interleaved) does not, so those variants
     give bit-identical solutions and iteration counts; the others converge
@@ -145,7 +148,7 @@ def test_tile_choices_agree(half):
     cg0, _ = run()
     try:
         for blk in (64, 128, 256):
-            for tfast in (0, 1):
+            for tfast in (0, 1, 8, 9):  # bit 3: FP64 hops of up to four systems per workgroup
                 first = None
                 for order in (tfast, tfast | 2, tfast | 4, tfast | 6):
                     assert lib.ostp_set_batch_tile(blk, order) >= 0
@@ -163,6 +166,9 @@ def test_tile_choices_agree(half):
     finally:
         lib.ostp_set_batch_tile(0, 0)
     assert lib.ostp_set_batch_tile(96, 0) == -1
+    assert lib.ostp_last_error()
+    lib.ostp_clear_error()
+    assert lib.ostp_set_batched_graph_iterations(-1) == -1
     assert lib.ostp_last_error()
     lib.ostp_clear_error()
 
