@@ -964,7 +964,7 @@ struct Tile {
     int order = 1;   // bits 0-2 as in kb_dslash; bit 3: FP64 hops by chunks (kb_dslash_mr)
     float us = 0.f;
 };
-constexpr int kMrSystems = 4;  // systems per chunk of kb_dslash_mr
+constexpr int kMrSystems = 8;  // systems per chunk of kb_dslash_mr
 struct Tiles {
     Tile inner, f64;
 };

@@ -1362,6 +1362,7 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
         !x0 || !h64 || !s64 || !r64)
         return 0;
     const double m2 = pars->ferm_mass * pars->ferm_mass;
+    const auto t_start = std::chrono::steady_clock::now();
 
     // x_k = x_k_f = x0 = x0_f = 0 ; r = p = p_k = (float) b
     if (!launch_d2f(db, bf, n3)) return 0;
@@ -1461,8 +1462,8 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
                            (float2*)pk, (const float2*)hist, n3, order, (const MsWin*)g_msw, ctl_of(kAlways));
     hipLaunchKernelGGL(k_msx_flush, dim3(nb3), dim3(kBlock), 0, S(), fl, n3, ctl_of(kAlways));
     if (!check_launch("flush")) return 0;
-    const auto t_cgm = std::chrono::steady_clock::now();
     if (verbosity_lv > 1 && !hip_ok(hipStreamSynchronize(S()), "timing")) return 0;
+    const auto t_cgm = std::chrono::steady_clock::now();
 
     // every shift goes to the batched solver, which starts from the CG-M
     // solution with the FP64 residual b - (M + shift_k) x_k (one that is already
@@ -1484,7 +1485,11 @@ int ostp_multishift_mixed_solve(inverter_package ip, ferm_param* pars, RationalA
                                                  true_res, half_inner != 0);
     if (verbosity_lv > 1 && hip_ok(hipStreamSynchronize(S()), "timing")) {
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_cgm).count();
-        printf("mixed multishift: refinement of %d shifts %.4f s after the CG-M\n", order, sec);
+        const double sec0 = std::chrono::duration<double>(t_cgm - t_start).count();
+        int rmax = 0;
+        for (int k = 0; cg_refine && k < order; k++) rmax = std::max(rmax, cg_refine[k]);
+        printf("mixed multishift: m %g, CG-M %d iterations %.4f s, refinement of %d shifts (at most %d iterations) "
+               "%.4f s\n", pars->ferm_mass, cg, sec0, order, rmax, sec);
     }
     if (verbosity_lv > 0)
         printf("Terminated mixed multishift: %d CG-M iterations, %d of %d shifts converged\n", cg, converged, order);
