@@ -413,7 +413,7 @@ int ostp_autotune_report(char *buf, int len);
  * threads, order bit 0 t-fastest tiles, bit 1 hardware workgroup order, bit 2
  * systems interleaved (the workgroups of one tile for every system follow
  * each other, so systems on one gauge field share its links through L2),
- * bit 3 chunked hops (one thread applies each link to up to four systems on
+ * bit 3 chunked hops (one thread applies each link to up to eight systems on
  * one gauge field: the FP64 hops and the FP16 inner solver's); blk = 0
  * returns to the autotuner.  Bits 1 and 2 do not change results (reductions
  * are per tile in a fixed order).  Returns the previous setting

@@ -148,7 +148,7 @@ def test_tile_choices_agree(half):
     cg0, _ = run()
     try:
         for blk in (64, 128, 256):
-            for tfast in (0, 1, 8, 9):  # bit 3: FP64 hops of up to four systems per workgroup
+            for tfast in (0, 1, 8, 9):  # bit 3: hops of up to eight systems per thread
                 first = None
                 for order in (tfast, tfast | 2, tfast | 4, tfast | 6):
                     assert lib.ostp_set_batch_tile(blk, order) >= 0
