@@ -323,38 +323,6 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
 // hop's arithmetic for each (so a system's result does not depend on its
 // chunk).  Links are 768 of the 928 B a half-site's FP64 hop moves, so the
 // link traffic of the chunk's systems drops NR-fold.
-template <bool DAG, int NR>
-__device__ __forceinline__ void hop_mr(const double2* __restrict__ U, const double* __restrict__ P,
-                                       const double2* const (&in)[NR], const bool (&act)[NR], int sizeh, int iu,
-                                       int iv, double2 (&o)[NR][3]) {
-    const double2 a0 = ld_link(U + iu), a1 = ld_link(U + sizeh + iu), a2 = ld_link(U + 2 * sizeh + iu);
-    const double2 b0 = ld_link(U + 3 * sizeh + iu), b1 = ld_link(U + 4 * sizeh + iu), b2 = ld_link(U + 5 * sizeh + iu);
-    const double arg = ld_link(P + iu);
-    double s, c;
-    sincos_t(arg, &s, &c);
-    const double2 phase = DAG ? make_double2(c, -s) : make_double2(c, s);
-    const double2 x0 = csub(cmul(a1, b2), cmul(a2, b1));
-    const double2 x1 = csub(cmul(a2, b0), cmul(a0, b2));
-    const double2 x2 = csub(cmul(a0, b1), cmul(a1, b0));
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-        if (!act[r]) continue;
-        const double2* v = in[r];
-        const double2 v0 = cmul(v[iv], phase), v1 = cmul(v[sizeh + iv], phase), v2 = cmul(v[2 * sizeh + iv], phase);
-        if (!DAG) {
-            double2 r0 = cmul(a0, v0); r0 = cfma(a1, v1, r0); r0 = cfma(a2, v2, r0);
-            double2 r1 = cmul(b0, v0); r1 = cfma(b1, v1, r1); r1 = cfma(b2, v2, r1);
-            double2 r2 = cmul(cconj(x0), v0); r2 = cfmac(x1, v1, r2); r2 = cfmac(x2, v2, r2);
-            o[r][0] = cadd(o[r][0], r0); o[r][1] = cadd(o[r][1], r1); o[r][2] = cadd(o[r][2], r2);
-        } else {
-            double2 r0 = cmul(cconj(a0), v0); r0 = cfmac(b0, v1, r0); r0 = cfma(x0, v2, r0);
-            double2 r1 = cmul(cconj(a1), v0); r1 = cfmac(b1, v1, r1); r1 = cfma(x1, v2, r1);
-            double2 r2 = cmul(cconj(a2), v0); r2 = cfmac(b2, v1, r2); r2 = cfma(x2, v2, r2);
-            o[r][0] = csub(o[r][0], r0); o[r][1] = csub(o[r][1], r1); o[r][2] = csub(o[r][2], r2);
-        }
-    }
-}
-
 template <int ROLE, int BLK, int NR>
 __global__ __launch_bounds__(BLK) void kb_dslash_mr(const BSys* __restrict__ sys, const int* __restrict__ chunks,
                                                     int pred, int order) {

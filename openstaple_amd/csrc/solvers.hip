@@ -15,6 +15,7 @@
 // updates are fused (x and r together; the multishift x_k and p_k together
 // with p) without changing what each element computes.
 #include "ostp_blas.h"
+#include "ostp_dslash.h"
 
 #include <algorithm>
 #include <cmath>
@@ -725,6 +726,143 @@ int mixed_solve(const MxBufs& B, double mass, double shift, double res, int max_
     return (std::sqrt(giustoono) <= res) ? INVERTER_SUCCESS : INVERTER_FAILURE;
 }
 
+// ------------------------------------------- per-shift verification (FP64)
+// The check after the CG-M loop (inverter_multishift_full.c:209-225):
+// |b - (M + sigma_k) x_k|^2 for every shift.  The shifts share the operator,
+// so chunks of up to kVerShifts solutions go through the two hops together:
+// each link is loaded once per half-site and applied to every solution of
+// the chunk (hop_mr), instead of one M application per shift.
+constexpr int kVerShifts = 8;
+struct VerArgs {
+    const double2* u[8];
+    const double* ph[8];
+    const double2* x[kVerShifts];
+    double m2[kVerShifts];
+    int n;             // solutions in this chunk
+    double2* h;        // kVerShifts D_oe x_k
+    const double2* b;
+    double* part;      // kVerShifts x gridDim.x block sums of |r_k|^2
+    SiteGeom g;
+};
+
+// EO = 1: h_k = D_oe x_k ; EO = 0: r_k = b - (m2_k x_k - D_eo h_k), block sums of |r_k|^2
+template <int EO>
+__global__ __launch_bounds__(kBlock) void k_ms_verify(VerArgs a) {
+    const int blk = xcd_block_id();
+    const int idx = blk * kBlock + (int)threadIdx.x;
+    const long n3 = 3L * a.g.sizeh;
+    bool act[kVerShifts];
+#pragma unroll
+    for (int r = 0; r < kVerShifts; r++) act[r] = r < a.n;
+    double dot[kVerShifts];
+#pragma unroll
+    for (int r = 0; r < kVerShifts; r++) dot[r] = 0.0;
+    if (idx < a.g.sizeh) {
+        const double2* in[kVerShifts];
+#pragma unroll
+        for (int r = 0; r < kVerShifts; r++) in[r] = EO == 1 ? a.x[act[r] ? r : 0] : a.h + (act[r] ? r : 0) * n3;
+        double2 o[kVerShifts][3];
+#pragma unroll
+        for (int r = 0; r < kVerShifts; r++) o[r][0] = o[r][1] = o[r][2] = make_double2(0.0, 0.0);
+        const Nbrs nb = site_neighbours<EO>(a.g, idx);
+        constexpr int F = EO == 0 ? 0 : 1, B = 1 - F;
+#pragma unroll
+        for (int mu = 0; mu < 4; mu++)
+            hop_mr<true, kVerShifts>(a.u[2 * mu + B], a.ph[2 * mu + B], in, act, a.g.sizeh, nb.im[mu], nb.im[mu], o);
+#pragma unroll
+        for (int mu = 0; mu < 4; mu++)
+            hop_mr<false, kVerShifts>(a.u[2 * mu + F], a.ph[2 * mu + F], in, act, a.g.sizeh, idx, nb.ip[mu], o);
+#pragma unroll
+        for (int r = 0; r < kVerShifts; r++) {
+            if (!act[r]) continue;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double2 oc = make_double2(0.5 * o[r][c].x, 0.5 * o[r][c].y);
+                const long k = (long)c * a.g.sizeh + idx;
+                if constexpr (EO == 1) {
+                    a.h[r * n3 + k] = oc;
+                } else {
+                    const double2 xx = a.x[r][k], bb = a.b[k];
+                    const double2 sv = make_double2(xx.x * a.m2[r] - oc.x, xx.y * a.m2[r] - oc.y);
+                    const double2 rr = make_double2(bb.x - sv.x, bb.y - sv.y);
+                    dot[r] += rr.x * rr.x + rr.y * rr.y;
+                }
+            }
+        }
+    }
+    if constexpr (EO == 0) {
+        __shared__ double lds[kVerShifts][kBlock / 64];
+#pragma unroll
+        for (int r = 0; r < kVerShifts; r++) {
+            if (!act[r]) continue;
+            const double t = block_sum<kBlock>(dot[r], lds[r]);
+            if (threadIdx.x == 0) a.part[(long)r * gridDim.x + blk] = t;
+        }
+    }
+}
+
+// sum of the block partials of solution blockIdx.x, in block order
+__global__ __launch_bounds__(kBlock) void k_ms_verify_sum(const double* __restrict__ part, int nb, double* out) {
+    __shared__ double lds[kBlock / 64];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kBlock) v += part[(long)blockIdx.x * nb + i];
+    const double t = block_sum<kBlock>(v, lds);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+static void* g_ver_buf = nullptr;
+static size_t g_ver_bytes = 0;
+
+// |b - (M + shifts_k) x_k|^2 / sn for the FP64 solutions x_k = out + k * vb
+static bool verify_shifts_f64(const void* u, const void* ph, const void* b, const void* out, int order, double m2,
+                              const double* shifts, double sn, double* g) {
+    const Lattice& L = ctx().lat;
+    const long sizeh = L.sizeh;
+    const size_t vb = L.vec3_bytes();
+    const int nb = blocks_for(sizeh);
+    const size_t need = kVerShifts * vb + ((size_t)kVerShifts * nb + kVerShifts) * sizeof(double) + 256;
+    if (g_ver_bytes < need) {
+        if (g_ver_buf) (void)hipFree(g_ver_buf);
+        g_ver_buf = nullptr;
+        g_ver_bytes = 0;
+        if (!hip_ok(hipMalloc(&g_ver_buf, need), "multishift verification scratch")) return false;
+        g_ver_bytes = need;
+    }
+    VerArgs a;
+    for (int l = 0; l < 8; l++) {
+        a.u[l] = (const double2*)((const char*)u + l * L.su3_bytes());
+        a.ph[l] = (const double*)((const char*)ph + l * L.dsoa_bytes());
+    }
+    a.h = (double2*)g_ver_buf;
+    a.part = (double*)((char*)g_ver_buf + kVerShifts * vb);
+    double* sums = a.part + (size_t)kVerShifts * nb;
+    a.b = (const double2*)b;
+    a.g.nd0 = L.nd[0];
+    a.g.nd0h = L.nd[0] / 2;
+    a.g.nd1 = L.nd[1];
+    a.g.nd2 = L.nd[2];
+    a.g.nd3 = L.nd[3];
+    a.g.sizeh = (int)sizeh;
+    for (int k0 = 0; k0 < order; k0 += kVerShifts) {
+        a.n = std::min(kVerShifts, order - k0);
+        for (int r = 0; r < kVerShifts; r++) {
+            const int k = k0 + (r < a.n ? r : 0);
+            a.x[r] = (const double2*)((const char*)out + k * vb);
+            a.m2[r] = m2 + shifts[k];
+        }
+        hipLaunchKernelGGL(k_ms_verify<1>, dim3(nb), dim3(kBlock), 0, S(), a);
+        hipLaunchKernelGGL(k_ms_verify<0>, dim3(nb), dim3(kBlock), 0, S(), a);
+        hipLaunchKernelGGL(k_ms_verify_sum, dim3(a.n), dim3(kBlock), 0, S(), (const double*)a.part, nb, sums);
+        double h_sums[kVerShifts];
+        if (!check_launch("multishift verification") ||
+            !hip_ok(hipMemcpyAsync(h_sums, sums, sizeof(double) * a.n, hipMemcpyDeviceToHost, S()), "verification") ||
+            !hip_ok(hipStreamSynchronize(S()), "verification"))
+            return false;
+        for (int r = 0; r < a.n; r++) g[k0 + r] = h_sums[r] / sn;
+    }
+    return true;
+}
+
 // CG-M (multishift_invert) on device pointers; R = double or float vectors
 // (the float variant is the sp_ multishift_invert_f: float data, double scalars).
 struct MsBufs {
@@ -857,7 +995,15 @@ int multishift_solve(const MsBufs& B, double mass, int order, const double* shif
 
     // per-shift verification with (M + sigma_k) (inverter_multishift_full.c:212-229)
     int check = 1;
-    for (int k = 0; k < order; k++) {
+    if constexpr (sizeof(R) == 8) {
+        double g[kMaxShifts];
+        if (order > 0 && !verify_shifts_f64(B.u, B.ph, B.b, B.out, order, m2, shifts, sn, g)) return INVERTER_FAILURE;
+        for (int k = 0; k < order; k++) {
+            if (shift_res) shift_res[k] = std::sqrt(g[k]);
+            check *= (g[k] <= 1) ? 1 : 0;
+        }
+    }
+    for (int k = 0; sizeof(R) != 8 && k < order; k++) {
         const double m2k = sizeof(R) == 8 ? m2 + shifts[k] : m2 + (double)(float)shifts[k];
         const void* xk = (const char*)B.out + k * vb;
         if (!launch_mdagm<R>(B.u, B.ph, xk, B.h, B.s, m2k, 1, kFinStore, ctl_of(kAlways))) return INVERTER_FAILURE;
