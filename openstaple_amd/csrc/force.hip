@@ -254,10 +254,8 @@ bool compute_fermion_force(const void* u, void* aux, const void* xs, const void*
     if (order <= 0) return true;
     void* hs = g_hs.get((size_t)order * L.vec3_bytes());
     if (!hs) return false;
-    for (int k = 0; k < order; k++)
-        if (!launch_dslash<double>(1, 0, kFinStore, u, phases, (const char*)xs + k * L.vec3_bytes(),
-                                   (char*)hs + k * L.vec3_bytes(), nullptr, 0.0, SolverCtl{}))
-            return false;
+    // all D_oe x_k, the shifts' solutions in chunks through shared-link hops
+    if (!launch_doe_multi(u, phases, xs, hs, order)) return false;
     return launch_direct_product(aux, xs, hs, ra->RA_a, order);
 }
 

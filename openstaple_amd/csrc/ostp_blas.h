@@ -16,6 +16,9 @@ template <> __host__ __device__ __forceinline__ float2 mkc<float2>(double a, dou
 template <typename R>
 bool launch_dslash(int eo, int mode, int fin, const void* u, const void* ph, const void* in, void* out,
                    const void* x, double m2, const SolverCtl& ctl);
+// D_oe of n FP64 even fields stored one after the other (xs -> hs), the
+// fields sharing each link load (solvers.hip)
+bool launch_doe_multi(const void* u, const void* ph, const void* xs, void* hs, int n);
 template <typename R>
 bool launch_mdagm(const void* u, const void* ph, const void* in, void* temp, void* out, double m2, int mode,
                   int fin, const SolverCtl& ctl);

@@ -813,6 +813,42 @@ __global__ __launch_bounds__(kBlock) void k_ms_verify_sum(const double* __restri
 static void* g_ver_buf = nullptr;
 static size_t g_ver_bytes = 0;
 
+static VerArgs ver_args(const void* u, const void* ph) {
+    const Lattice& L = ctx().lat;
+    VerArgs a;
+    for (int l = 0; l < 8; l++) {
+        a.u[l] = (const double2*)((const char*)u + l * L.su3_bytes());
+        a.ph[l] = (const double*)((const char*)ph + l * L.dsoa_bytes());
+    }
+    a.g.nd0 = L.nd[0];
+    a.g.nd0h = L.nd[0] / 2;
+    a.g.nd1 = L.nd[1];
+    a.g.nd2 = L.nd[2];
+    a.g.nd3 = L.nd[3];
+    a.g.sizeh = (int)L.sizeh;
+    a.b = nullptr;
+    a.part = nullptr;
+    return a;
+}
+
+// hs_k = D_oe xs_k for n FP64 even fields xs_k = xs + k * vb (the fermion
+// force's odd parts), chunks of kVerShifts per launch
+bool launch_doe_multi(const void* u, const void* ph, const void* xs, void* hs, int n) {
+    const Lattice& L = ctx().lat;
+    const size_t vb = L.vec3_bytes();
+    VerArgs a = ver_args(u, ph);
+    for (int k0 = 0; k0 < n; k0 += kVerShifts) {
+        a.n = std::min(kVerShifts, n - k0);
+        for (int r = 0; r < kVerShifts; r++) {
+            a.x[r] = (const double2*)((const char*)xs + (k0 + (r < a.n ? r : 0)) * vb);
+            a.m2[r] = 0.0;
+        }
+        a.h = (double2*)((char*)hs + k0 * vb);
+        hipLaunchKernelGGL(k_ms_verify<1>, dim3(blocks_for(L.sizeh)), dim3(kBlock), 0, S(), a);
+    }
+    return check_launch("k_ms_verify<1>");
+}
+
 // |b - (M + shifts_k) x_k|^2 / sn for the FP64 solutions x_k = out + k * vb
 static bool verify_shifts_f64(const void* u, const void* ph, const void* b, const void* out, int order, double m2,
                               const double* shifts, double sn, double* g) {
@@ -828,21 +864,11 @@ static bool verify_shifts_f64(const void* u, const void* ph, const void* b, cons
         if (!hip_ok(hipMalloc(&g_ver_buf, need), "multishift verification scratch")) return false;
         g_ver_bytes = need;
     }
-    VerArgs a;
-    for (int l = 0; l < 8; l++) {
-        a.u[l] = (const double2*)((const char*)u + l * L.su3_bytes());
-        a.ph[l] = (const double*)((const char*)ph + l * L.dsoa_bytes());
-    }
+    VerArgs a = ver_args(u, ph);
     a.h = (double2*)g_ver_buf;
     a.part = (double*)((char*)g_ver_buf + kVerShifts * vb);
     double* sums = a.part + (size_t)kVerShifts * nb;
     a.b = (const double2*)b;
-    a.g.nd0 = L.nd[0];
-    a.g.nd0h = L.nd[0] / 2;
-    a.g.nd1 = L.nd[1];
-    a.g.nd2 = L.nd[2];
-    a.g.nd3 = L.nd[3];
-    a.g.sizeh = (int)sizeh;
     for (int k0 = 0; k0 < order; k0 += kVerShifts) {
         a.n = std::min(kVerShifts, order - k0);
         for (int r = 0; r < kVerShifts; r++) {
