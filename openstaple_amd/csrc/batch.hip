@@ -1247,8 +1247,8 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
     }
     for (int k = 0; k < nsys; k++) hs[k].partials = (double*)take(sizeof(double) * part_per);
     {
-        // chunks of up to kMrSystems systems on one gauge field (same FP64 links,
-        // phases and extents), in system order
+        // chunks of up to kMrSystems systems on one gauge field (same FP64 links
+        // and link stride, phases and extents), in system order
         std::vector<int> tab;
         std::vector<char> done(nsys, 0);
         int shared = 0;
@@ -1256,7 +1256,8 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
             if (done[k]) continue;
             std::vector<int> grp;
             for (int j = k; j < nsys; j++)
-                if (!done[j] && hs[j].u64[0] == hs[k].u64[0] && hs[j].ph64[0] == hs[k].ph64[0] &&
+                if (!done[j] && hs[j].u64[0] == hs[k].u64[0] && hs[j].u64[1] == hs[k].u64[1] &&
+                    hs[j].ph64[0] == hs[k].ph64[0] &&
                     hs[j].g.nd0 == hs[k].g.nd0 && hs[j].g.nd1 == hs[k].g.nd1 && hs[j].g.nd2 == hs[k].g.nd2 &&
                     hs[j].g.nd3 == hs[k].g.nd3) {
                     grp.push_back(j);
