@@ -414,7 +414,7 @@ int ostp_autotune_report(char *buf, int len);
  * systems interleaved (the workgroups of one tile for every system follow
  * each other, so systems on one gauge field share its links through L2),
  * bit 3 chunked hops (one thread applies each link to up to eight systems on
- * one gauge field: the FP64 hops and the FP16 inner solver's); blk = 0
+ * one gauge field, in every precision); blk = 0
  * returns to the autotuner.  Bits 1 and 2 do not change results (reductions
  * are per tile in a fixed order).  Returns the previous setting
  * (blk * 16 + order, 0 for the autotuner), -1 and a recorded error for a bad

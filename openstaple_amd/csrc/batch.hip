@@ -314,20 +314,22 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
     }
 }
 
-// ------------------------------------------------ FP64 hops, several systems
+// --------------------------------------- FP32/FP64 hops, several systems
 // Systems of a batch that share a gauge field (noise sources on one
 // configuration, the shifts of a rational solve) form chunks of up to NR; a
-// chunk's workgroup computes the FP64 hops of all of them: each link (two
-// rows, the rebuilt third, its phase) is loaded and prepared once per
-// half-site and applied to the NR systems' spinors, with the single-system
-// hop's arithmetic for each (so a system's result does not depend on its
-// chunk).  Links are 768 of the 928 B a half-site's FP64 hop moves, so the
+// chunk's workgroup computes the hops of all of them (the FP32 inner hops or
+// the FP64 ones): each link (two rows, the rebuilt third, its phase) is
+// loaded and prepared once per half-site and applied to the NR systems'
+// spinors, with the single-system hop's arithmetic for each.  Links are 768
+// of the 928 B a half-site's FP64 hop moves (416 of 512 in FP32), so the
 // link traffic of the chunk's systems drops NR-fold.
 template <int ROLE, int BLK, int NR>
 __global__ __launch_bounds__(BLK) void kb_dslash_mr(const BSys* __restrict__ sys, const int* __restrict__ chunks,
                                                     int pred, int order) {
-    static_assert(ROLE == kRoleF64Doe || ROLE == kRoleF64Deo || ROLE == kRoleF64DeoResid, "FP64 roles");
-    constexpr int EO = ROLE == kRoleF64Doe ? 1 : 0;
+    static_assert(ROLE <= kRoleF64DeoResid, "FP32 and FP64 roles");
+    using R = typename std::conditional<(ROLE <= kRoleF32DeoDot), float, double>::type;
+    using C = typename Cplx<R>::T;
+    constexpr int EO = (ROLE == kRoleF32Doe || ROLE == kRoleF64Doe) ? 1 : 0;
     int ci = (int)blockIdx.y, tile = -1;
     if (order & 4) sys_tile_interleaved(order & 2, ci, tile);
     int sidx[NR];
@@ -361,47 +363,64 @@ __global__ __launch_bounds__(BLK) void kb_dslash_mr(const BSys* __restrict__ sys
 #pragma unroll
     for (int r = 0; r < NR; r++) dot[r] = 0.0;
     if (idx < g.sizeh) {
-        const double2* in[NR];
+        const C* in[NR];
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const BSys& S = sys[act[r] ? sidx[r] : sidx[0]];
-            in[r] = EO == 1 ? S.x : S.h64;
+            if constexpr (ROLE == kRoleF32Doe) in[r] = S.pf;
+            else if constexpr (ROLE == kRoleF32DeoDot) in[r] = S.hf;
+            else if constexpr (ROLE == kRoleF64Doe) in[r] = S.x;
+            else in[r] = S.h64;
         }
-        double2 o[NR][3];
+        const C* const* u;
+        const R* const* ph;
+        if constexpr (sizeof(R) == 4) {
+            u = S0.u32;
+            ph = S0.ph32;
+        } else {
+            u = S0.u64;
+            ph = S0.ph64;
+        }
+        C o[NR][3];
 #pragma unroll
-        for (int r = 0; r < NR; r++) o[r][0] = o[r][1] = o[r][2] = make_double2(0.0, 0.0);
+        for (int r = 0; r < NR; r++) o[r][0] = o[r][1] = o[r][2] = mk<C>(0.0, 0.0);
         const Nbrs nb = site_neighbours<EO>(g, idx);
         constexpr int F = EO == 0 ? 0 : 1, B = 1 - F;
 #pragma unroll
         for (int mu = 0; mu < 4; mu++)
-            hop_mr<true, NR>(S0.u64[2 * mu + B], S0.ph64[2 * mu + B], in, act, g.sizeh, nb.im[mu], nb.im[mu], o);
+            hop_mr<R, true, NR>(u[2 * mu + B], ph[2 * mu + B], in, act, g.sizeh, nb.im[mu], nb.im[mu], o);
 #pragma unroll
         for (int mu = 0; mu < 4; mu++)
-            hop_mr<false, NR>(S0.u64[2 * mu + F], S0.ph64[2 * mu + F], in, act, g.sizeh, idx, nb.ip[mu], o);
+            hop_mr<R, false, NR>(u[2 * mu + F], ph[2 * mu + F], in, act, g.sizeh, idx, nb.ip[mu], o);
+        const R half = (R)0.5;
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             if (!act[r]) continue;
             const BSys& S = sys[sidx[r]];
-            double2 o0 = o[r][0], o1 = o[r][1], o2 = o[r][2];
-            o0.x *= 0.5; o0.y *= 0.5; o1.x *= 0.5; o1.y *= 0.5; o2.x *= 0.5; o2.y *= 0.5;
+            C o0 = o[r][0], o1 = o[r][1], o2 = o[r][2];
+            o0.x *= half; o0.y *= half; o1.x *= half; o1.y *= half; o2.x *= half; o2.y *= half;
             if constexpr (EO == 1) {
-                S.h64[idx] = o0;
-                S.h64[g.sizeh + idx] = o1;
-                S.h64[2 * g.sizeh + idx] = o2;
+                C* out = ROLE == kRoleF32Doe ? (C*)S.hf : (C*)S.h64;
+                out[idx] = o0;
+                out[g.sizeh + idx] = o1;
+                out[2 * g.sizeh + idx] = o2;
             } else {
-                const double m2 = S.m2;
-                const double2 x0 = S.x[idx], x1 = S.x[g.sizeh + idx], x2 = S.x[2 * g.sizeh + idx];
-                const double2 sv[3] = {make_double2(x0.x * m2 - o0.x, x0.y * m2 - o0.y),
-                                       make_double2(x1.x * m2 - o1.x, x1.y * m2 - o1.y),
-                                       make_double2(x2.x * m2 - o2.x, x2.y * m2 - o2.y)};
+                const C* x = ROLE == kRoleF32DeoDot ? (const C*)S.pf : (const C*)S.x;
+                C* out = ROLE == kRoleF32DeoDot ? (C*)S.sf : (C*)S.s64;
+                const double m2 = ROLE == kRoleF32DeoDot ? S.m2f : S.m2;
+                const C x0 = x[idx], x1 = x[g.sizeh + idx], x2 = x[2 * g.sizeh + idx];
+                const C sv[3] = {mk<C>((double)x0.x * m2 - (double)o0.x, (double)x0.y * m2 - (double)o0.y),
+                                 mk<C>((double)x1.x * m2 - (double)o1.x, (double)x1.y * m2 - (double)o1.y),
+                                 mk<C>((double)x2.x * m2 - (double)o2.x, (double)x2.y * m2 - (double)o2.y)};
 #pragma unroll
-                for (int c = 0; c < 3; c++) S.s64[(long)c * g.sizeh + idx] = sv[c];
+                for (int c = 0; c < 3; c++) out[(long)c * g.sizeh + idx] = sv[c];
+                if constexpr (ROLE == kRoleF32DeoDot) dot[r] = re_dot(x0, sv[0]) + re_dot(x1, sv[1]) + re_dot(x2, sv[2]);
                 if constexpr (ROLE == kRoleF64DeoResid) {
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
                         const long k = (long)c * g.sizeh + idx;
                         const double2 bb = S.b[k];
-                        const double2 rr = make_double2(bb.x - sv[c].x, bb.y - sv[c].y);
+                        const double2 rr = make_double2(bb.x - (double)sv[c].x, bb.y - (double)sv[c].y);
                         S.r64[k] = rr;
                         dot[r] += rr.x * rr.x + rr.y * rr.y;
                     }
@@ -409,7 +428,7 @@ __global__ __launch_bounds__(BLK) void kb_dslash_mr(const BSys* __restrict__ sys
             }
         }
     }
-    if constexpr (ROLE == kRoleF64DeoResid) {
+    if constexpr (ROLE == kRoleF32DeoDot || ROLE == kRoleF64DeoResid) {
 #pragma unroll
         for (int r = 0; r < NR; r++)
             if (act[r]) block_partial_at<BLK, 1>(dot[r], 0.0, sys[sidx[r]].partials, tile);
@@ -1006,7 +1025,7 @@ static void launch_role(const Tile& t, int nsys, int max_sizeh, int pred) {
             return;
         }
     }
-    if constexpr (ROLE == kRoleF64Doe || ROLE == kRoleF64Deo || ROLE == kRoleF64DeoResid) {
+    if constexpr (ROLE <= kRoleF64DeoResid) {
         if (t.order & 8) {
             const dim3 grid(tiles, g_batch.nchunks);
             const int o = t.order & 7;
@@ -1078,7 +1097,6 @@ static Tiles autotune(const std::string& sig, int nsys, int max_sizeh, bool half
     if (g_forced_tile.blk) {
         Tiles t;
         t.inner = t.f64 = g_forced_tile;
-        if (!half) t.inner.order &= 7;  // chunked inner hops: FP16 only
         return t;
     }
     auto it = g_tuned.find(sig);
@@ -1097,8 +1115,7 @@ static Tiles autotune(const std::string& sig, int nsys, int max_sizeh, bool half
     for (int fam = 0; fam < 2; fam++) {
         rep += fam == 0 ? " inner D_oe:" : " fp64 D_oe:";
         Tile& b = fam == 0 ? best.inner : best.f64;
-        // chunked hops: FP64, and the FP16 inner solver's (not the FP32 one's)
-        const int norders = (g_batch.shared && (fam == 1 || half)) ? 16 : 8;
+        const int norders = g_batch.shared ? 16 : 8;  // chunked hops only where systems share a field
         for (int blk : {256, 128, 64})
             for (int order = 0; order < norders; order++) {
                 Tile c;
@@ -1247,8 +1264,8 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
     }
     for (int k = 0; k < nsys; k++) hs[k].partials = (double*)take(sizeof(double) * part_per);
     {
-        // chunks of up to kMrSystems systems on one gauge field (same FP64 links
-        // and link stride, phases and extents), in system order
+        // chunks of up to kMrSystems systems on one gauge field (same FP64 and
+        // FP32 links and link strides, phases and extents), in system order
         std::vector<int> tab;
         std::vector<char> done(nsys, 0);
         int shared = 0;
@@ -1257,7 +1274,8 @@ static int batched_solve(int nsys, const ostp_batch_system* sys, double res, int
             std::vector<int> grp;
             for (int j = k; j < nsys; j++)
                 if (!done[j] && hs[j].u64[0] == hs[k].u64[0] && hs[j].u64[1] == hs[k].u64[1] &&
-                    hs[j].ph64[0] == hs[k].ph64[0] &&
+                    hs[j].ph64[0] == hs[k].ph64[0] && hs[j].u32[0] == hs[k].u32[0] &&
+                    hs[j].u32[1] == hs[k].u32[1] && hs[j].ph32[0] == hs[k].ph32[0] &&
                     hs[j].g.nd0 == hs[k].g.nd0 && hs[j].g.nd1 == hs[k].g.nd1 && hs[j].g.nd2 == hs[k].g.nd2 &&
                     hs[j].g.nd3 == hs[k].g.nd3) {
                     grp.push_back(j);

@@ -128,36 +128,37 @@ __device__ __forceinline__ void dslash_site(const typename Cplx<R>::T* const* u,
     o0.x *= half; o0.y *= half; o1.x *= half; o1.y *= half; o2.x *= half; o2.y *= half;
 }
 
-// FP64 hop of several right-hand sides on one gauge field: the link (two
-// rows, the rebuilt third, its phase) is loaded and prepared once and applied
-// to each active system's spinor with the arithmetic of hop<double, DAG>.
-template <bool DAG, int NR>
-__device__ __forceinline__ void hop_mr(const double2* __restrict__ U, const double* __restrict__ P,
-                                       const double2* const (&in)[NR], const bool (&act)[NR], int sizeh, int iu,
-                                       int iv, double2 (&o)[NR][3]) {
-    const double2 a0 = ld_link(U + iu), a1 = ld_link(U + sizeh + iu), a2 = ld_link(U + 2 * sizeh + iu);
-    const double2 b0 = ld_link(U + 3 * sizeh + iu), b1 = ld_link(U + 4 * sizeh + iu), b2 = ld_link(U + 5 * sizeh + iu);
-    const double arg = ld_link(P + iu);
-    double s, c;
+// Hop of several right-hand sides on one gauge field: the link (two rows,
+// the rebuilt third, its phase) is loaded and prepared once and applied to
+// each active system's spinor with the arithmetic of hop<R, DAG>.
+template <typename R, bool DAG, int NR>
+__device__ __forceinline__ void hop_mr(const typename Cplx<R>::T* __restrict__ U, const R* __restrict__ P,
+                                       const typename Cplx<R>::T* const (&in)[NR], const bool (&act)[NR], int sizeh,
+                                       int iu, int iv, typename Cplx<R>::T (&o)[NR][3]) {
+    using C = typename Cplx<R>::T;
+    const C a0 = ld_link(U + iu), a1 = ld_link(U + sizeh + iu), a2 = ld_link(U + 2 * sizeh + iu);
+    const C b0 = ld_link(U + 3 * sizeh + iu), b1 = ld_link(U + 4 * sizeh + iu), b2 = ld_link(U + 5 * sizeh + iu);
+    const R arg = ld_link(P + iu);
+    R s, c;
     sincos_t(arg, &s, &c);
-    const double2 phase = DAG ? make_double2(c, -s) : make_double2(c, s);
-    const double2 x0 = csub(cmul(a1, b2), cmul(a2, b1));
-    const double2 x1 = csub(cmul(a2, b0), cmul(a0, b2));
-    const double2 x2 = csub(cmul(a0, b1), cmul(a1, b0));
+    const C phase = DAG ? mk<C>(c, -s) : mk<C>(c, s);
+    const C x0 = csub(cmul(a1, b2), cmul(a2, b1));
+    const C x1 = csub(cmul(a2, b0), cmul(a0, b2));
+    const C x2 = csub(cmul(a0, b1), cmul(a1, b0));
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         if (!act[r]) continue;
-        const double2* v = in[r];
-        const double2 v0 = cmul(v[iv], phase), v1 = cmul(v[sizeh + iv], phase), v2 = cmul(v[2 * sizeh + iv], phase);
+        const C* v = in[r];
+        const C v0 = cmul(v[iv], phase), v1 = cmul(v[sizeh + iv], phase), v2 = cmul(v[2 * sizeh + iv], phase);
         if (!DAG) {
-            double2 r0 = cmul(a0, v0); r0 = cfma(a1, v1, r0); r0 = cfma(a2, v2, r0);
-            double2 r1 = cmul(b0, v0); r1 = cfma(b1, v1, r1); r1 = cfma(b2, v2, r1);
-            double2 r2 = cmul(cconj(x0), v0); r2 = cfmac(x1, v1, r2); r2 = cfmac(x2, v2, r2);
+            C r0 = cmul(a0, v0); r0 = cfma(a1, v1, r0); r0 = cfma(a2, v2, r0);
+            C r1 = cmul(b0, v0); r1 = cfma(b1, v1, r1); r1 = cfma(b2, v2, r1);
+            C r2 = cmul(cconj(x0), v0); r2 = cfmac(x1, v1, r2); r2 = cfmac(x2, v2, r2);
             o[r][0] = cadd(o[r][0], r0); o[r][1] = cadd(o[r][1], r1); o[r][2] = cadd(o[r][2], r2);
         } else {
-            double2 r0 = cmul(cconj(a0), v0); r0 = cfmac(b0, v1, r0); r0 = cfma(x0, v2, r0);
-            double2 r1 = cmul(cconj(a1), v0); r1 = cfmac(b1, v1, r1); r1 = cfma(x1, v2, r1);
-            double2 r2 = cmul(cconj(a2), v0); r2 = cfmac(b2, v1, r2); r2 = cfma(x2, v2, r2);
+            C r0 = cmul(cconj(a0), v0); r0 = cfmac(b0, v1, r0); r0 = cfma(x0, v2, r0);
+            C r1 = cmul(cconj(a1), v0); r1 = cfmac(b1, v1, r1); r1 = cfma(x1, v2, r1);
+            C r2 = cmul(cconj(a2), v0); r2 = cfmac(b2, v1, r2); r2 = cfma(x2, v2, r2);
             o[r][0] = csub(o[r][0], r0); o[r][1] = csub(o[r][1], r1); o[r][2] = csub(o[r][2], r2);
         }
     }

@@ -768,10 +768,10 @@ __global__ __launch_bounds__(kBlock) void k_ms_verify(VerArgs a) {
         constexpr int F = EO == 0 ? 0 : 1, B = 1 - F;
 #pragma unroll
         for (int mu = 0; mu < 4; mu++)
-            hop_mr<true, kVerShifts>(a.u[2 * mu + B], a.ph[2 * mu + B], in, act, a.g.sizeh, nb.im[mu], nb.im[mu], o);
+            hop_mr<double, true, kVerShifts>(a.u[2 * mu + B], a.ph[2 * mu + B], in, act, a.g.sizeh, nb.im[mu], nb.im[mu], o);
 #pragma unroll
         for (int mu = 0; mu < 4; mu++)
-            hop_mr<false, kVerShifts>(a.u[2 * mu + F], a.ph[2 * mu + F], in, act, a.g.sizeh, idx, nb.ip[mu], o);
+            hop_mr<double, false, kVerShifts>(a.u[2 * mu + F], a.ph[2 * mu + F], in, act, a.g.sizeh, idx, nb.ip[mu], o);
 #pragma unroll
         for (int r = 0; r < kVerShifts; r++) {
             if (!act[r]) continue;
