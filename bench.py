@@ -81,8 +81,8 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
-                                   "(t-slab halo exchange, see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this library runs on one GPU; sharding over GPUs (t-slabs with a "
+                                   "one-site spinor halo per Dslash) is not built", "n_gpus": args.gpus}))
         sys.exit(2)
     # the CPU baseline's builds run make: before this process touches the GPU
     cpu_libs = None

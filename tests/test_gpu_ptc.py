@@ -203,6 +203,12 @@ def test_fcg_mixed_reaches_fp64_residual(s8):
     # a start already within the tolerance runs no pass
     st, its2, outer2, fb2, tr2 = learned.solve_mixed(s8.op, p, x.p, db.p, 1e-10)
     assert st == 1 and its2 == 0 and outer2 == 0 and tr2 <= 1e-10
+    # stopped after 5 FP32 iterations: the plain-CG fallback finishes from the iterate
+    x.put(np.zeros_like(b))
+    st, its3, outer3, fb3, tr3 = learned.solve_mixed(s8.op, p, x.p, db.p, 1e-10, max_cg=5)
+    assert st == 1 and its3 == 5 and fb3 > 0 and tr3 <= 1e-10
+    r = b - o.mdagm(s8.u, x.get(b), s8.ph, s8.mass)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
     # an inner tolerance outside (0, 1) is an error
     assert lib.ostp_fcg_mixed_solve(s8.op.u, C.byref(s8.op.fp), s8.op.u_f, np.zeros(ost.PTC_NPARAM,
                                     np.float32).ctypes.data, x.p, db.p, 1e-10, 100, 1.0, C.byref(C.c_int()),
