@@ -406,6 +406,16 @@ int ostp_set_multishift_precision(int mode, int force_only);
  * follow graph launches).  Returns the previous value, -1 and a recorded
  * error outside 0..4096. */
 int ostp_set_batched_graph_iterations(int n);
+/* No reference counterpart.  fermion_force_soloopenacc solves the rational
+ * systems of all its (flavour, pseudofermion) sources before forming the
+ * forces; when they are FP64 solves (no reduced-precision or single-precision
+ * accelerated mode) on one operator up to the mass (the same U(1) phases:
+ * one array, or equal values), up to four run as one CG-M each whose
+ * iterations share the two hops (each link applied to every source's
+ * spinor).  Each source's iterations, solutions and convergence are those
+ * of its own multishift_invert.  on = 0 solves them one by one; returns the
+ * previous setting (default 1). */
+int ostp_set_multishift_batching(int on);
 /* Text of the autotuner's measurements so far (candidates and choice per
  * batch signature); returns its length, copies up to len-1 bytes into buf. */
 int ostp_autotune_report(char *buf, int len);

@@ -365,6 +365,7 @@ _SIGS = {
                                       C.POINTER(_D)]),
     "ostp_batched_half_solve": (_I, [_I, C.POINTER(BatchSystem), _D, _I, _D, _I, C.POINTER(_I), C.POINTER(_I),
                                      C.POINTER(_D)]),
+    "ostp_set_multishift_batching": (_I, [_I]),
     "ostp_set_batched_graph_iterations": (_I, [_I]),
     "ostp_set_batch_tile": (_I, [_I, _I]),
     "ostp_autotune_report": (_I, [C.c_char_p, _I]),

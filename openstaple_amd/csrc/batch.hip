@@ -314,6 +314,37 @@ __global__ __launch_bounds__(BLK) void kb_dslash(const BSys* __restrict__ sys, i
     }
 }
 
+// Block sums of one value per system of a chunk into each active system's
+// partial slot: every wave reduces all NR values, one barrier, thread 0
+// adds the wave sums in wave order (the order of block_partial_at, so a
+// system's partial does not depend on its chunk).  One LDS row per system:
+// calling block_partial_at once per system would let a wave overwrite a
+// wave sum of the previous system before thread 0 has read it.
+template <int BLK, int NR>
+__device__ __forceinline__ void chunk_partials(const double (&v)[NR], const bool (&act)[NR], const int (&sidx)[NR],
+                                               const BSys* __restrict__ sys, int slot) {
+    __shared__ double lds[NR][BLK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        double x = v[r];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) lds[r][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (!act[r]) continue;
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < BLK / 64; w++) t += lds[r][w];
+            sys[sidx[r]].partials[2 * slot] = t;
+        }
+    }
+}
+
 // --------------------------------------- FP32/FP64 hops, several systems
 // Systems of a batch that share a gauge field (noise sources on one
 // configuration, the shifts of a rational solve) form chunks of up to NR; a
@@ -428,11 +459,7 @@ __global__ __launch_bounds__(BLK) void kb_dslash_mr(const BSys* __restrict__ sys
             }
         }
     }
-    if constexpr (ROLE == kRoleF32DeoDot || ROLE == kRoleF64DeoResid) {
-#pragma unroll
-        for (int r = 0; r < NR; r++)
-            if (act[r]) block_partial_at<BLK, 1>(dot[r], 0.0, sys[sidx[r]].partials, tile);
-    }
+    if constexpr (ROLE == kRoleF32DeoDot || ROLE == kRoleF64DeoResid) chunk_partials<BLK, NR>(dot, act, sidx, sys, tile);
 }
 
 // ------------------------------------------- FP16 hops, several systems
@@ -630,11 +657,7 @@ __global__ __launch_bounds__(BLK) void kb_dslash_h_mr(const BSys* __restrict__ s
             }
         }
     }
-    if constexpr (EOH == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; r++)
-            if (act[r]) block_partial_at<BLK, 1>(dot[r], 0.0, sys[sidx[r]].partials, tile);
-    }
+    if constexpr (EOH == 0) chunk_partials<BLK, NR>(dot, act, sidx, sys, tile);
 }
 
 // Elementwise kernels: one thread per half-site, 3 colours each, kBlock threads.

@@ -14,8 +14,10 @@
 #include "ostp_blas.h"
 #include "ostp_su3.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <vector>
 
 namespace ostp {
 
@@ -245,7 +247,11 @@ struct Scratch {
         return p;
     }
 };
-static Scratch g_hs, g_th, g_ta, g_conf_f;
+static Scratch g_hs, g_th, g_ta, g_conf_f, g_xs_all;
+
+// the rational solves of several sources on one operator, together (solvers.hip)
+bool multishift_fp64_many(const void* u, ferm_param* const* pars, const void* const* in, void* const* out, int n,
+                          double res, int max_cg, int convergence_importance);
 
 // sum over shifts of the outer products for one pseudofermion
 bool compute_fermion_force(const void* u, void* aux, const void* xs, const void* phases, const RationalApprox* ra) {
@@ -367,7 +373,60 @@ void fermion_force_soloopenacc(su3_soa* tconf_acc, su3_soa* tstout_conf_acc_arr,
         ostp_convert_double_to_float_gauge((const su3_soa*)conf_to_use, (su3_soa_f*)cf);
         ipt.u_f = (const su3_soa_f*)cf;
     }
-    for (int iflav = 0; iflav < tNDiffFlavs; iflav++) {
+    // every (flavour, pseudofermion) source's rational solve first, all together
+    // when they share the operator (same phases; FP64 solves), then the forces
+    bool together = false;
+    {
+        std::vector<ferm_param*> fps;
+        std::vector<const void*> ins;
+        std::vector<void*> outs;
+        int maxo = 1;
+        for (int iflav = 0; iflav < tNDiffFlavs; iflav++) {
+            ferm_param* fp = &tfermion_parameters[iflav];
+            maxo = std::max(maxo, fp->approx_md.approx_order);
+            for (int ips = 0; ips < fp->number_of_ps; ips++) {
+                fps.push_back(fp);
+                ins.push_back(resolve((const char*)ferm_in_acc + (size_t)(fp->index_of_the_first_ps + ips) *
+                                                                     L.vec3_bytes(),
+                                      L.vec3_bytes(), nm));
+            }
+        }
+        const int n = (int)fps.size();
+        char* xs_all = n >= 2 ? (char*)g_xs_all.get((size_t)n * maxo * L.vec3_bytes()) : nullptr;
+        if (n >= 2 && !xs_all) {
+            (void)hipGetLastError();
+            ostp_clear_error();
+        }
+        if (xs_all) {
+            for (int k = 0; k < n; k++) {
+                if (!ins[k]) return;
+                outs.push_back(xs_all + (size_t)k * maxo * L.vec3_bytes());
+            }
+            together = multishift_fp64_many(conf_to_use, fps.data(), ins.data(), outs.data(), n, res, max_cg,
+                                            CONVERGENCE_NONCRITICAL);
+            if (!together && ostp_last_error()[0]) return;
+        }
+        if (together) {
+            int k = 0;
+            for (int iflav = 0; iflav < tNDiffFlavs; iflav++) {
+                ferm_param* fp = &tfermion_parameters[iflav];
+                if (!hip_ok(hipMemsetAsync(dtaux, 0, gb, s), nm)) return;
+                const void* dph = resolve(fp->phases, 8 * L.dsoa_bytes(), nm);
+                if (!dph) return;
+                for (int ips = 0; ips < fp->number_of_ps; ips++, k++)
+                    if (!compute_fermion_force(conf_to_use, dtaux, outs[k], dph, &fp->approx_md)) return;
+                if (!launch_backfield_times_force(dtaux, dgl3, dph)) return;
+            }
+            // the shifted solutions of the last source stay in tferm_shiftmulti_acc, as after the loop below
+            const int order = fps[n - 1]->approx_md.approx_order;
+            void* dxs = resolve(tferm_shiftmulti_acc, (size_t)(order > 0 ? order : 1) * L.vec3_bytes(), nm);
+            if (!dxs || (order > 0 && !hip_ok(hipMemcpyAsync(dxs, outs[n - 1], (size_t)order * L.vec3_bytes(),
+                                                             hipMemcpyDeviceToDevice, s),
+                                              nm)))
+                return;
+        }
+    }
+    for (int iflav = 0; !together && iflav < tNDiffFlavs; iflav++) {
         ferm_param* fp = &tfermion_parameters[iflav];
         if (!hip_ok(hipMemsetAsync(dtaux, 0, gb, s), nm)) return;
         const int ifps = fp->index_of_the_first_ps;

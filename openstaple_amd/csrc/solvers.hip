@@ -1064,6 +1064,363 @@ int multishift_solve(const MsBufs& B, double mass, int order, const double* shif
 template int multishift_solve<double>(const MsBufs&, double, int, const double*, double, int, int*, double*);
 template int multishift_solve<float>(const MsBufs&, double, int, const double*, double, int, int*, double*);
 
+// ------------------------------------------- several CG-Ms on one operator
+// Rational solves of several sources whose operators differ only by the mass
+// (the flavours and pseudofermions of an RHMC force on one smeared gauge
+// field, when their U(1) phases agree) advance together: each iteration's
+// two hops run once for all of them, every link applied to each system's
+// spinor (hop_mr, the single-system hop's arithmetic, with k_dslash's tile
+// order and reduction slots), and every other kernel of the iteration runs
+// per system on its own device state, window and partials.  Each system's
+// iterations, solutions and residuals are those of multishift_solve.
+constexpr int kMsMulti = 4;
+struct MsMultiHop {
+    const double2* u[8];
+    const double* ph[8];
+    const double2* in[kMsMulti];
+    double2* out[kMsMulti];
+    const double2* x[kMsMulti];   // EO = 0: out = m2 x - D_eo in, <x, out> partials
+    double m2[kMsMulti];
+    SolverState* st[kMsMulti];
+    double* part[kMsMulti];
+    int n, pred, tmajor, rev;
+    SiteGeom g;
+};
+
+template <int EO>
+__global__ __launch_bounds__(kBlock) void k_ms_multi_hop(MsMultiHop a) {
+    bool act[kMsMulti];
+    int first = -1;
+#pragma unroll
+    for (int r = 0; r < kMsMulti; r++) {
+        SolverCtl c;
+        c.st = a.st[r < a.n ? r : 0];
+        c.partials = a.part[r < a.n ? r : 0];
+        c.pred = a.pred;
+        act[r] = r < a.n && pred_pass(c);
+        if (act[r] && first < 0) first = r;
+    }
+    if (first < 0) return;
+    // k_dslash's workgroup -> tile map (and so its reduction slots)
+    const int blk = xcd_block_id_dir(a.rev);
+    int idx;
+    if (a.tmajor) {
+        const int s3 = a.g.sizeh / a.g.nd3;
+        const int tile = blk / a.g.nd3, tt = blk - tile * a.g.nd3;
+        idx = tt * s3 + tile * kBlock + (int)threadIdx.x;
+    } else {
+        idx = blk * kBlock + (int)threadIdx.x;
+    }
+    double dot[kMsMulti];
+#pragma unroll
+    for (int r = 0; r < kMsMulti; r++) dot[r] = 0.0;
+    if (idx < a.g.sizeh) {
+        const double2* in[kMsMulti];
+#pragma unroll
+        for (int r = 0; r < kMsMulti; r++) in[r] = a.in[act[r] ? r : first];
+        double2 o[kMsMulti][3];
+#pragma unroll
+        for (int r = 0; r < kMsMulti; r++) o[r][0] = o[r][1] = o[r][2] = make_double2(0.0, 0.0);
+        const Nbrs nb = site_neighbours<EO>(a.g, idx);
+        constexpr int F = EO == 0 ? 0 : 1, B = 1 - F;
+#pragma unroll
+        for (int mu = 0; mu < 4; mu++)
+            hop_mr<double, true, kMsMulti>(a.u[2 * mu + B], a.ph[2 * mu + B], in, act, a.g.sizeh, nb.im[mu],
+                                           nb.im[mu], o);
+#pragma unroll
+        for (int mu = 0; mu < 4; mu++)
+            hop_mr<double, false, kMsMulti>(a.u[2 * mu + F], a.ph[2 * mu + F], in, act, a.g.sizeh, idx, nb.ip[mu],
+                                            o);
+        const long sh = a.g.sizeh;
+#pragma unroll
+        for (int r = 0; r < kMsMulti; r++) {
+            if (!act[r]) continue;
+            double2 o0 = o[r][0], o1 = o[r][1], o2 = o[r][2];
+            const double half = 0.5;
+            o0.x *= half; o0.y *= half; o1.x *= half; o1.y *= half; o2.x *= half; o2.y *= half;
+            if constexpr (EO == 1) {
+                st_nt(a.out[r] + idx, o0);
+                st_nt(a.out[r] + sh + idx, o1);
+                st_nt(a.out[r] + 2 * sh + idx, o2);
+            } else {
+                const double2* x = a.x[r];
+                const double2 x0 = x[idx], x1 = x[sh + idx], x2 = x[2 * sh + idx];
+                const double m2 = a.m2[r];
+                const double2 s0 = make_double2(x0.x * m2 - o0.x, x0.y * m2 - o0.y);
+                const double2 s1 = make_double2(x1.x * m2 - o1.x, x1.y * m2 - o1.y);
+                const double2 s2 = make_double2(x2.x * m2 - o2.x, x2.y * m2 - o2.y);
+                st_nt(a.out[r] + idx, s0);
+                st_nt(a.out[r] + sh + idx, s1);
+                st_nt(a.out[r] + 2 * sh + idx, s2);
+                dot[r] = re_dot(x0, s0) + re_dot(x1, s1) + re_dot(x2, s2);
+            }
+        }
+    }
+    if constexpr (EO == 0) {
+        // block_partial_at's sums, one LDS row per system (no wave may overwrite
+        // a wave sum of another system before thread 0 has read it)
+        __shared__ double lds[kMsMulti][kBlock / 64];
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+        for (int r = 0; r < kMsMulti; r++) {
+            double v = dot[r];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) lds[r][wid] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int r = 0; r < kMsMulti; r++) {
+                if (!act[r]) continue;
+                double t = 0.0;
+#pragma unroll
+                for (int w = 0; w < kBlock / 64; w++) t += lds[r][w];
+                a.part[r][2 * blk] = t;
+            }
+        }
+    }
+}
+
+struct MsSys {
+    const void* b;
+    void *out, *shiftferm, *r, *h, *s, *p;
+    double mass;
+    int order;
+    const double* shifts;
+    double residuo;
+    int max_cg;
+    int cg = 0;          // out: CG-M iterations
+    int status = INVERTER_FAILURE;
+};
+
+static SolverState* g_mm_st = nullptr;       // device states of the systems
+static SolverState* g_mm_st_host = nullptr;  // pinned mirror
+static double* g_mm_part = nullptr;          // kMsMulti x 2 nbh partials
+static size_t g_mm_part_n = 0;
+static MsWin* g_mm_win = nullptr;            // kMsMulti windows
+static void* g_mm_hist = nullptr;            // kMsMulti x W residuals
+static size_t g_mm_hist_bytes = 0;
+
+// multishift_solve<double> for n <= kMsMulti systems on the operator (u, ph);
+// false (nothing changed) when the window memory is not there, so the caller
+// solves them one by one
+static bool multishift_solve_multi(const void* u, const void* ph, MsSys* sy, int n) {
+    const Lattice& L = ctx().lat;
+    const long sizeh = L.sizeh, n3 = 3 * sizeh;
+    const size_t vb = L.vec3_bytes();
+    const int nbh = blocks_for(sizeh), nb3 = blocks_for(n3);
+    if (n < 1 || n > kMsMulti) return false;
+    for (int k = 0; k < n; k++)
+        if (sy[k].order < 0 || sy[k].order > kMaxShifts) return false;
+    const int W = msw_window(vb);
+    if (W < 2) return false;
+    // device resources
+    if (!g_mm_st) {
+        if (!hip_ok(hipMalloc(&g_mm_st, kMsMulti * sizeof(SolverState)), "multi states") ||
+            !hip_ok(hipHostMalloc(&g_mm_st_host, kMsMulti * sizeof(SolverState), hipHostMallocDefault),
+                    "multi states") ||
+            !hip_ok(hipMalloc(&g_mm_win, kMsMulti * sizeof(MsWin)), "multi windows"))
+            return false;
+    }
+    if (g_mm_part_n < (size_t)2 * nbh) {
+        if (g_mm_part) (void)hipFree(g_mm_part);
+        g_mm_part = nullptr;
+        g_mm_part_n = 0;
+        if (!hip_ok(hipMalloc(&g_mm_part, sizeof(double) * 2 * nbh * kMsMulti), "multi partials")) return false;
+        g_mm_part_n = (size_t)2 * nbh;
+    }
+    const size_t hb = (size_t)W * vb;
+    if (g_mm_hist_bytes < (size_t)n * hb) {
+        if (g_mm_hist) (void)hipFree(g_mm_hist);
+        g_mm_hist = nullptr;
+        g_mm_hist_bytes = 0;
+        if (hipMalloc(&g_mm_hist, (size_t)n * hb) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        g_mm_hist_bytes = (size_t)n * hb;
+    }
+    auto ctl_k = [&](int k, int pred, int iter = 0) {
+        SolverCtl c;
+        c.st = g_mm_st + k;
+        c.partials = g_mm_part + (size_t)k * g_mm_part_n;
+        c.pred = pred;
+        c.iter = iter;
+        return c;
+    };
+    auto hist = [&](int k) { return (char*)g_mm_hist + (size_t)k * hb; };
+    auto slot = [&](int k, int j) { return (double2*)(hist(k) + (size_t)((j - 1) % W) * vb); };
+
+    // x_k = 0 ; r = p = b ; p_k = b ; states as multishift_solve sets them
+    double sn[kMsMulti];
+    for (int k = 0; k < n; k++) {
+        const MsSys& d = sy[k];
+        for (int i = 0; i < d.order; i++) {
+            if (!hip_ok(hipMemsetAsync((char*)d.out + i * vb, 0, vb, S()), "zero x_k") ||
+                !hip_ok(hipMemcpyAsync((char*)d.shiftferm + i * vb, d.b, vb, hipMemcpyDeviceToDevice, S()),
+                        "p_k = b"))
+                return false;
+        }
+        if (!hip_ok(hipMemcpyAsync(d.r, d.b, vb, hipMemcpyDeviceToDevice, S()), "r = b") ||
+            !hip_ok(hipMemcpyAsync(d.p, d.b, vb, hipMemcpyDeviceToDevice, S()), "p = b") ||
+            !reduce_to_host(0, d.b, nullptr, 0, &sn[k], nullptr))
+            return false;
+        SolverState& s = g_mm_st_host[k];
+        std::memset(&s, 0, sizeof s);
+        s.res = d.residuo;
+        s.safety = 0.95;
+        s.max_cg = d.max_cg;
+        s.restart_every = 0;
+        s.status = kRunning;
+        s.delta = sn[k];
+        s.source_norm = sn[k];
+        s.omega = 1.0;
+        s.gammag = 0.0;
+        s.order = d.order;
+        s.flag_cur = 0;
+        for (int i = 0; i < d.order; i++) {
+            s.flags[1][i] = 1;
+            s.zeta_i[i] = 1.0;
+            s.zeta_ii[i] = 1.0;
+            s.gammas[i] = 0.0;
+            s.shifts[i] = d.shifts[i];
+        }
+        s.maxiter_next = d.order;
+        if (d.order == 0) s.status = kDone;
+    }
+    if (!hip_ok(hipMemcpyAsync(g_mm_st, g_mm_st_host, n * sizeof(SolverState), hipMemcpyHostToDevice, S()),
+                "multi states") ||
+        !hip_ok(hipStreamSynchronize(S()), "multi states"))
+        return false;
+    for (int k = 0; k < n; k++)
+        hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_mm_win + k, ctl_k(k, kAlways));
+
+    MsMultiHop a;
+    for (int l = 0; l < 8; l++) {
+        a.u[l] = (const double2*)((const char*)u + l * L.su3_bytes());
+        a.ph[l] = (const double*)((const char*)ph + l * L.dsoa_bytes());
+    }
+    a.g.nd0 = L.nd[0];
+    a.g.nd0h = L.nd[0] / 2;
+    a.g.nd1 = L.nd[1];
+    a.g.nd2 = L.nd[2];
+    a.g.nd3 = L.nd[3];
+    a.g.sizeh = (int)sizeh;
+    a.tmajor = (sizeh / L.nd[3]) % kBlock == 0 ? 1 : 0;  // as k_dslash<double>
+    a.n = n;
+    a.pred = kIfRunning;
+    for (int k = 0; k < kMsMulti; k++) {
+        const int q = k < n ? k : 0;
+        a.st[k] = g_mm_st + q;
+        a.part[k] = g_mm_part + (size_t)q * g_mm_part_n;
+        a.m2[k] = sy[q].mass * sy[q].mass;
+    }
+    auto iteration = [&](int j) -> bool {
+        // h_k = D_oe p_k ; s_k = m2_k p_k - D_eo h_k, <p_k, s_k>
+        for (int k = 0; k < kMsMulti; k++) {
+            const int q = k < n ? k : 0;
+            a.in[k] = (const double2*)sy[q].p;
+            a.out[k] = (double2*)sy[q].h;
+            a.x[k] = nullptr;
+        }
+        a.rev = ctx().dslash_serpentine ? (int)(ctx().dslash_launches++ & 1u) : 0;
+        hipLaunchKernelGGL(k_ms_multi_hop<1>, dim3(nbh), dim3(kBlock), 0, S(), a);
+        for (int k = 0; k < kMsMulti; k++) {
+            const int q = k < n ? k : 0;
+            a.in[k] = (const double2*)sy[q].h;
+            a.out[k] = (double2*)sy[q].s;
+            a.x[k] = (const double2*)sy[q].p;
+        }
+        a.rev = ctx().dslash_serpentine ? (int)(ctx().dslash_launches++ & 1u) : 0;
+        hipLaunchKernelGGL(k_ms_multi_hop<0>, dim3(nbh), dim3(kBlock), 0, S(), a);
+        for (int k = 0; k < n; k++) {
+            if (!launch_finalize(kFinMsAlpha, ctl_k(k, kIfRunning), nbh)) return false;
+            const double2* r_old = j == 1 ? (const double2*)sy[k].r : slot(k, j - 1);
+            hipLaunchKernelGGL((k_msw_r<double2>), dim3(nbh), dim3(kBlock), 0, S(), r_old, slot(k, j),
+                               (const double2*)sy[k].s, (int)sizeh, ctl_k(k, kIfRunning));
+            if (!launch_finalize(kFinMsLambda, ctl_k(k, kIfRunning), nbh)) return false;
+            hipLaunchKernelGGL((k_msw_p<double2>), dim3(nb3), dim3(kBlock), 0, S(), (double2*)sy[k].p,
+                               (const double2*)slot(k, j), n3, g_mm_win + k, ctl_k(k, kIfIter, j),
+                               (double2*)nullptr);
+            if (j % W == 0) {
+                hipLaunchKernelGGL((k_msw_flush<double2, double2>), dim3(nb3), dim3(kBlock), 0, S(),
+                                   (double2*)sy[k].out, (double2*)sy[k].shiftferm, (const double2*)hist(k), n3,
+                                   sy[k].order, (const MsWin*)(g_mm_win + k), ctl_k(k, kIfIter, j));
+                hipLaunchKernelGGL(k_msw_reset, dim3(1), dim3(kBlock), 0, S(), g_mm_win + k, ctl_k(k, kIfIter, j));
+            }
+        }
+        return check_launch("multi multishift iteration");
+    };
+    // batches as run_batches, sized by the slowest system
+    int batch = 4, last_total = -1;
+    double prev_d = NAN;
+    int last_cg = -1;
+    for (;;) {
+        if (!hip_ok(hipMemcpyAsync(g_mm_st_host, g_mm_st, n * sizeof(SolverState), hipMemcpyDeviceToHost, S()),
+                    "multi states") ||
+            !hip_ok(hipStreamSynchronize(S()), "multi states"))
+            return false;
+        int running = 0, total = 0, cg0 = 0;
+        double d = -INFINITY;
+        for (int k = 0; k < n; k++) {
+            const SolverState& s = g_mm_st_host[k];
+            total += s.cg;
+            if (s.status == kRunning) {
+                running++;
+                cg0 = std::max(cg0, s.cg);
+                const double dk = dist_ms(s);
+                d = std::isfinite(dk) ? std::max(d, dk) : d;
+            }
+        }
+        if (!running) break;
+        if (total == last_total) return fail("multi multishift: solver states did not advance (iteration %d)", cg0);
+        last_total = total;
+        int nq = batch;
+        if (std::isfinite(d) && std::isfinite(prev_d) && last_cg >= 0 && cg0 > last_cg) {
+            const double rate = (prev_d - d) / (double)(cg0 - last_cg);
+            if (rate > 1e-9) {
+                const double left = d > 0.0 ? d / rate : 0.0;
+                nq = std::min(nq, std::max(2, (int)std::ceil(1.1 * left) + 1));
+            }
+        }
+        prev_d = std::isfinite(d) ? d : NAN;
+        last_cg = cg0;
+        for (int j = 1; j <= nq; j++)
+            if (!iteration(cg0 + j)) return false;
+        batch = std::min(2 * batch, 64);
+    }
+    // the partial last windows, final residuals, the per-shift check
+    for (int k = 0; k < n; k++) {
+        MsSys& d = sy[k];
+        const int cg = g_mm_st_host[k].cg;
+        d.cg = cg;
+        hipLaunchKernelGGL((k_msw_flush<double2, double2>), dim3(nb3), dim3(kBlock), 0, S(), (double2*)d.out,
+                           (double2*)d.shiftferm, (const double2*)hist(k), n3, d.order,
+                           (const MsWin*)(g_mm_win + k), ctl_k(k, kAlways));
+        if (cg > 0 && !hip_ok(hipMemcpyAsync(d.r, slot(k, cg), vb, hipMemcpyDeviceToDevice, S()), "final residual"))
+            return false;
+        if (!check_launch("multi multishift window flush")) return false;
+        multishift_invert_iterations += cg;
+        if (cg == d.max_cg) printf("WARNING: maximum number of iterations reached in invert\n");
+        int check = 1;
+        if (d.order > 0) {
+            double g[kMaxShifts];
+            if (!verify_shifts_f64(u, ph, d.b, d.out, d.order, d.mass * d.mass, d.shifts, sn[k], g)) return false;
+            for (int i = 0; i < d.order; i++) check *= (g[i] <= 1) ? 1 : 0;
+        }
+        if (!hip_ok(hipMemcpyAsync(d.p, (const char*)d.out + (d.order > 0 ? (d.order - 1) : 0) * vb, vb,
+                                   hipMemcpyDeviceToDevice, S()),
+                    "loc_p"))
+            return false;
+        if (verbosity_lv > 0)
+            printf("Terminated multishift_invert ( target res = %1.1e,source_norm = %1.1e )\tCG count %d\n",
+                   d.residuo, sn[k], cg);
+        d.status = check == 1 ? INVERTER_SUCCESS : INVERTER_FAILURE;
+    }
+    return hip_ok(hipStreamSynchronize(S()), "synchronize");
+}
+
 bool recombine(const void* xs, const void* in, void* out, const RationalApprox* ra) {
     RecombineCoeffs c;
     c.a0 = ra->RA_a0;
@@ -1140,6 +1497,109 @@ struct ScratchSwap {
         ctx().max_blocks = max_blocks;
     }
 };
+
+// 1 where two FP64 arrays of n doubles differ in any bit
+__global__ __launch_bounds__(kBlock) void k_differ(const unsigned long long* __restrict__ a,
+                                                   const unsigned long long* __restrict__ b, long n, int* flag) {
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n && a[i] != b[i]) *flag = 1;
+}
+static int* g_differ_flag = nullptr;
+
+// the U(1) phases of two flavours are the same (one array, or equal values)
+static bool same_phases(const void* a, const void* b) {
+    if (a == b) return true;
+    const long n = 8 * ctx().lat.sizeh;
+    if (!g_differ_flag && !hip_ok(hipMalloc(&g_differ_flag, sizeof(int)), "phase comparison")) return false;
+    int h = 0;
+    if (!hip_ok(hipMemsetAsync(g_differ_flag, 0, sizeof(int), S()), "phase comparison")) return false;
+    hipLaunchKernelGGL(k_differ, dim3(blocks_for(n)), dim3(kBlock), 0, S(), (const unsigned long long*)a,
+                       (const unsigned long long*)b, n, g_differ_flag);
+    if (!check_launch("k_differ") ||
+        !hip_ok(hipMemcpyAsync(&h, g_differ_flag, sizeof(int), hipMemcpyDeviceToHost, S()), "phase comparison") ||
+        !hip_ok(hipStreamSynchronize(S()), "phase comparison"))
+        return false;
+    return h == 0;
+}
+
+static void* g_many_work = nullptr;
+static size_t g_many_work_bytes = 0;
+static int g_ms_many = 1;  // rational solves of several sources together where they share the operator
+
+// The rational solves of n sources (in_k with the approximation and mass of
+// pars[k], solutions to out_k, order_k vectors each) as
+// inverter_multishift_wrapper would run them one by one in FP64, run
+// together (multishift_solve_multi, in groups of kMsMulti) when every source
+// has the same U(1) phases.  false, with nothing solved, when that does not
+// apply (another precision mode, the single-precision accelerated path,
+// fewer than two sources, different phases, or no memory for it).
+bool multishift_fp64_many(const void* u, ferm_param* const* pars, const void* const* in, void* const* out, int n,
+                          double res, int max_cg, int convergence_importance) {
+    if (!g_ms_many || n < 2 || g_ms_precision != 0 || inverter_tricks.singlePInvAccelMultiInv) return false;
+    const Lattice& L = ctx().lat;
+    const size_t vb = L.vec3_bytes();
+    const char* nm = "inverter_multishift_wrapper";
+    const void* ph0 = resolve(pars[0]->phases, 8 * L.dsoa_bytes(), nm);
+    if (!ph0) return false;
+    int maxo = 0;
+    for (int k = 0; k < n; k++) {
+        const void* ph = resolve(pars[k]->phases, 8 * L.dsoa_bytes(), nm);
+        const int order = pars[k]->approx_md.approx_order;
+        if (!ph || order < 0 || order > kMaxShifts || !same_phases(ph0, ph)) return false;
+        maxo = std::max(maxo, order);
+    }
+    // per system of a group: r, h, s, p and the search directions p_k
+    const size_t per = (size_t)(4 + std::max(maxo, 1)) * vb;
+    char* w = (char*)grow_buffer(g_many_work, g_many_work_bytes, kMsMulti * per);
+    if (!w) {
+        (void)hipGetLastError();
+        ostp_clear_error();
+        return false;
+    }
+    for (int k0 = 0; k0 < n; k0 += kMsMulti) {
+        const int m = std::min(kMsMulti, n - k0);
+        MsSys sy[kMsMulti];
+        for (int r = 0; r < m; r++) {
+            ferm_param* fp = pars[k0 + r];
+            char* base = w + (size_t)r * per;
+            MsSys& d = sy[r];
+            d.b = in[k0 + r];
+            d.out = out[k0 + r];
+            d.r = base;
+            d.h = base + vb;
+            d.s = base + 2 * vb;
+            d.p = base + 3 * vb;
+            d.shiftferm = base + 4 * vb;
+            d.mass = fp->ferm_mass;
+            d.order = fp->approx_md.approx_order;
+            d.shifts = fp->approx_md.RA_b;
+            d.residuo = res;
+            d.max_cg = max_cg;
+        }
+        if (!multishift_solve_multi(u, ph0, sy, m)) {
+            if (ostp_last_error()[0]) return false;
+            // no window memory for the group: one by one, as the wrapper does
+            for (int r = 0; r < m; r++) {
+                MsBufs B;
+                B.u = u;
+                B.ph = ph0;
+                B.b = sy[r].b;
+                B.out = sy[r].out;
+                B.shiftferm = sy[r].shiftferm;
+                B.r = sy[r].r;
+                B.h = sy[r].h;
+                B.s = sy[r].s;
+                B.p = sy[r].p;
+                int cg = 0;
+                sy[r].status = multishift_solve<double>(B, sy[r].mass, sy[r].order, sy[r].shifts, res, max_cg, &cg,
+                                                        nullptr);
+                sy[r].cg = cg;
+            }
+        }
+        for (int r = 0; r < m; r++) convergence_messages(convergence_importance, sy[r].status);
+    }
+    return !ostp_last_error()[0];
+}
 
 }  // namespace ostp
 
@@ -1459,6 +1919,12 @@ int inverter_multishift_wrapper(inverter_package ip, ferm_param* pars, RationalA
         total_iterations += cg_return;
     }
     return total_iterations;
+}
+
+int ostp_set_multishift_batching(int on) {
+    const int prev = g_ms_many;
+    g_ms_many = on ? 1 : 0;
+    return prev;
 }
 
 int ostp_set_batched_graph_iterations(int n) {
