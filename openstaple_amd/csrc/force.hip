@@ -250,8 +250,9 @@ struct Scratch {
 static Scratch g_hs, g_th, g_ta, g_conf_f, g_xs_all;
 
 // the rational solves of several sources on one operator, together (solvers.hip)
-bool multishift_fp64_many(const void* u, ferm_param* const* pars, const void* const* in, void* const* out, int n,
-                          double res, int max_cg, int convergence_importance);
+bool multishift_fp64_many(const void* u, ferm_param* const* pars, const RationalApprox* const* ras,
+                          const void* const* in, void* const* out, int n, double res, int max_cg,
+                          int convergence_importance);
 
 // sum over shifts of the outer products for one pseudofermion
 bool compute_fermion_force(const void* u, void* aux, const void* xs, const void* phases, const RationalApprox* ra) {
@@ -378,6 +379,7 @@ void fermion_force_soloopenacc(su3_soa* tconf_acc, su3_soa* tstout_conf_acc_arr,
     bool together = false;
     {
         std::vector<ferm_param*> fps;
+        std::vector<const RationalApprox*> ras;
         std::vector<const void*> ins;
         std::vector<void*> outs;
         int maxo = 1;
@@ -386,6 +388,7 @@ void fermion_force_soloopenacc(su3_soa* tconf_acc, su3_soa* tstout_conf_acc_arr,
             maxo = std::max(maxo, fp->approx_md.approx_order);
             for (int ips = 0; ips < fp->number_of_ps; ips++) {
                 fps.push_back(fp);
+                ras.push_back(&fp->approx_md);
                 ins.push_back(resolve((const char*)ferm_in_acc + (size_t)(fp->index_of_the_first_ps + ips) *
                                                                      L.vec3_bytes(),
                                       L.vec3_bytes(), nm));
@@ -402,8 +405,8 @@ void fermion_force_soloopenacc(su3_soa* tconf_acc, su3_soa* tstout_conf_acc_arr,
                 if (!ins[k]) return;
                 outs.push_back(xs_all + (size_t)k * maxo * L.vec3_bytes());
             }
-            together = multishift_fp64_many(conf_to_use, fps.data(), ins.data(), outs.data(), n, res, max_cg,
-                                            CONVERGENCE_NONCRITICAL);
+            together = multishift_fp64_many(conf_to_use, fps.data(), ras.data(), ins.data(), outs.data(), n, res,
+                                            max_cg, CONVERGENCE_NONCRITICAL);
             if (!together && ostp_last_error()[0]) return;
         }
         if (together) {

@@ -20,6 +20,7 @@
 // (ostp_set_metropolis_uniform) or, by default, from the same seed.
 #include "ostp_blas.h"
 
+#include <vector>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -50,6 +51,15 @@ alloc_settings alloc_info;
 }
 
 static_assert(sizeof(alloc_settings) == 44, "alloc_settings layout (alloc_settings.h:4-20)");
+
+namespace ostp {
+// solvers.hip: the rational solves of several sources on one operator together,
+// and the recombination of a rational function's shifted solutions
+bool multishift_fp64_many(const void* u, ferm_param* const* pars, const RationalApprox* const* ras,
+                          const void* const* in, void* const* out, int n, double res, int max_cg,
+                          int convergence_importance);
+bool recombine(const void* xs, const void* in, void* out, const RationalApprox* ra);
+}  // namespace ostp
 
 namespace {
 
@@ -87,7 +97,7 @@ struct DevScratch {
         return p;
     }
 };
-DevScratch g_conf_f, g_shift_f, g_loc_f;
+DevScratch g_conf_f, g_shift_f, g_loc_f, g_xs_many;
 
 // gauge action S = beta/3 (c0 sum Re tr P + c1 sum Re tr R) as the reference
 // sums it (update_versatile.c:215-221)
@@ -245,6 +255,53 @@ int UPDATE_SOLOACC_UNOSTEP_VERSATILE(su3_soa* tconf_acc, double res_metro, doubl
     auto apply_rational = [&](int which, vec3_soa* xs, vec3_soa* ys, double res) {
         inverter_package ip;
         if (!package(ip)) return false;
+        // every source's rational solve together where they share the operator
+        // (inverter_multishift_wrapper's FP64 path, same phases), then each y = r(M) x
+        {
+            std::vector<ferm_param*> fps;
+            std::vector<const RationalApprox*> ras;
+            std::vector<const void*> ins;
+            std::vector<void*> outs;
+            std::vector<int> idx;
+            for (int f = 0; f < nflav; f++) {
+                ferm_param& fp = fermions_parameters[f];
+                for (int ips = 0; ips < fp.number_of_ps; ips++) {
+                    fps.push_back(&fp);
+                    ras.push_back(which == 0 ? &fp.approx_fi : &fp.approx_li);
+                    idx.push_back(fp.index_of_the_first_ps + ips);
+                }
+            }
+            const int n = (int)fps.size();
+            const size_t per = (size_t)(max_order > 0 ? max_order : 1) * vb;
+            char* xs_all = n >= 2 ? (char*)g_xs_many.get((size_t)n * per) : nullptr;
+            if (n >= 2 && !xs_all) ostp_clear_error();
+            if (xs_all) {
+                const void* du_fm = resolve(ip.u, 8 * L.su3_bytes(), nm);
+                if (!du_fm) return false;
+                for (int k = 0; k < n; k++) {
+                    const void* dx = resolve(at(xs, idx[k], vb), vb, nm);
+                    if (!dx) return false;
+                    ins.push_back(dx);
+                    outs.push_back(xs_all + (size_t)k * per);
+                }
+                if (multishift_fp64_many(du_fm, fps.data(), ras.data(), ins.data(), outs.data(), n, res, max_cg,
+                                         CONVERGENCE_CRITICAL)) {
+                    for (int k = 0; k < n; k++) {
+                        void* dy = resolve(at(ys, idx[k], vb), vb, nm);
+                        if (!dy || !recombine(outs[k], ins[k], dy, ras[k])) return false;
+                    }
+                    // the last source's shifted solutions stay in ferm_shiftmulti_acc, as after the loop below
+                    const int order = ras[n - 1]->approx_order;
+                    void* dsm = resolve(ferm_shiftmulti_acc, per, nm);
+                    if (!dsm || (order > 0 && !hip_ok(hipMemcpyAsync(dsm, outs[n - 1], (size_t)order * vb,
+                                                                     hipMemcpyDeviceToDevice, ctx().stream),
+                                                      nm)))
+                        return false;
+                    return !failed();
+                }
+                if (failed()) return false;
+            }
+        }
         for (int f = 0; f < nflav; f++) {
             ferm_param& fp = fermions_parameters[f];
             RationalApprox* ra = which == 0 ? &fp.approx_fi : &fp.approx_li;

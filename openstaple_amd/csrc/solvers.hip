@@ -1526,15 +1526,16 @@ static void* g_many_work = nullptr;
 static size_t g_many_work_bytes = 0;
 static int g_ms_many = 1;  // rational solves of several sources together where they share the operator
 
-// The rational solves of n sources (in_k with the approximation and mass of
-// pars[k], solutions to out_k, order_k vectors each) as
+// The rational solves of n sources (in_k with the approximation ras[k] and
+// the mass of pars[k], solutions to out_k, order_k vectors each) as
 // inverter_multishift_wrapper would run them one by one in FP64, run
 // together (multishift_solve_multi, in groups of kMsMulti) when every source
 // has the same U(1) phases.  false, with nothing solved, when that does not
 // apply (another precision mode, the single-precision accelerated path,
 // fewer than two sources, different phases, or no memory for it).
-bool multishift_fp64_many(const void* u, ferm_param* const* pars, const void* const* in, void* const* out, int n,
-                          double res, int max_cg, int convergence_importance) {
+bool multishift_fp64_many(const void* u, ferm_param* const* pars, const RationalApprox* const* ras,
+                          const void* const* in, void* const* out, int n, double res, int max_cg,
+                          int convergence_importance) {
     if (!g_ms_many || n < 2 || g_ms_precision != 0 || inverter_tricks.singlePInvAccelMultiInv) return false;
     const Lattice& L = ctx().lat;
     const size_t vb = L.vec3_bytes();
@@ -1544,7 +1545,7 @@ bool multishift_fp64_many(const void* u, ferm_param* const* pars, const void* co
     int maxo = 0;
     for (int k = 0; k < n; k++) {
         const void* ph = resolve(pars[k]->phases, 8 * L.dsoa_bytes(), nm);
-        const int order = pars[k]->approx_md.approx_order;
+        const int order = ras[k]->approx_order;
         if (!ph || order < 0 || order > kMaxShifts || !same_phases(ph0, ph)) return false;
         maxo = std::max(maxo, order);
     }
@@ -1571,8 +1572,8 @@ bool multishift_fp64_many(const void* u, ferm_param* const* pars, const void* co
             d.p = base + 3 * vb;
             d.shiftferm = base + 4 * vb;
             d.mass = fp->ferm_mass;
-            d.order = fp->approx_md.approx_order;
-            d.shifts = fp->approx_md.RA_b;
+            d.order = ras[k0 + r]->approx_order;
+            d.shifts = ras[k0 + r]->RA_b;
             d.residuo = res;
             d.max_cg = max_cg;
         }
