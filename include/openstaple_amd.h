@@ -412,9 +412,11 @@ int ostp_set_batched_graph_iterations(int n);
  * accelerated mode) on one operator up to the mass (the same U(1) phases:
  * one array, or equal values), up to four run as one CG-M each whose
  * iterations share the two hops (each link applied to every source's
- * spinor).  Each source's iterations, solutions and convergence are those
- * of its own multishift_invert.  on = 0 solves them one by one; returns the
- * previous setting (default 1). */
+ * spinor); UPDATE_SOLOACC_UNOSTEP_VERSATILE does the same for its heat-bath
+ * and acceptance solves and for the flavours' power iterations (eigenvalue
+ * bounds).  Each source's iterations, solutions and convergence are those of
+ * its own multishift_invert (power method).  on = 0 runs them one by one;
+ * returns the previous setting (default 1). */
 int ostp_set_multishift_batching(int on);
 /* Text of the autotuner's measurements so far (candidates and choice per
  * batch signature); returns its length, copies up to len-1 bytes into buf. */

@@ -59,6 +59,8 @@ bool multishift_fp64_many(const void* u, ferm_param* const* pars, const Rational
                           const void* const* in, void* const* out, int n, double res, int max_cg,
                           int convergence_importance);
 bool recombine(const void* xs, const void* in, void* out, const RationalApprox* ra);
+bool find_max_eigenvalues_many(const void* u, ferm_param* const* pars, void* const* r, void* const* h,
+                               void* const* p, int n, double* maxv);
 }  // namespace ostp
 
 namespace {
@@ -97,7 +99,7 @@ struct DevScratch {
         return p;
     }
 };
-DevScratch g_conf_f, g_shift_f, g_loc_f, g_xs_many;
+DevScratch g_conf_f, g_shift_f, g_loc_f, g_xs_many, g_pm_many;
 
 // gauge action S = beta/3 (c0 sum Re tr P + c1 sum Re tr R) as the reference
 // sums it (update_versatile.c:215-221)
@@ -203,6 +205,47 @@ int UPDATE_SOLOACC_UNOSTEP_VERSATILE(su3_soa* tconf_acc, double res_metro, doubl
         ostp_generate_vec3_soa_gauss(kloc_s, g_seed, g_stream++);
         find_min_max_eigenvalue_soloopenacc(gconf_as_fermionmatrix, &fermions_parameters[f], kloc_r, kloc_h, kloc_p,
                                             kloc_s, minmax[f]);
+        return !failed();
+    };
+    // the bounds of every flavour: the start vectors drawn as bounds(f) draws
+    // them, then the power iterations of all flavours together where their
+    // phases agree (one by one otherwise)
+    auto bounds_all = [&]() {
+        if (nflav < 2 || nflav > 4) {
+            for (int f = 0; f < nflav; f++)
+                if (!bounds(f)) return false;
+            return true;
+        }
+        char* w = (char*)g_pm_many.get((size_t)3 * nflav * vb);
+        if (!w) {
+            ostp_clear_error();
+            for (int f = 0; f < nflav; f++)
+                if (!bounds(f)) return false;
+            return true;
+        }
+        void *r[4], *h[4], *pv[4];
+        ferm_param* fps[4];
+        for (int f = 0; f < nflav; f++) {
+            pv[f] = w + (size_t)(3 * f) * vb;
+            r[f] = w + (size_t)(3 * f + 1) * vb;
+            h[f] = w + (size_t)(3 * f + 2) * vb;
+            fps[f] = &fermions_parameters[f];
+            ostp_generate_vec3_soa_gauss((vec3_soa*)pv[f], g_seed, g_stream++);
+            ostp_generate_vec3_soa_gauss(kloc_s, g_seed, g_stream++);
+        }
+        if (failed()) return false;
+        const void* du_fm = resolve(gconf_as_fermionmatrix, 8 * L.su3_bytes(), nm);
+        if (!du_fm) return false;
+        double maxv[4];
+        const bool together = find_max_eigenvalues_many(du_fm, fps, r, h, pv, nflav, maxv);
+        if (!together && failed()) return false;
+        for (int f = 0; f < nflav; f++) {
+            minmax[f][0] = fermions_parameters[f].ferm_mass * fermions_parameters[f].ferm_mass;
+            minmax[f][1] = together ? maxv[f]
+                                    : ker_find_max_eigenvalue_openacc(gconf_as_fermionmatrix, &fermions_parameters[f],
+                                                                      (vec3_soa*)r[f], (vec3_soa*)h[f],
+                                                                      (vec3_soa*)pv[f]);
+        }
         return !failed();
     };
     auto rescale = [&](int f, int which) {
@@ -331,8 +374,9 @@ int UPDATE_SOLOACC_UNOSTEP_VERSATILE(su3_soa* tconf_acc, double res_metro, doubl
 
     // 3.-5. stout, bounds, first-inversion approximation, initial action
     if (!stout_links()) return acc;
+    if (!bounds_all()) return acc;
     for (int f = 0; f < nflav; f++)
-        if (!bounds(f) || !rescale(f, 0)) return acc;
+        if (!rescale(f, 0)) return acc;
     if (metro && !actions(&g_info.action_in, &g_info.action_mom_in, ferm_phi_acc, ferm_phi_acc, &g_info.action_ferm_in))
         return acc;
     // 6. heat bath chi = r_fi phi; 7. MD approximation
@@ -355,8 +399,9 @@ int UPDATE_SOLOACC_UNOSTEP_VERSATILE(su3_soa* tconf_acc, double res_metro, doubl
     if (!stout_links()) return acc;
     if (metro) {
         // 10. bounds and last-inversion approximation; phi' = r_li chi
+        if (!bounds_all()) return acc;
         for (int f = 0; f < nflav; f++)
-            if (!bounds(f) || !rescale(f, 2)) return acc;
+            if (!rescale(f, 2)) return acc;
         if (!apply_rational(2, ferm_chi_acc, ferm_phi_acc, res_metro)) return acc;
         // 11. final action and the Metropolis test
         if (!actions(&g_info.action_fin, &g_info.action_mom_fin, ferm_chi_acc, ferm_phi_acc,

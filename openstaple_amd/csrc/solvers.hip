@@ -1498,6 +1498,121 @@ struct ScratchSwap {
     }
 };
 
+// power_method for n <= kMsMulti start vectors p_k on the operators
+// coef_k - Deo Doe that differ only by coef_k (the flavours of one gauge
+// field): the two hops of each iteration run once for all of them
+// (k_ms_multi_hop), the scaling and the norm per system on its own state, so
+// each system's iterations and norms are those of power_method.
+static bool power_method_multi(const void* u, const void* ph, const double* coef, void* const* r, void* const* h,
+                               void* const* p, int n, double* lambda, int* iters) {
+    const Lattice& L = ctx().lat;
+    const long sizeh = L.sizeh, n3 = 3 * sizeh;
+    const int nbh = blocks_for(sizeh);
+    if (n < 1 || n > kMsMulti) return false;
+    if (!g_mm_st) {
+        if (!hip_ok(hipMalloc(&g_mm_st, kMsMulti * sizeof(SolverState)), "multi states") ||
+            !hip_ok(hipHostMalloc(&g_mm_st_host, kMsMulti * sizeof(SolverState), hipHostMallocDefault),
+                    "multi states") ||
+            !hip_ok(hipMalloc(&g_mm_win, kMsMulti * sizeof(MsWin)), "multi windows"))
+            return false;
+    }
+    if (g_mm_part_n < (size_t)2 * nbh) {
+        if (g_mm_part) (void)hipFree(g_mm_part);
+        g_mm_part = nullptr;
+        g_mm_part_n = 0;
+        if (!hip_ok(hipMalloc(&g_mm_part, sizeof(double) * 2 * nbh * kMsMulti), "multi partials")) return false;
+        g_mm_part_n = (size_t)2 * nbh;
+    }
+    auto ctl_k = [&](int k, int pred) {
+        SolverCtl c;
+        c.st = g_mm_st + k;
+        c.partials = g_mm_part + (size_t)k * g_mm_part_n;
+        c.pred = pred;
+        return c;
+    };
+    for (int k = 0; k < n; k++) {
+        double sn = 0.0;
+        if (!reduce_to_host(0, p[k], nullptr, 0, &sn, nullptr)) return false;
+        SolverState& s = g_mm_st_host[k];
+        std::memset(&s, 0, sizeof s);
+        s.max_cg = 1000000;  // the reference has no iteration limit
+        s.status = kRunning;
+        s.lambda = std::sqrt(sn);
+    }
+    if (!hip_ok(hipMemcpyAsync(g_mm_st, g_mm_st_host, n * sizeof(SolverState), hipMemcpyHostToDevice, S()),
+                "multi states") ||
+        !hip_ok(hipStreamSynchronize(S()), "multi states"))
+        return false;
+    MsMultiHop a;
+    for (int l = 0; l < 8; l++) {
+        a.u[l] = (const double2*)((const char*)u + l * L.su3_bytes());
+        a.ph[l] = (const double*)((const char*)ph + l * L.dsoa_bytes());
+    }
+    a.g.nd0 = L.nd[0];
+    a.g.nd0h = L.nd[0] / 2;
+    a.g.nd1 = L.nd[1];
+    a.g.nd2 = L.nd[2];
+    a.g.nd3 = L.nd[3];
+    a.g.sizeh = (int)sizeh;
+    a.tmajor = (sizeh / L.nd[3]) % kBlock == 0 ? 1 : 0;
+    a.n = n;
+    a.pred = kIfRunning;
+    for (int k = 0; k < kMsMulti; k++) {
+        const int q = k < n ? k : 0;
+        a.st[k] = g_mm_st + q;
+        a.part[k] = g_mm_part + (size_t)q * g_mm_part_n;
+        a.m2[k] = coef[q];
+    }
+    auto iteration = [&]() -> bool {
+        // r = p / norm ; p = coef r - Deo Doe r ; norm = |p|, stop test
+        for (int k = 0; k < n; k++)
+            hipLaunchKernelGGL(k_pm_scale, dim3(blocks_for(n3)), dim3(kBlock), 0, S(), (const double2*)p[k],
+                               (double2*)r[k], n3, ctl_k(k, kIfRunning));
+        for (int k = 0; k < kMsMulti; k++) {
+            const int q = k < n ? k : 0;
+            a.in[k] = (const double2*)r[q];
+            a.out[k] = (double2*)h[q];
+            a.x[k] = nullptr;
+        }
+        a.rev = ctx().dslash_serpentine ? (int)(ctx().dslash_launches++ & 1u) : 0;
+        hipLaunchKernelGGL(k_ms_multi_hop<1>, dim3(nbh), dim3(kBlock), 0, S(), a);
+        for (int k = 0; k < kMsMulti; k++) {
+            const int q = k < n ? k : 0;
+            a.in[k] = (const double2*)h[q];
+            a.out[k] = (double2*)p[q];
+            a.x[k] = (const double2*)r[q];
+        }
+        a.rev = ctx().dslash_serpentine ? (int)(ctx().dslash_launches++ & 1u) : 0;
+        hipLaunchKernelGGL(k_ms_multi_hop<0>, dim3(nbh), dim3(kBlock), 0, S(), a);
+        for (int k = 0; k < n; k++)
+            if (!launch_norm2<double2, kFinPower>(p[k], ctl_k(k, kIfRunning))) return false;
+        return check_launch("multi power method");
+    };
+    int batch = 4, last_total = -1;
+    for (;;) {
+        if (!hip_ok(hipMemcpyAsync(g_mm_st_host, g_mm_st, n * sizeof(SolverState), hipMemcpyDeviceToHost, S()),
+                    "multi states") ||
+            !hip_ok(hipStreamSynchronize(S()), "multi states"))
+            return false;
+        int running = 0, total = 0;
+        for (int k = 0; k < n; k++) {
+            total += g_mm_st_host[k].cg;
+            if (g_mm_st_host[k].status == kRunning) running++;
+        }
+        if (!running) break;
+        if (total == last_total) return fail("multi power method: states did not advance");
+        last_total = total;
+        for (int j = 0; j < batch; j++)
+            if (!iteration()) return false;
+        batch = std::min(2 * batch, 64);
+    }
+    for (int k = 0; k < n; k++) {
+        lambda[k] = g_mm_st_host[k].lambda;
+        if (iters) iters[k] = g_mm_st_host[k].cg;
+    }
+    return true;
+}
+
 // 1 where two FP64 arrays of n doubles differ in any bit
 __global__ __launch_bounds__(kBlock) void k_differ(const unsigned long long* __restrict__ a,
                                                    const unsigned long long* __restrict__ b, long n, int* flag) {
@@ -1600,6 +1715,24 @@ bool multishift_fp64_many(const void* u, ferm_param* const* pars, const Rational
         for (int r = 0; r < m; r++) convergence_messages(convergence_importance, sy[r].status);
     }
     return !ostp_last_error()[0];
+}
+
+// ker_find_max_eigenvalue_openacc for n flavours (start vectors p_k, work
+// r_k, h_k) together when their U(1) phases agree; false, with nothing done,
+// otherwise (or when batching is off), so the caller runs them one by one.
+bool find_max_eigenvalues_many(const void* u, ferm_param* const* pars, void* const* r, void* const* h,
+                               void* const* p, int n, double* maxv) {
+    if (!g_ms_many || n < 2 || n > kMsMulti) return false;
+    const Lattice& L = ctx().lat;
+    const void* ph0 = resolve(pars[0]->phases, 8 * L.dsoa_bytes(), "find_min_max_eigenvalue_soloopenacc");
+    if (!ph0) return false;
+    for (int k = 1; k < n; k++) {
+        const void* ph = resolve(pars[k]->phases, 8 * L.dsoa_bytes(), "find_min_max_eigenvalue_soloopenacc");
+        if (!ph || !same_phases(ph0, ph)) return false;
+    }
+    double coef[kMsMulti];
+    for (int k = 0; k < n; k++) coef[k] = pars[k]->ferm_mass * pars[k]->ferm_mass;
+    return power_method_multi(u, ph0, coef, r, h, p, n, maxv, nullptr);
 }
 
 }  // namespace ostp
