@@ -258,7 +258,8 @@ def smeared_field(lib, dmalloc, gb, eps, seed):
 
 
 def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
-    """Wall time of each solve to ||b - M x|| / ||b|| < 1e-10 (FP64 true residual)."""
+    """Wall time of each solve to ||b - M x|| / ||b|| < 1e-10 (FP64 true residual): the median of three
+    runs after any warm-up, the three times listed beside it."""
     sizeh = L.sizeh
     vb = L.vec3_bytes
     mass = 0.01
@@ -293,15 +294,26 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
         lib.combine_in1_minus_in2(a, ref, diff)
         return (lib.l2norm2_global(diff) / lib.l2norm2_global(ref)) ** 0.5
 
+    def timed(run, reset=None, reps=3):
+        """Median wall time of `reps` calls of run() (each after reset()), the
+        stream drained before and after each; returns (median, all, last result)."""
+        ts, r = [], None
+        for _ in range(reps):
+            if reset:
+                reset()
+            lib.ostp_synchronize()
+            t0 = time.perf_counter()
+            r = run()
+            lib.ostp_synchronize()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2], [round(t, 4) for t in ts], r
+
     # (a) FP64 CG-M
     cg = C.c_int(0)
     sres = (C.c_double * 25)()
-    lib.ostp_synchronize()
-    t0 = time.perf_counter()
-    st = lib.ostp_multishift_solve(u, C.byref(fp), C.byref(ra), out, b, 1e-10, *d64, sf, 20000, 0, C.byref(cg), sres)
-    lib.ostp_synchronize()
-    t1 = time.perf_counter()
-    res["multishift_fp64"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "status": st,
+    tm, tall, st = timed(lambda: lib.ostp_multishift_solve(u, C.byref(fp), C.byref(ra), out, b, 1e-10, *d64, sf,
+                                                           20000, 0, C.byref(cg), sres))
+    res["multishift_fp64"] = {"seconds": round(tm, 4), "seconds_runs": tall, "iterations": cg.value, "status": st,
                               "max_shift_true_residual": max(sres[k] for k in range(order)), "shifts": order}
     for k in range(order):
         lib.assign_in_to_out(out + k * vb, out64 + k * vb)
@@ -340,31 +352,21 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     cgr, trs_ms = (C.c_int * order)(), (C.c_double * order)()
     lib.ostp_multishift_mixed_solve(ip, C.byref(fp), C.byref(ra), out, b, 1e-10, 20000, 1, C.byref(cgm), cgr,
                                     trs_ms)  # warm-up: scratch, autotune
-    lib.ostp_synchronize()
-    t0 = time.perf_counter()
-    conv = lib.ostp_multishift_mixed_solve(ip, C.byref(fp), C.byref(ra), out, b, 1e-10, 20000, 1, C.byref(cgm),
-                                           cgr, trs_ms)
-    lib.ostp_synchronize()
-    t1 = time.perf_counter()
-    res["multishift_mixed_ru"] = {"seconds": round(t1 - t0, 4), "cgm_iterations": cgm.value,
+    tm, tall, conv = timed(lambda: lib.ostp_multishift_mixed_solve(ip, C.byref(fp), C.byref(ra), out, b, 1e-10, 20000,
+                                                                   1, C.byref(cgm), cgr, trs_ms))
+    t0, t1 = 0.0, tm
+    res["multishift_mixed_ru"] = {"seconds": round(t1 - t0, 4), "seconds_runs": tall, "cgm_iterations": cgm.value,
                                   "refine_iterations": [cgr[k] for k in range(order)],
                                   "converged_shifts": conv, "max_shift_true_residual": max(trs_ms[k] for k in range(order)),
                                   "max_shift_err_vs_fp64": max(rel_err(out + k * vb, out64 + k * vb)
                                                                for k in range(order))}
     res["multishift_speedup_mixed_ru_vs_fp64"] = round(res["multishift_fp64"]["seconds"] / (t1 - t0), 3)
     # the same two at the MD force's tolerance, 1e-8
-    lib.ostp_synchronize()
-    t0 = time.perf_counter()
-    lib.ostp_multishift_solve(u, C.byref(fp), C.byref(ra), out, b, 1e-8, *d64, sf, 20000, 0, C.byref(cg), sres)
-    lib.ostp_synchronize()
-    t1 = time.perf_counter()
-    f64_8 = t1 - t0
-    lib.ostp_synchronize()
-    t0 = time.perf_counter()
-    conv8 = lib.ostp_multishift_mixed_solve(ip, C.byref(fp), C.byref(ra), out, b, 1e-8, 20000, 1, C.byref(cgm),
-                                            cgr, trs_ms)
-    lib.ostp_synchronize()
-    t1 = time.perf_counter()
+    f64_8, _, _ = timed(lambda: lib.ostp_multishift_solve(u, C.byref(fp), C.byref(ra), out, b, 1e-8, *d64, sf, 20000,
+                                                          0, C.byref(cg), sres))
+    tm, _, conv8 = timed(lambda: lib.ostp_multishift_mixed_solve(ip, C.byref(fp), C.byref(ra), out, b, 1e-8, 20000, 1,
+                                                                 C.byref(cgm), cgr, trs_ms))
+    t0, t1 = 0.0, tm
     res["multishift_at_1e-8"] = {"fp64_seconds": round(f64_8, 4), "fp64_iterations": cg.value,
                                  "mixed_ru_seconds": round(t1 - t0, 4), "mixed_ru_cgm_iterations": cgm.value,
                                  "mixed_ru_refine_iterations": [cgr[k] for k in range(order)],
@@ -381,24 +383,20 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     fp4 = ost.ferm_param(m4, ph, phf)
     ip.u, ip.u_f = u4, u4f
     tr = C.c_double(0)
-    lib.ostp_memset(x, 0, vb)
-    lib.ostp_synchronize()
-    t0 = time.perf_counter()
-    st = lib.ostp_cg_solve(u4, C.byref(fp4), x, b, 1e-10, *d64, 40000, 0.0, 10000, C.byref(cg), C.byref(tr))
-    lib.ostp_synchronize()
-    t1 = time.perf_counter()
-    res["cg_fp64"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "true_residual": tr.value,
+    zero_x = lambda: lib.ostp_memset(x, 0, vb)
+    tm, tall, st = timed(lambda: lib.ostp_cg_solve(u4, C.byref(fp4), x, b, 1e-10, *d64, 40000, 0.0, 10000,
+                                                   C.byref(cg), C.byref(tr)), zero_x)
+    t0, t1 = 0.0, tm
+    res["cg_fp64"] = {"seconds": round(t1 - t0, 4), "seconds_runs": tall, "iterations": cg.value,
+                      "true_residual": tr.value,
                       "status": st, "mass": m4, "field": "eps=0.7 seed 2, 2 stout steps rho=0.15"}
     lib.assign_in_to_out(x, x64)
     ru = C.c_int(0)
-    lib.ostp_memset(x, 0, vb)
-    lib.ostp_synchronize()
-    t0 = time.perf_counter()
-    st = lib.ostp_mixed_solve(ip, C.byref(fp4), x, b, 1e-10, 40000, 0.0, 0.1, C.byref(cg), C.byref(ru),
-                              C.byref(tr))
-    lib.ostp_synchronize()
-    t1 = time.perf_counter()
-    res["cg_mixed"] = {"seconds": round(t1 - t0, 4), "iterations": cg.value, "reliable_updates": ru.value,
+    tm, tall, st = timed(lambda: lib.ostp_mixed_solve(ip, C.byref(fp4), x, b, 1e-10, 40000, 0.0, 0.1, C.byref(cg),
+                                                      C.byref(ru), C.byref(tr)), zero_x)
+    t0, t1 = 0.0, tm
+    res["cg_mixed"] = {"seconds": round(t1 - t0, 4), "seconds_runs": tall, "iterations": cg.value,
+                       "reliable_updates": ru.value,
                        "true_residual": tr.value, "status": st, "err_vs_fp64": rel_err(x, x64)}
     res["cg_speedup_mixed_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
 
@@ -413,25 +411,21 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
     cgs, rus, trs = (C.c_int * 1)(), (C.c_int * 1)(), (C.c_double * 1)()
     lib.ostp_memset(x, 0, vb)
     lib.ostp_batched_mixed_solve(1, arr, 1e-10, 40000, 0.1, args.iters_per_graph, cgs, rus, trs)  # warm-up: autotune, graph
-    lib.ostp_memset(x, 0, vb)
-    lib.ostp_synchronize()
-    t0 = time.perf_counter()
-    conv = lib.ostp_batched_mixed_solve(1, arr, 1e-10, 40000, 0.1, args.iters_per_graph, cgs, rus, trs)
-    lib.ostp_synchronize()
-    t1 = time.perf_counter()
-    res["cg_mixed_graph"] = {"seconds": round(t1 - t0, 4), "iterations": cgs[0], "reliable_updates": rus[0],
+    tm, tall, conv = timed(lambda: lib.ostp_batched_mixed_solve(1, arr, 1e-10, 40000, 0.1, args.iters_per_graph, cgs,
+                                                                rus, trs), zero_x)
+    t0, t1 = 0.0, tm
+    res["cg_mixed_graph"] = {"seconds": round(t1 - t0, 4), "seconds_runs": tall, "iterations": cgs[0],
+                             "reliable_updates": rus[0],
                              "true_residual": trs[0], "status": int(conv == 1), "err_vs_fp64": rel_err(x, x64)}
     res["cg_speedup_mixed_graph_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
     # FP16 storage in the inner solver (same graph-captured driver)
     lib.ostp_memset(x, 0, vb)
     lib.ostp_batched_half_solve(1, arr, 1e-10, 40000, 0.1, args.iters_per_graph, cgs, rus, trs)  # warm-up
-    lib.ostp_memset(x, 0, vb)
-    lib.ostp_synchronize()
-    t0 = time.perf_counter()
-    conv = lib.ostp_batched_half_solve(1, arr, 1e-10, 40000, 0.1, args.iters_per_graph, cgs, rus, trs)
-    lib.ostp_synchronize()
-    t1 = time.perf_counter()
-    res["cg_half_graph"] = {"seconds": round(t1 - t0, 4), "iterations": cgs[0], "reliable_updates": rus[0],
+    tm, tall, conv = timed(lambda: lib.ostp_batched_half_solve(1, arr, 1e-10, 40000, 0.1, args.iters_per_graph, cgs,
+                                                               rus, trs), zero_x)
+    t0, t1 = 0.0, tm
+    res["cg_half_graph"] = {"seconds": round(t1 - t0, 4), "seconds_runs": tall, "iterations": cgs[0],
+                            "reliable_updates": rus[0],
                             "true_residual": trs[0], "status": int(conv == 1), "err_vs_fp64": rel_err(x, x64)}
     res["cg_speedup_half_graph_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t1 - t0), 3)
 
@@ -448,13 +442,10 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
         pairs.free()
         lib.ostp_memset(x, 0, vb)
         learned.solve(op, params, x, b, 1e-10)  # warm-up (scratch allocation)
-        lib.ostp_memset(x, 0, vb)
-        lib.ostp_synchronize()
-        t3 = time.perf_counter()
-        st, its, fb, trr = learned.solve(op, params, x, b, 1e-10)
-        lib.ostp_synchronize()
-        t4 = time.perf_counter()
-        res["cg_learned_fcg"] = {"seconds": round(t4 - t3, 4), "iterations": its, "fallback_iterations": fb,
+        tm, tall, (st, its, fb, trr) = timed(lambda: learned.solve(op, params, x, b, 1e-10), zero_x)
+        t3, t4 = 0.0, tm
+        res["cg_learned_fcg"] = {"seconds": round(t4 - t3, 4), "seconds_runs": tall, "iterations": its,
+                                 "fallback_iterations": fb,
                                  "true_residual": trr, "status": st, "err_vs_fp64": rel_err(x, x64),
                                  "training": {"pairs": args.learned_pairs, "pair_seconds": round(t1 - t0, 2),
                                               "steps": args.learned_steps, "train_seconds": round(t2 - t1, 2),
@@ -466,13 +457,10 @@ def run_solves(ost, lib, L, u, ph, b, uf, phf, args):
         # the same network inside the FP32 flexible CG, corrected in FP64
         lib.ostp_memset(x, 0, vb)
         learned.solve_mixed(op, params, x, b, 1e-10)  # warm-up (scratch allocation)
-        lib.ostp_memset(x, 0, vb)
-        lib.ostp_synchronize()
-        t3 = time.perf_counter()
-        st, its, outer, fb, trr = learned.solve_mixed(op, params, x, b, 1e-10)
-        lib.ostp_synchronize()
-        t4 = time.perf_counter()
-        res["cg_learned_fcg_fp32"] = {"seconds": round(t4 - t3, 4), "iterations": its, "reliable_updates": outer,
+        tm, tall, (st, its, outer, fb, trr) = timed(lambda: learned.solve_mixed(op, params, x, b, 1e-10), zero_x)
+        t3, t4 = 0.0, tm
+        res["cg_learned_fcg_fp32"] = {"seconds": round(t4 - t3, 4), "seconds_runs": tall, "iterations": its,
+                                      "reliable_updates": outer,
                                       "update_every": 0.1, "fallback_iterations": fb, "true_residual": trr,
                                       "status": st, "err_vs_fp64": rel_err(x, x64)}
         res["cg_speedup_learned_fp32_vs_fp64"] = round(res["cg_fp64"]["seconds"] / (t4 - t3), 3)
