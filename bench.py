@@ -753,7 +753,8 @@ def run_rhmc(ost, lib, args):
     out = {
         "workload": "RHMC trajectory %s, TLSM beta=3.8, 2 stout rho=0.15, Nf=2+1 m_l=0.0015 m_s=0.04, order 19, "
                     "%d Omelyan steps x 5 gauge sub-steps, tau=%g, FP64 multishift (force 1e-8, "
-                    "accept 1e-10)" % ("x".join(map(str, shape)), args.rhmc_steps, args.rhmc_tau),
+                    "accept 1e-10; the two flavours' solves and power iterations share their hops)"
+                    % ("x".join(map(str, shape)), args.rhmc_steps, args.rhmc_tau),
         "s_per_trajectory": round(t1 - t0, 3),
         "accepted": int(acc),
         "delta_S": info.delta_S,
