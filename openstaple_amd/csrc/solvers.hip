@@ -1087,12 +1087,13 @@ struct MsMultiHop {
     SiteGeom g;
 };
 
-template <int EO>
+template <int EO, int NR>
 __global__ __launch_bounds__(kBlock) void k_ms_multi_hop(MsMultiHop a) {
-    bool act[kMsMulti];
+    static_assert(NR <= kMsMulti, "systems per launch");
+    bool act[NR];
     int first = -1;
 #pragma unroll
-    for (int r = 0; r < kMsMulti; r++) {
+    for (int r = 0; r < NR; r++) {
         SolverCtl c;
         c.st = a.st[r < a.n ? r : 0];
         c.partials = a.part[r < a.n ? r : 0];
@@ -1111,29 +1112,29 @@ __global__ __launch_bounds__(kBlock) void k_ms_multi_hop(MsMultiHop a) {
     } else {
         idx = blk * kBlock + (int)threadIdx.x;
     }
-    double dot[kMsMulti];
+    double dot[NR];
 #pragma unroll
-    for (int r = 0; r < kMsMulti; r++) dot[r] = 0.0;
+    for (int r = 0; r < NR; r++) dot[r] = 0.0;
     if (idx < a.g.sizeh) {
-        const double2* in[kMsMulti];
+        const double2* in[NR];
 #pragma unroll
-        for (int r = 0; r < kMsMulti; r++) in[r] = a.in[act[r] ? r : first];
-        double2 o[kMsMulti][3];
+        for (int r = 0; r < NR; r++) in[r] = a.in[act[r] ? r : first];
+        double2 o[NR][3];
 #pragma unroll
-        for (int r = 0; r < kMsMulti; r++) o[r][0] = o[r][1] = o[r][2] = make_double2(0.0, 0.0);
+        for (int r = 0; r < NR; r++) o[r][0] = o[r][1] = o[r][2] = make_double2(0.0, 0.0);
         const Nbrs nb = site_neighbours<EO>(a.g, idx);
         constexpr int F = EO == 0 ? 0 : 1, B = 1 - F;
 #pragma unroll
         for (int mu = 0; mu < 4; mu++)
-            hop_mr<double, true, kMsMulti>(a.u[2 * mu + B], a.ph[2 * mu + B], in, act, a.g.sizeh, nb.im[mu],
+            hop_mr<double, true, NR>(a.u[2 * mu + B], a.ph[2 * mu + B], in, act, a.g.sizeh, nb.im[mu],
                                            nb.im[mu], o);
 #pragma unroll
         for (int mu = 0; mu < 4; mu++)
-            hop_mr<double, false, kMsMulti>(a.u[2 * mu + F], a.ph[2 * mu + F], in, act, a.g.sizeh, idx, nb.ip[mu],
+            hop_mr<double, false, NR>(a.u[2 * mu + F], a.ph[2 * mu + F], in, act, a.g.sizeh, idx, nb.ip[mu],
                                             o);
         const long sh = a.g.sizeh;
 #pragma unroll
-        for (int r = 0; r < kMsMulti; r++) {
+        for (int r = 0; r < NR; r++) {
             if (!act[r]) continue;
             double2 o0 = o[r][0], o1 = o[r][1], o2 = o[r][2];
             const double half = 0.5;
@@ -1159,10 +1160,10 @@ __global__ __launch_bounds__(kBlock) void k_ms_multi_hop(MsMultiHop a) {
     if constexpr (EO == 0) {
         // block_partial_at's sums, one LDS row per system (no wave may overwrite
         // a wave sum of another system before thread 0 has read it)
-        __shared__ double lds[kMsMulti][kBlock / 64];
+        __shared__ double lds[NR][kBlock / 64];
         const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-        for (int r = 0; r < kMsMulti; r++) {
+        for (int r = 0; r < NR; r++) {
             double v = dot[r];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -1171,7 +1172,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_multi_hop(MsMultiHop a) {
         __syncthreads();
         if (threadIdx.x == 0) {
 #pragma unroll
-            for (int r = 0; r < kMsMulti; r++) {
+            for (int r = 0; r < NR; r++) {
                 if (!act[r]) continue;
                 double t = 0.0;
 #pragma unroll
@@ -1180,6 +1181,15 @@ __global__ __launch_bounds__(kBlock) void k_ms_multi_hop(MsMultiHop a) {
             }
         }
     }
+}
+
+// two systems (the common case: two flavours) with the two-system form
+template <int EO>
+static void launch_multi_hop(const MsMultiHop& a, int nbh) {
+    if (a.n <= 2)
+        hipLaunchKernelGGL((k_ms_multi_hop<EO, 2>), dim3(nbh), dim3(kBlock), 0, S(), a);
+    else
+        hipLaunchKernelGGL((k_ms_multi_hop<EO, kMsMulti>), dim3(nbh), dim3(kBlock), 0, S(), a);
 }
 
 struct MsSys {
@@ -1325,7 +1335,7 @@ static bool multishift_solve_multi(const void* u, const void* ph, MsSys* sy, int
             a.x[k] = nullptr;
         }
         a.rev = ctx().dslash_serpentine ? (int)(ctx().dslash_launches++ & 1u) : 0;
-        hipLaunchKernelGGL(k_ms_multi_hop<1>, dim3(nbh), dim3(kBlock), 0, S(), a);
+        launch_multi_hop<1>(a, nbh);
         for (int k = 0; k < kMsMulti; k++) {
             const int q = k < n ? k : 0;
             a.in[k] = (const double2*)sy[q].h;
@@ -1333,7 +1343,7 @@ static bool multishift_solve_multi(const void* u, const void* ph, MsSys* sy, int
             a.x[k] = (const double2*)sy[q].p;
         }
         a.rev = ctx().dslash_serpentine ? (int)(ctx().dslash_launches++ & 1u) : 0;
-        hipLaunchKernelGGL(k_ms_multi_hop<0>, dim3(nbh), dim3(kBlock), 0, S(), a);
+        launch_multi_hop<0>(a, nbh);
         for (int k = 0; k < n; k++) {
             if (!launch_finalize(kFinMsAlpha, ctl_k(k, kIfRunning), nbh)) return false;
             const double2* r_old = j == 1 ? (const double2*)sy[k].r : slot(k, j - 1);
@@ -1575,7 +1585,7 @@ static bool power_method_multi(const void* u, const void* ph, const double* coef
             a.x[k] = nullptr;
         }
         a.rev = ctx().dslash_serpentine ? (int)(ctx().dslash_launches++ & 1u) : 0;
-        hipLaunchKernelGGL(k_ms_multi_hop<1>, dim3(nbh), dim3(kBlock), 0, S(), a);
+        launch_multi_hop<1>(a, nbh);
         for (int k = 0; k < kMsMulti; k++) {
             const int q = k < n ? k : 0;
             a.in[k] = (const double2*)h[q];
@@ -1583,7 +1593,7 @@ static bool power_method_multi(const void* u, const void* ph, const double* coef
             a.x[k] = (const double2*)r[q];
         }
         a.rev = ctx().dslash_serpentine ? (int)(ctx().dslash_launches++ & 1u) : 0;
-        hipLaunchKernelGGL(k_ms_multi_hop<0>, dim3(nbh), dim3(kBlock), 0, S(), a);
+        launch_multi_hop<0>(a, nbh);
         for (int k = 0; k < n; k++)
             if (!launch_norm2<double2, kFinPower>(p[k], ctl_k(k, kIfRunning))) return false;
         return check_launch("multi power method");
