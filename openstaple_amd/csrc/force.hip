@@ -378,6 +378,7 @@ void fermion_force_soloopenacc(su3_soa* tconf_acc, su3_soa* tstout_conf_acc_arr,
     // when they share the operator (same phases; FP64 solves), then the forces
     bool together = false;
     {
+        const unsigned long e0 = error_count();
         std::vector<ferm_param*> fps;
         std::vector<const RationalApprox*> ras;
         std::vector<const void*> ins;
@@ -398,7 +399,7 @@ void fermion_force_soloopenacc(su3_soa* tconf_acc, su3_soa* tstout_conf_acc_arr,
         char* xs_all = n >= 2 ? (char*)g_xs_all.get((size_t)n * maxo * L.vec3_bytes()) : nullptr;
         if (n >= 2 && !xs_all) {
             (void)hipGetLastError();
-            ostp_clear_error();
+            forgive_errors(e0);
         }
         if (xs_all) {
             for (int k = 0; k < n; k++) {
@@ -407,7 +408,7 @@ void fermion_force_soloopenacc(su3_soa* tconf_acc, su3_soa* tstout_conf_acc_arr,
             }
             together = multishift_fp64_many(conf_to_use, fps.data(), ras.data(), ins.data(), outs.data(), n, res,
                                             max_cg, CONVERGENCE_NONCRITICAL);
-            if (!together && ostp_last_error()[0]) return;
+            if (!together && error_count() != e0) return;
         }
         if (together) {
             int k = 0;

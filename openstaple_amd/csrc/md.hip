@@ -306,11 +306,11 @@ void multistep_2MN_SOLOOPENACC(tamat_soa* tipdot_acc, su3_soa* tconf_acc, su3_so
     nMdInversionPerformed = 0;
 
     // an error recorded inside the force (non-aborting mode) ends the trajectory
-    const bool had_error = ostp_last_error()[0] != 0;
+    const unsigned long e0 = error_count();
     auto fermion_step = [&](int idx) {
         fermion_force_soloopenacc(tconf_acc, tstout_conf_acc_arr, tauxbis_conf_acc, tipdot_acc, tfermions_parameters,
                                   tNDiffFlavs, ferm_in_acc, res, taux_conf_acc, tferm_shiftmulti_acc, ip, max_cg);
-        if (!had_error && ostp_last_error()[0]) return false;
+        if (error_count() != e0) return false;
         return launch_mom_sum_mult(dm, di, deltas_Omelyan[idx]);
     };
     auto gauge = [&]() { return md_gauge(du, dt, di, dm); };

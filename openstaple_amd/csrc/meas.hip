@@ -200,6 +200,7 @@ int ostp_fermion_measures(const su3_soa* u, const su3_soa_f* u_f, ferm_param* pa
                           int* cg_return) {
     const char* nm = "ostp_fermion_measures";
     if (!require_init(nm)) return 0;
+    const unsigned long e0 = error_count();
     if (nvec < 1 || !pars || !out || n_chiral_susc < 0 || n_qn_susc < 0) {
         fail("%s: invalid arguments", nm);
         return 0;
@@ -244,7 +245,7 @@ int ostp_fermion_measures(const su3_soa* u, const su3_soa_f* u_f, ferm_param* pa
             acc_Doe(u, V(k, kPhi), V(k, oe), pars->phases);
             combine_in1_minus_in2_allxfact(V(k, io), V(k, kPhi), 1.0 / m, V(k, oo));
         }
-        return ostp_last_error()[0] == 0;
+        return error_count() == e0;
     };
     // <rnd, (a_e, a_o)> times sign and factor into out[12 k + 2 q]
     auto dot = [&](int k, int q, int ae, int ao, double sign) {

@@ -67,6 +67,13 @@ Context& ctx();
 // Returns false so callers can `return fail(...)`.
 bool fail(const char* fmt, ...);
 bool hip_ok(hipError_t e, const char* what);
+// Failures recorded so far.  Code that calls other entry points compares two
+// readings to learn whether those calls failed, whatever message an earlier
+// call left recorded; a fallback that recovers from a failure it caused
+// (an allocation it can do without) takes it back with forgive_errors(mark),
+// which also clears the message.
+unsigned long error_count();
+void forgive_errors(unsigned long mark);
 bool require_init(const char* what);
 
 // Translate a host address registered with ostp_acc_* (or a device address)

@@ -171,8 +171,8 @@ int UPDATE_SOLOACC_UNOSTEP_VERSATILE(su3_soa* tconf_acc, double res_metro, doubl
     void* dbkp = metro ? resolve(conf_acc_bkp, gb, nm) : nullptr;
     if (!du || (metro && !dbkp)) return acc;
     hipStream_t s = ctx().stream;
-    const bool had_error = ostp_last_error()[0] != 0;
-    auto failed = [&]() { return !had_error && ostp_last_error()[0] != 0; };
+    const unsigned long e0 = error_count();
+    auto failed = [&]() { return error_count() != e0; };
 
     // 1. backup; 2. momenta and pseudofermions
     if (metro && !hip_ok(hipMemcpyAsync(dbkp, du, gb, hipMemcpyDeviceToDevice, s), nm)) return acc;
@@ -216,9 +216,11 @@ int UPDATE_SOLOACC_UNOSTEP_VERSATILE(su3_soa* tconf_acc, double res_metro, doubl
                 if (!bounds(f)) return false;
             return true;
         }
+        const unsigned long mark = error_count();
         char* w = (char*)g_pm_many.get((size_t)3 * nflav * vb);
         if (!w) {
-            ostp_clear_error();
+            (void)hipGetLastError();
+            forgive_errors(mark);
             for (int f = 0; f < nflav; f++)
                 if (!bounds(f)) return false;
             return true;
@@ -316,8 +318,12 @@ int UPDATE_SOLOACC_UNOSTEP_VERSATILE(su3_soa* tconf_acc, double res_metro, doubl
             }
             const int n = (int)fps.size();
             const size_t per = (size_t)(max_order > 0 ? max_order : 1) * vb;
+            const unsigned long mark = error_count();
             char* xs_all = n >= 2 ? (char*)g_xs_many.get((size_t)n * per) : nullptr;
-            if (n >= 2 && !xs_all) ostp_clear_error();
+            if (n >= 2 && !xs_all) {
+                (void)hipGetLastError();
+                forgive_errors(mark);
+            }
             if (xs_all) {
                 const void* du_fm = resolve(ip.u, 8 * L.su3_bytes(), nm);
                 if (!du_fm) return false;

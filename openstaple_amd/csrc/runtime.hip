@@ -40,6 +40,13 @@ static std::mutex g_mu;
 
 Context& ctx() { return g_ctx; }
 
+static unsigned long g_error_count = 0;
+unsigned long error_count() { return g_error_count; }
+void forgive_errors(unsigned long mark) {
+    g_error_count = mark;
+    g_last_error.clear();
+}
+
 bool fail(const char* fmt, ...) {
     char buf[1024];
     va_list ap;
@@ -47,6 +54,7 @@ bool fail(const char* fmt, ...) {
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
     g_last_error = buf;
+    g_error_count++;
     if (g_ctx.abort_on_error) {
         fprintf(stderr, "openstaple_amd: FATAL: %s\n", buf);
         fflush(stderr);

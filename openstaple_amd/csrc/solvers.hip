@@ -1662,6 +1662,7 @@ bool multishift_fp64_many(const void* u, ferm_param* const* pars, const Rational
                           const void* const* in, void* const* out, int n, double res, int max_cg,
                           int convergence_importance) {
     if (!g_ms_many || n < 2 || g_ms_precision != 0 || inverter_tricks.singlePInvAccelMultiInv) return false;
+    const unsigned long e0 = error_count();
     const Lattice& L = ctx().lat;
     const size_t vb = L.vec3_bytes();
     const char* nm = "inverter_multishift_wrapper";
@@ -1679,7 +1680,7 @@ bool multishift_fp64_many(const void* u, ferm_param* const* pars, const Rational
     char* w = (char*)grow_buffer(g_many_work, g_many_work_bytes, kMsMulti * per);
     if (!w) {
         (void)hipGetLastError();
-        ostp_clear_error();
+        forgive_errors(e0);
         return false;
     }
     for (int k0 = 0; k0 < n; k0 += kMsMulti) {
@@ -1703,7 +1704,7 @@ bool multishift_fp64_many(const void* u, ferm_param* const* pars, const Rational
             d.max_cg = max_cg;
         }
         if (!multishift_solve_multi(u, ph0, sy, m)) {
-            if (ostp_last_error()[0]) return false;
+            if (error_count() != e0) return false;
             // no window memory for the group: one by one, as the wrapper does
             for (int r = 0; r < m; r++) {
                 MsBufs B;
@@ -1724,7 +1725,7 @@ bool multishift_fp64_many(const void* u, ferm_param* const* pars, const Rational
         }
         for (int r = 0; r < m; r++) convergence_messages(convergence_importance, sy[r].status);
     }
-    return !ostp_last_error()[0];
+    return error_count() == e0;
 }
 
 // ker_find_max_eigenvalue_openacc for n flavours (start vectors p_k, work
@@ -1976,6 +1977,7 @@ int inverter_multishift_wrapper(inverter_package ip, ferm_param* pars, RationalA
     if (g_ms_precision != 0 && (!g_ms_force_only || in_fermion_force())) {
         // the library's reduced-precision rational solve (ostp_set_multishift_precision),
         // with FP32 links, phases and vectors from library buffers where the package has none
+        const unsigned long e0 = error_count();
         const Lattice& L = ctx().lat;
         const int order = approx->approx_order;
         const size_t vbf = L.vec3f_bytes();
@@ -2004,7 +2006,7 @@ int inverter_multishift_wrapper(inverter_package ip, ferm_param* pars, RationalA
             q.out_f = (vec3_soa_f*)(v + 4 * vbf);
             q.ferm_shift_temp_f = (vec3_soa_f*)(v + 5 * vbf);
         }
-        if (ostp_last_error()[0]) return 0;
+        if (error_count() != e0) return 0;
         std::vector<int> cgr(order > 0 ? order : 1, 0);
         std::vector<double> trs(order > 0 ? order : 1, 0.0);
         const int conv = ostp_multishift_mixed_solve(q, &pf, approx, out, in, res, max_cg, g_ms_precision == 2 ? 1 : 0,
